@@ -1,0 +1,106 @@
+/*
+ * pespace_amd C-ABI: MI355X (gfx950) frequency-domain lnL(+grad) hot path.
+ *
+ * The reference (nnrui/pespace) is pure Python with taichi/JAX backends and has
+ * no native FFI; each entry point below replaces one stage of its FD path and
+ * is bound from Python with ctypes (pespace_amd/_lib.py).  All device buffers
+ * passed to the evaluation entry points are caller-owned; the context owns the
+ * constant detector data it uploads and its scratch buffers.  No entry point
+ * synchronises the host inside an evaluation, so evaluations may be captured in
+ * a HIP graph once the context's scratch has been sized by a first call.
+ *
+ * Errors: every function returns a status (PES_OK = 0); pes_last_error()
+ * returns the message of the last failure on the calling thread.
+ */
+#ifndef PESPACE_AMD_H
+#define PESPACE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PES_OK 0
+#define PES_ERR_INVALID 1
+#define PES_ERR_HIP 2
+#define PES_ERR_STATE 3
+
+/* parameter row layout of every batch (reference likelihood namedtuple order,
+ * pespace/inference/likelihood_jax.py:119-127 with waveform.required_params first) */
+#define PES_N_PARAMS 11
+#define PES_P_CHIRP_MASS 0          /* solar masses                  */
+#define PES_P_MASS_RATIO 1          /* m2/m1 in (0, 1]               */
+#define PES_P_CHI_1 2
+#define PES_P_CHI_2 3
+#define PES_P_LUMINOSITY_DISTANCE 4 /* Mpc                           */
+#define PES_P_REFERENCE_PHASE 5
+#define PES_P_INCLINATION 6
+#define PES_P_ECLIPTIC_LONGITUDE 7
+#define PES_P_ECLIPTIC_LATITUDE 8
+#define PES_P_POLARIZATION 9
+#define PES_P_COALESCENCE_TIME 10   /* seconds                       */
+
+/* precision modes of pes_lnl_batch */
+#define PES_PREC_FP64 0
+#define PES_PREC_MIXED 1
+
+typedef struct pes_context pes_context;
+
+const char* pes_last_error(void);
+
+/* Waveform model + interpolation grid.
+ * Replaces the waveform object of interface_ripple.py:27-90 / tiwave
+ * (modes: ids 0:(2,2) 1:(2,1) 2:(3,3) 3:(3,2) 4:(4,4) 5:(4,3)).
+ * n_nodes > 1: amplitude and phase are evaluated on n_nodes log-spaced nodes in
+ *              [f_node_min, f_node_max] and cubic-Hermite interpolated to the bins;
+ * n_nodes == 0: direct evaluation at every bin (the reference's behaviour,
+ *              interface_ripple.py:149-151).
+ * f_ref: reference frequency [Hz]; <= 0 means the amplitude peak. */
+int pes_context_create(pes_context** out, int device, int n_modes, const int* mode_ids, double f_ref,
+                       int n_nodes, double f_node_min, double f_node_max);
+
+/* One detector: orbit, TDI combination, grid, data and PSD (host arrays, copied).
+ * Replaces InterferometerAntenna + TDIChannelData setup (antenna.py:128-202,
+ * tdi.py:585-649, 771-837) and the static data of likelihood_jax.py:132-205.
+ * orbit_kind: 0 heliocentric (orbit.py:155-241), 1 geocentric (orbit.py:67-152,
+ *   TianQin defaults for the rotation and reference source).
+ * tdi_generation: 15 or 20 (tdi.py:1614-1628); orthogonal: 1 A,E,T / 0 X,Y,Z.
+ * f[n_bins]: bin frequencies.  weight[n_bins]: per-bin frequency weight df_k (may
+ * be NULL: then the scalar df is used for every bin, tdi.py:130).
+ * data_re/data_im/psd: [3][n_bins] channel-major; channel_mask[3]: which of the
+ * three outputs enter lnL (likelihood_ti.py:61, interface_bilby.py:70-75). */
+int pes_context_add_detector(pes_context* ctx, int orbit_kind, double arm_length_m, double rotation_initial,
+                             double revolution_initial, int tdi_generation, int orthogonal, int n_bins,
+                             const double* f, const double* weight, double df, const double* data_re,
+                             const double* data_im, const double* psd, const int* channel_mask);
+
+/* Upload detector data; must be called once after the last add_detector. */
+int pes_context_finalize(pes_context* ctx);
+
+void pes_context_destroy(pes_context* ctx);
+
+/* Batched lnL (and gradient) for P parameter rows.
+ * Replaces compute_whittle_likelihood_ti (likelihood_ti.py:17-69) summed over the
+ * detectors (interface_bilby.py:97-116), vmapped over particles
+ * (sampler/base.py:27-35).  d_params: device [P][11]; d_lnl: device [P];
+ * d_grad: device [P][11] or NULL (no gradient).  Invalid rows (q outside (0,1],
+ * |chi| >= 1, dL <= 0, non-finite) give lnL = -inf and a zero gradient.
+ * stream: hipStream_t (NULL = default stream). */
+int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, double* d_lnl, double* d_grad, int precision,
+                  void* stream);
+
+/* Detector response h_c(f) for P rows on detector det: device out [P][3][n_bins][2]
+ * (re, im).  Replaces update_detector_response + tdi_response_numpy
+ * (antenna.py:204-229, 305-313; tdi.py:1764-1793). */
+int pes_response(pes_context* ctx, int det, const double* d_params, int P, double* d_out, void* stream);
+
+/* Waveform container on detector det's bins: device out [P][n_modes][n_bins][5] =
+ * plus (re, im), cross (re, im), tf, with t_c = 0 (interface_ripple.py:42-51). */
+int pes_waveform(pes_context* ctx, int det, const double* d_params, int P, double* d_out, void* stream);
+
+/* Number of bins of detector det (after finalize). */
+int pes_detector_bins(const pes_context* ctx, int det);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

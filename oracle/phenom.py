@@ -1,0 +1,577 @@
+"""Oracle: IMRPhenomD (2,2) and PhenomHM-style higher modes, restated in numpy.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+The reference holds no PhenomD/HM code: its waveforms come from the external
+``tiwave`` (taichi) and ``ripplegw`` (JAX) packages, neither present, and it
+uses them only with IMRPhenomXAS/XHM (pespace/waveform/interface_ripple.py:16-24).
+What the reference *does* fix is the waveform contract:
+
+* per mode {"plus", "cross", "tf"} with t_c = 0 inside the waveform; the
+  response applies exp(-2 pi i f t_c) and t = tf + t_c
+  (interface_ripple.py:42-51, 135-147; antenna.py:435-449);
+* the conventional Fourier sign, h~ = A exp(-i Psi), tf = (1/2pi) dPsi/df
+  (examples/conventions_difference.ipynb cell 1);
+* the 22 harmonic factors h+ = -(1/8)sqrt(5/pi)(1+cos^2 i) h22,
+  hx = i(1/8)sqrt(5/pi)(2 cos i) h22 at phi_ref = 0
+  (conventions_difference.ipynb cell 6), i.e. lalsimulation's azimuth
+  pi/2 - phi_ref;
+* the TaylorF2 phasing coefficients of get_PN_d_phase
+  (pespace/waveform/interface_lalsim.py:353-443), used here as the PN part of
+  PhenomD's inspiral phase (terms through 3.5PN; the 4PN spin-orbit terms
+  phi_8, phi_8l of that function are not part of PhenomD's ansatz).
+
+IMRPhenomD follows Khan et al. 2016 (arXiv:1508.07253) and Husa et al. 2016
+(arXiv:1508.07250): three-region amplitude (PN + rho pseudo-PN; quartic
+intermediate fixed by five conditions; Lorentzian x exponential ringdown) and
+phase (TaylorF2 + sigma pseudo-PN; beta; alpha with an arctan ringdown), C1
+phase matching at Mf = 0.018 and 0.5 fRD, final spin and radiated energy fits
+of Husa et al. (eqs. 3.6, 3.8), the 19 calibrated coefficients of Khan et al.
+table V.  Ringdown and damping frequencies come from the Leaver QNM table
+(``oracle/qnm.py``) instead of lalsimulation's tabulated QNM data.  Time and
+phase alignment: t(f_peak) = 0 at the amplitude peak f_peak (= fmax), and
+Psi(f_ref) = 0.
+
+The higher modes (2,1),(3,3),(3,2),(4,4),(4,3) follow London et al. 2018
+(arXiv:1708.00404)'s construction: each mode is the (2,2) amplitude and
+phase evaluated at a mapped frequency f22(f), scaled by the leading-order PN
+amplitude ratio, with the (l,m) QNM setting where the ringdown lands.  The map
+is f22 = 2f/m below f_i = (m/2)*0.014, f22 = (fRD22/fRDlm) f above fRDlm, and
+the C1 cubic Hermite between, so t_lm(f) is continuous.
+
+Pins (tests/test_oracle_phenom.py): the PN part of dPsi/df equals the
+reference's get_PN_d_phase for non-spinning systems (where its 4PN terms
+vanish), up to its constant time shift; C0/C1 continuity of amplitude and
+phase at every region boundary; Newtonian scaling A ~ f^(-7/6) at low f;
+exact mass scaling h(f; lam M) = lam^2 h(lam f; M), t(f; lam M) = lam t(lam f; M);
+1/dL scaling; (2,2) of the HM model is bitwise PhenomD; odd-m modes vanish
+for q = 1, chi1 = chi2; Psi_lm(f) = (m/2) Psi_22(2f/m) in the inspiral; the
+harmonic factors match closed-form spin-weighted harmonics.  The fit
+coefficients themselves cannot be checked against any executable in this
+container (tiwave, lalsuite and BBHx are absent): parity unpinned for them.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+
+from . import dual as du
+from .detector import MRSUN_SI, MTSUN_SI, PC_SI, PI
+from .qnm import TABLE_AMIN, TABLE_N, TABLE_STEP, interp_table
+
+EULER_GAMMA = 0.5772156649015329
+
+# --- Khan et al. 2016 table V: lambda00, lambda10, (xi) 01 11 21, (xi^2) 02 12 22, (xi^3) 03 13 23
+FITS = {
+    "rho1": [3931.8979897196696, -17395.758706812805, 3132.375545898835, 343965.86092361377, -1.2162565819981997e6,
+             -70698.00600428853, 1.383907177859705e6, -3.9662761890979446e6, -60017.52423652596, 803515.1181825735,
+             -2.091710365941658e6],
+    "rho2": [-40105.47653771657, 112253.0169706701, 23561.696065836168, -3.476180699403351e6, 1.137593670849482e7,
+             754313.1127166454, -1.308476044625268e7, 3.6444584853928134e7, 596226.612472288, -7.4277901143564405e6,
+             1.8928977514040343e7],
+    "rho3": [83208.35471266537, -191237.7264145924, -210916.2454782992, 8.71797508352568e6, -2.6914942420669552e7,
+             -1.9889806527362722e6, 3.0888029960154563e7, -8.390870279256162e7, -1.4535031953446497e6,
+             1.7063528990822166e7, -4.2748659731120914e7],
+    "v2": [0.8149838730507785, 2.5747553517454658, 1.1610198035496786, -2.3627771785551537, 6.771038707057573,
+           0.7570782938606834, -2.7256896890432474, 7.1140380397149965, 0.1766934149293479, -0.7978690983168183,
+           2.1162391502005153],
+    "gamma1": [0.006927402739328343, 0.03020474290328911, 0.006308024337706171, -0.12074130661131138,
+               0.26271598905781324, 0.0034151773647198794, -0.10779338611188374, 0.27098966966891747,
+               0.0007374185938559283, -0.02749621038376281, 0.0733150789135702],
+    "gamma2": [1.010344404799477, 0.0008993122007234548, 0.283949116804459, -4.049752962958005, 13.207828172665366,
+               0.10396278486805426, -7.025059158961947, 24.784892370130475, 0.03093202475605892, -2.6924023896851663,
+               9.609374464684983],
+    "gamma3": [1.3081615607036106, -0.005537729694807678, -0.06782917938621007, -0.6689834970767117,
+               3.403147966134083, -0.05296577374411866, -0.9923793203111362, 4.820681208409587,
+               -0.006134139870393713, -0.38429253308696365, 1.7561754421985984],
+    "sigma1": [2096.551999295543, 1463.7493168261553, 1312.5493286098522, 18307.330017082117, -43534.1440746107,
+               -833.2889543511114, 32047.31997183187, -108609.45037520859, 452.25136398112204, 8353.439546391714,
+               -44531.3250037322],
+    "sigma2": [-10114.056472621156, -44631.01109458185, -6541.308761668722, -266959.23419307504, 686328.3229317984,
+               3405.6372187679685, -437507.7208209015, 1.6318171307344697e6, -7462.648563007646, -114585.25177153319,
+               674402.4689098676],
+    "sigma3": [22933.658273436497, 230960.00814979506, 14961.083974183695, 1.1940181342318142e6,
+               -3.1042239693052764e6, -3038.166617199259, 1.8720322849093592e6, -7.309145012085539e6,
+               42738.22871475411, 467502.018616601, -3.064853498512499e6],
+    "sigma4": [-14621.71522218357, -377812.8579387104, -9608.682631509726, -1.7108925257214056e6,
+               4.332924601416521e6, -22366.683262266528, -2.5019716386377467e6, 1.0274495902259542e7,
+               -85360.30079034246, -570025.3441737515, 4.396844346849777e6],
+    "beta1": [97.89747327985583, -42.659730877489224, 153.48421037904913, -1417.0620760768954, 2752.8614143665027,
+              138.7406469558649, -1433.6585075135881, 2857.7418952430758, 41.025109467376126, -423.680737974639,
+              850.3594335657173],
+    "beta2": [-3.282701958759534, -9.051384468245866, -12.415449742258042, 55.4716447709787, -106.05109938966335,
+              -11.953044553690658, 76.80704618365418, -155.33172948098394, -3.4129261592393263, 25.572377569952536,
+              -54.408036707740465],
+    "beta3": [-0.000025156429818799565, 0.000019750256942201327, -0.000018370671469295915, 0.000021886317041311973,
+              0.00008250240316860033, 7.157371250566708e-6, -0.000055780000112270685, 0.00019142082884072178,
+              5.447166261464217e-6, -0.00003220610095021982, 0.00007974016714984341],
+    "alpha1": [43.31514709695348, 638.6332679188081, -32.85768747216059, 2415.8938269370315, -5766.875169379177,
+               -61.85459307173841, 2953.967762459948, -8986.29057591497, -21.571435779762044, 981.2158224673428,
+               -3239.5664895930286],
+    "alpha2": [-0.07020209449091723, -0.16269798450687084, -0.1872514685185499, 1.138313650449945,
+               -2.8334196304430046, -0.17137955686840617, 1.7197549338119527, -4.539717148261272,
+               -0.049983437357548705, 0.6062072055948309, -1.682769616644546],
+    "alpha3": [9.5988072383479, -397.05438595557433, 16.202126189517813, -1574.8286986717037, 3600.3410843831093,
+               27.092429659075467, -1786.482357315139, 5152.919378666511, 11.175710130033895, -577.7999423177481,
+               1808.730762932043],
+    "alpha4": [-0.02989487384493607, 1.4022106448583738, -0.07356049468633846, 0.8337006542278661,
+               0.2240008282397391, -0.055202870001177226, 0.5667186343606578, 0.7186931973380503,
+               -0.015507437354325743, 0.15750322779277187, 0.21076815715176228],
+    "alpha5": [0.9974408278363099, -0.007884449714907203, -0.059046901195591035, 1.3958712396764088,
+               -4.516631601676276, -0.05585343136869692, 1.7516580039343603, -5.990208965347804,
+               -0.017945336522161195, 0.5965097794825992, -2.0608879367971804],
+}
+
+AMP_F_INS = 0.014  # amplitude inspiral -> intermediate
+PHI_F_INS = 0.018  # phase inspiral -> intermediate
+MF_CUT = 0.2  # PhenomD domain end
+HM_MAP_JOIN = 0.014  # 22-frequency where the HM frequency map leaves f22 = 2f/m
+MODES = ((2, 2), (2, 1), (3, 3), (3, 2), (4, 4), (4, 3))
+
+_QNM = None
+
+
+def qnm_table():
+    global _QNM
+    if _QNM is None:
+        path = os.path.join(os.path.dirname(__file__), "..", "pespace_amd", "data", "qnm_table.json")
+        with open(path) as fh:
+            _QNM = json.load(fh)
+        assert _QNM["amin"] == TABLE_AMIN and _QNM["step"] == TABLE_STEP and _QNM["n"] == TABLE_N
+    return _QNM
+
+
+def fit(name, eta, chi):
+    c = FITS[name]
+    xi = chi - 1.0
+    eta2 = eta * eta
+    return (c[0] + c[1] * eta + xi * (c[2] + c[3] * eta + c[4] * eta2)
+            + xi * xi * (c[5] + c[6] * eta + c[7] * eta2) + xi * xi * xi * (c[8] + c[9] * eta + c[10] * eta2))
+
+
+def final_spin(eta, chi1, chi2):
+    """Husa et al. 2016 eq. 3.6 (FinalSpin0815), m1 >= m2, chi1 on m1."""
+    d = np.sqrt(1.0 - 4.0 * eta)
+    m1, m2 = 0.5 * (1.0 + d), 0.5 * (1.0 - d)
+    s = m1 * m1 * chi1 + m2 * m2 * chi2
+    e2, e3 = eta * eta, eta * eta * eta
+    s2, s3 = s * s, s * s * s
+    return eta * (3.4641016151377544 - 4.399247300629289 * eta + 9.397292189321194 * e2
+                  - 13.180949901606242 * e3
+                  + s * ((1.0 / eta - 0.0850917821418767 - 5.837029316602263 * eta)
+                         + (0.1014665242971878 - 2.0967746996832157 * eta) * s
+                         + (-1.3546806617824356 + 4.108962025369336 * eta) * s2
+                         + (-0.8676969352555539 + 2.064046835273906 * eta) * s3))
+
+
+def radiated_energy(eta, chi1, chi2):
+    """Husa et al. 2016 eq. 3.8 (EradRational0815)."""
+    d = np.sqrt(1.0 - 4.0 * eta)
+    m1, m2 = 0.5 * (1.0 + d), 0.5 * (1.0 - d)
+    s = (m1 * m1 * chi1 + m2 * m2 * chi2) / (m1 * m1 + m2 * m2)
+    e2, e3, e4 = eta ** 2, eta ** 3, eta ** 4
+    return ((0.055974469826360077 * eta + 0.5809510763115132 * e2 - 0.9606726679372312 * e3
+             + 3.352411249771192 * e4)
+            * (1.0 + (-0.0030302335878845507 - 2.0066110851351073 * eta + 7.7050567802399215 * e2) * s)
+            / (1.0 + (-0.6714403054720589 - 1.4756929437702908 * eta + 7.304676214885011 * e2) * s))
+
+
+def qnm_freqs(mode, af, erad):
+    """(fRD, fDM) in units of 1/M (total mass) for mode (l, m)."""
+    tab = qnm_table()["modes"][f"{mode[0]}{mode[1]}"]
+    fr = float(interp_table(tab["fring"], af))
+    fd = float(interp_table(tab["fdamp"], af))
+    return fr / (1.0 - erad), fd / (1.0 - erad)
+
+
+def tf2_coeffs(eta, chi1, chi2):
+    """TaylorF2 coefficients of interface_lalsim.py:353-433 (through 3.5PN)."""
+    pi = PI
+    pi_third = pi ** (1 / 3)
+    pi_two_thirds = pi_third * pi_third
+    pi_four_thirds = pi * pi_third
+    pi_five_thirds = pi_four_thirds * pi_third
+    e2, e3 = eta * eta, eta * eta * eta
+    delta = np.sqrt(1.0 - 4.0 * eta)
+    ca = (chi1 - chi2) * 0.5
+    cs = (chi1 + chi2) * 0.5
+    ca2, ca3, cs2, cs3 = ca * ca, ca ** 3, cs * cs, cs ** 3
+    cf = -3.0 / 128.0 / eta
+    p = {}
+    p[0] = 1.0 / pi_five_thirds * cf
+    p[1] = 0.0 / pi_four_thirds * cf
+    p[2] = (37.15 / 7.56 + 55.0 / 9.0 * eta) / pi * cf
+    p[3] = ((-16.0 * pi + 113.0 / 3.0 * delta * ca + (113.0 / 3.0 - 76.0 / 3.0 * eta) * cs) / pi_two_thirds * cf)
+    p[4] = ((152.93365 / 5.08032 + 271.45 / 5.04 * eta + 308.5 / 7.2 * e2 + (-405.0 / 8.0 + 200.0 * eta) * ca2
+             - 405.0 / 4.0 * delta * ca * cs + (-405.0 / 8.0 + 5.0 / 2.0 * eta) * cs2) / pi_third * cf)
+    p["5l"] = ((386.45 / 7.56 - 65.0 / 9.0 * eta) * pi + (-732.985 / 2.268 - 140.0 / 9.0 * eta) * delta * ca
+               + (-732.985 / 2.268 + 2426.0 / 8.1 * eta + 340.0 / 9.0 * e2) * cs) * cf
+    p[6] = (((11583.231236531 / 4.694215680 - 640.0 / 3.0 * pi * pi - 684.8 / 2.1 * EULER_GAMMA
+              + (-15737.765635 / 3.048192 + 225.5 / 1.2 * pi * pi) * eta + 76.055 / 1.728 * e2
+              - 127.825 / 1.296 * e3 - np.log(2.0) * 1369.6 / 2.1)
+             + 2270.0 / 3.0 * pi * delta * ca + (2270.0 / 3.0 - 520.0 * eta) * pi * cs
+             + (755.15 / 1.44 - 822.5 / 1.8 * eta) * delta * cs * ca
+             + (755.15 / 2.88 - 2632.45 / 2.52 * eta - 480.0 * e2) * ca2
+             + (755.15 / 2.88 - 2324.15 / 5.04 * eta + 1255.0 / 9.0 * e2) * cs2) * pi_third * cf)
+    p["6l"] = -684.8 / 6.3 * pi_third * cf
+    p[7] = (((770.96675 / 2.54016 + 378.515 / 1.512 * eta - 740.45 / 7.56 * e2) * pi
+             + (-25150.083775 / 3.048192 + 26804.935 / 6.048 * eta - 198.5 / 4.8 * e2) * delta * ca
+             + (-25150.083775 / 3.048192 + 105666.55595 / 7.62048 * eta - 1042.165 / 3.024 * e2
+                + 534.5 / 3.6 * e3) * cs
+             - 1140.0 * pi * delta * ca * cs + (-570.0 + 2240.0 * eta) * pi * ca2
+             + (-570.0 + 40.0 * eta) * pi * cs2 + (14585.0 / 8.0 - 215.0 / 2.0 * eta) * delta * ca * cs2
+             + (14585.0 / 8.0 - 7270.0 * eta + 80.0 * e2) * ca2 * cs + (1458.5 / 2.4 - 2380.0 * eta) * delta * ca3
+             + (1458.5 / 2.4 - 475.0 / 6.0 * eta + 100.0 / 3.0 * e2) * cs3) * pi_two_thirds * cf)
+    return p
+
+
+def amp_pn_prefactors(eta, chi1, chi2):
+    """PN amplitude coefficients of the PhenomD inspiral (Khan et al. 2016 eq. 29, 3PN with spins)."""
+    pi = PI
+    d = np.sqrt(1.0 - 4.0 * eta)
+    dp1 = d + 1.0
+    c12, c22 = chi1 * chi1, chi2 * chi2
+    e2, e3 = eta * eta, eta ** 3
+    a = {}
+    a["2/3"] = ((-969 + 1804 * eta) * pi ** (2 / 3)) / 672.0
+    a["1"] = ((chi1 * (81 * dp1 - 44 * eta) + chi2 * (81 - 81 * d - 44 * eta)) * pi) / 48.0
+    a["4/3"] = ((-27312085.0 - 10287648 * c22 - 10287648 * c12 * dp1 + 10287648 * c22 * d
+                 + 24 * (-1975055 + 857304 * c12 - 994896 * chi1 * chi2 + 857304 * c22) * eta
+                 + 35371056 * e2) * pi ** (4 / 3)) / 8.128512e6
+    a["5/3"] = (pi ** (5 / 3) * (chi2 * (-285197 * (-1 + d) + 4 * (-91902 + 1579 * d) * eta - 35632 * e2)
+                                 + chi1 * (285197 * dp1 - 4 * (91902 + 1579 * d) * eta - 35632 * e2)
+                                 + 42840 * (-1.0 + 4 * eta) * pi)) / 32256.0
+    a["2"] = -(pi * pi * (-336 * (-3248849057.0 + 2943675504 * c12 - 3339284256 * chi1 * chi2 + 2943675504 * c22) * e2
+                          - 324322727232 * e3
+                          - 7 * (-177520268561 + 107414046432 * c22 + 107414046432 * c12 * dp1
+                                 - 107414046432 * c22 * d + 11087290368 * (chi1 + chi2 + chi1 * d - chi2 * d) * pi)
+                          + 12 * eta * (-545384828789 - 176491177632 * chi1 * chi2 + 202603761360 * c22
+                                        + 77616 * c12 * (2610335 + 995766 * d) - 77287373856 * c22 * d
+                                        + 5841690624 * (chi1 + chi2) * pi + 21384760320 * pi * pi))) / 6.0085960704e10
+    return a
+
+
+class PhenomD:
+    """Per-binary PhenomD coefficients (all f-independent setup), Mf units."""
+
+    def __init__(self, m1, m2, chi1, chi2):
+        if m2 > m1:
+            m1, m2, chi1, chi2 = m2, m1, chi2, chi1
+        self.m1, self.m2, self.chi1, self.chi2 = m1, m2, chi1, chi2
+        M = m1 + m2
+        self.M = M
+        eta = m1 * m2 / (M * M)
+        eta = min(eta, 0.25)
+        self.eta = eta
+        self.delta = np.sqrt(1.0 - 4.0 * eta)
+        chis, chia = 0.5 * (chi1 + chi2), 0.5 * (chi1 - chi2)
+        self.chis, self.chia = chis, chia
+        self.chipn = chis * (1.0 - 76.0 * eta / 113.0) + self.delta * chia
+        self.af = final_spin(eta, chi1, chi2)
+        self.erad = radiated_energy(eta, chi1, chi2)
+        self.fRD, self.fDM = qnm_freqs((2, 2), self.af, self.erad)
+        chi = self.chipn
+        g = {k: fit(k, eta, chi) for k in FITS}
+        self.g = g
+        self.apn = amp_pn_prefactors(eta, chi1, chi2)
+        self.tf2 = tf2_coeffs(eta, chi1, chi2)
+        # amplitude: peak and intermediate collocation (Khan et al. sec. V.B)
+        g2, g3 = g["gamma2"], g["gamma3"]
+        if g2 <= 1.0:
+            self.fmax = abs(self.fRD + self.fDM * (-1.0 + np.sqrt(1.0 - g2 * g2)) * g3 / g2)
+        else:
+            self.fmax = abs(self.fRD + self.fDM * (-1.0) * g3 / g2)
+        f1, f3 = AMP_F_INS, self.fmax
+        f2 = 0.5 * (f1 + f3)
+        ins = self.amp_ins(du.D(f1, 1.0))
+        mrd = self.amp_mrd(du.D(f3, 1.0))
+        v1, d1 = float(ins.v), float(ins.d)
+        v3, d3 = float(mrd.v), float(mrd.d)
+        v2 = g["v2"]
+        Amat = np.array([
+            [1.0, f1, f1 ** 2, f1 ** 3, f1 ** 4],
+            [0.0, 1.0, 2 * f1, 3 * f1 ** 2, 4 * f1 ** 3],
+            [1.0, f2, f2 ** 2, f2 ** 3, f2 ** 4],
+            [1.0, f3, f3 ** 2, f3 ** 3, f3 ** 4],
+            [0.0, 1.0, 2 * f3, 3 * f3 ** 2, 4 * f3 ** 3],
+        ])
+        self.delta_int = np.linalg.solve(Amat, np.array([v1, d1, v2, v3, d3]))
+        # phase connection (C1 at PHI_F_INS and 0.5 fRD)
+        self.C1Int = self.C2Int = self.C1MRD = self.C2MRD = 0.0
+        fa, fb = PHI_F_INS, 0.5 * self.fRD
+        pi_ = self.phi_ins(du.D(fa, 1.0))
+        pn_ = self.phi_int(du.D(fa, 1.0))
+        self.C2Int = float(pi_.d - pn_.d)
+        self.C1Int = float(pi_.v - pn_.v - self.C2Int * fa)
+        pn2 = self.phi_int(du.D(fb, 1.0))
+        pm2 = self.phi_mrd(du.D(fb, 1.0))
+        self.C2MRD = float(pn2.d - pm2.d)
+        self.C1MRD = float(pn2.v - pm2.v - self.C2MRD * fb)
+        self.t0 = float(self.phase(du.D(self.fmax, 1.0)).d)  # dphi/dMf at the amplitude peak
+
+    # ---- amplitude (reduced ansatz, without the amp0 Mf^-7/6 prefactor) ---------------
+    def amp_ins(self, f):
+        a, g = self.apn, self.g
+        f13 = f ** (1 / 3)
+        return (1.0 + f13 ** 2 * a["2/3"] + f * a["1"] + f13 ** 4 * a["4/3"] + f13 ** 5 * a["5/3"]
+                + f * f * a["2"] + f13 ** 7 * g["rho1"] + f13 ** 8 * g["rho2"] + f * f * f * g["rho3"])
+
+    def amp_mrd(self, f):
+        g = self.g
+        fdg = self.fDM * g["gamma3"]
+        dfr = f - self.fRD
+        return du.exp(-(dfr) * g["gamma2"] / fdg) * (fdg * g["gamma1"]) / (dfr * dfr + fdg * fdg)
+
+    def amp_int(self, f):
+        d = self.delta_int
+        return d[0] + f * (d[1] + f * (d[2] + f * (d[3] + f * d[4])))
+
+    def amplitude(self, f):
+        """Full 22 amplitude (Mf units, no M^2/dL): sqrt(2 eta/3) pi^-1/6 Mf^-7/6 x ansatz."""
+        fv = du.val(f)
+        ins, itm, mrd = self.amp_ins(f), self.amp_int(f), self.amp_mrd(f)
+        red = du.where(fv <= AMP_F_INS, ins, du.where(fv >= self.fmax, mrd, itm))
+        pre = np.sqrt(2.0 * self.eta / 3.0) * PI ** (-1.0 / 6.0)
+        return red * pre * f ** (-7.0 / 6.0)
+
+    # ---- phase -----------------------------------------------------------------------
+    def phi_ins(self, f):
+        p, g, eta = self.tf2, self.g, self.eta
+        f13 = f ** (1 / 3)
+        tf2 = -(p[0] / f13 ** 5 + p[1] / f13 ** 4 + p[2] / f + p[3] / f13 ** 2 + p[4] / f13
+                + p["5l"] * du.log(f) + p[6] * f13 + p["6l"] * f13 * du.log(PI * f) + p[7] * f13 ** 2)
+        return tf2 + (g["sigma1"] * f + 0.75 * g["sigma2"] * f13 ** 4 + 0.6 * g["sigma3"] * f13 ** 5
+                      + 0.5 * g["sigma4"] * f * f) / eta
+
+    def phi_int(self, f):
+        g, eta = self.g, self.eta
+        return ((g["beta1"] * f - g["beta3"] / (3.0 * f * f * f) + g["beta2"] * du.log(f)) / eta
+                + self.C1Int + self.C2Int * f)
+
+    def phi_mrd(self, f):
+        g, eta = self.g, self.eta
+        return ((g["alpha1"] * f - g["alpha2"] / f + (4.0 / 3.0) * g["alpha3"] * f ** 0.75
+                 + g["alpha4"] * du.atan((f - g["alpha5"] * self.fRD) / self.fDM)) / eta
+                + self.C1MRD + self.C2MRD * f)
+
+    def phase(self, f):
+        fv = du.val(f)
+        return du.where(fv < PHI_F_INS, self.phi_ins(f),
+                        du.where(fv >= 0.5 * self.fRD, self.phi_mrd(f), self.phi_int(f)))
+
+
+def _m1m2(chirp_mass, mass_ratio):
+    """bilby convention: mass_ratio = m2/m1 <= 1."""
+    q = mass_ratio
+    m1 = chirp_mass * (1.0 + q) ** 0.2 / q ** 0.6
+    return m1, m1 * q
+
+
+class PhenomHM:
+    """22 = PhenomD; higher modes by the London et al. frequency-map construction."""
+
+    def __init__(self, chirp_mass, mass_ratio, chi1, chi2, f_ref, modes=((2, 2),)):
+        m1, m2 = _m1m2(chirp_mass, mass_ratio)
+        self.pd = PhenomD(m1, m2, chi1, chi2)
+        pd = self.pd
+        self.Msec = pd.M * MTSUN_SI
+        self.amp_scale = pd.M * MRSUN_SI * pd.M * MTSUN_SI  # M^2 (m s); divided by dL later
+        self.modes = tuple(modes)
+        self.Mf_ref = pd.fmax if f_ref <= 0 else f_ref * self.Msec
+        self.phi_ref0 = float(pd.phase(du.D(self.Mf_ref, 0.0)).v)
+        self.qnm = {md: qnm_freqs(md, pd.af, pd.erad) for md in MODES}
+
+    # -- 22 with alignment: Psi22(Mf) = phi_D(Mf) - phi_D(Mf_ref) - t0 (Mf - Mf_ref)
+    def psi22(self, Mf):
+        pd = self.pd
+        return pd.phase(Mf) - self.phi_ref0 - pd.t0 * (Mf - self.Mf_ref)
+
+    def fmap(self, mode, Mf):
+        """f22(f) for mode (l, m): dual in, dual out."""
+        l, m = mode
+        if mode == (2, 2):
+            return Mf
+        fRD22 = self.qnm[(2, 2)][0]
+        fRDlm = self.qnm[mode][0]
+        rho = fRD22 / fRDlm
+        fi = 0.5 * m * HM_MAP_JOIN
+        fr = fRDlm
+        w = fr - fi
+        t = (Mf - fi) / w
+        P0, P1 = HM_MAP_JOIN, fRD22
+        M0, M1 = (2.0 / m) * w, rho * w
+        c2 = 3.0 * (P1 - P0) - 2.0 * M0 - M1
+        c3 = 2.0 * (P0 - P1) + M0 + M1
+        cub = P0 + t * (M0 + t * (c2 + t * c3))
+        fv = du.val(Mf)
+        return du.where(fv < fi, (2.0 / m) * Mf, du.where(fv >= fr, rho * Mf, cub))
+
+    def beta(self, mode, Mf):
+        """Signed leading-order PN amplitude ratio sqrt(2/m)|H_lm|/|H_22| (frozen above f_i)."""
+        l, m = mode
+        if mode == (2, 2):
+            return du.lift(1.0)
+        pd = self.pd
+        fi = 0.5 * m * HM_MAP_JOIN
+        fv = du.val(Mf)
+        Mfe = du.where(fv < fi, Mf, du.D(np.full_like(fv, fi), 0.0))
+        v = (2.0 * PI * Mfe / m) ** (1.0 / 3.0)
+        d, eta = pd.delta, pd.eta
+        if mode == (2, 1):
+            return np.sqrt(2.0) / 3.0 * (d * v - 1.5 * (pd.chia + d * pd.chis) * v * v)
+        if mode == (3, 3):
+            return np.sqrt(2.0 / 3.0) * 0.75 * np.sqrt(15.0 / 14.0) * d * v
+        if mode == (3, 2):
+            return (1.0 / 3.0) * np.sqrt(5.0 / 7.0) * (1.0 - 3.0 * eta) * v * v
+        if mode == (4, 4):
+            return np.sqrt(0.5) * (8.0 / 9.0) * np.sqrt(5.0 / 7.0) * (1.0 - 3.0 * eta) * v * v
+        if mode == (4, 3):
+            return np.sqrt(2.0 / 3.0) * (9.0 / (4.0 * np.sqrt(70.0))) * d * (1.0 - 2.0 * eta) * v * v * v
+        raise ValueError(mode)
+
+    def mode_cut(self, mode):
+        """Mf above which the mode is zero: f22(Mf) = 0.2."""
+        if mode == (2, 2):
+            return MF_CUT
+        return MF_CUT * self.qnm[mode][0] / self.qnm[(2, 2)][0]
+
+    def amp_phase(self, mode, Mf):
+        """Real amplitude (without 1/dL, m s units) and phase of mode at Mf (dual-aware)."""
+        l, m = mode
+        f22 = self.fmap(mode, Mf)
+        A = self.beta(mode, Mf) * self.pd.amplitude(f22) * self.amp_scale
+        P = (0.5 * m) * self.psi22(f22)
+        return A, P
+
+
+# unit complex factor of each mode: conj(arg H_lm) * exp(-i (m-2) pi/8)
+MODE_PHASE = {
+    (2, 2): 1.0 + 0j, (2, 1): -1j, (3, 3): 1j, (3, 2): 1.0 + 0j, (4, 4): -1.0 + 0j, (4, 3): -1j,
+}
+
+
+def mode_factor(mode):
+    l, m = mode
+    return MODE_PHASE[mode] * np.exp(-1j * (m - 2) * PI / 8.0)
+
+
+def sw_harmonic_theta(l, m, theta):
+    """Real theta part y_lm of the s=-2 spin-weighted harmonic, -2Y_lm = y_lm e^{i m phi}.
+
+    Goldberg et al. 1967 general formula:
+    sY_lm = (-1)^m sqrt((l+m)!(l-m)!(2l+1)/(4pi(l+s)!(l-s)!)) sin^{2l}(t/2)
+            sum_r C(l-s,r) C(l+s,r+s-m) (-1)^{l-r-s} cot^{2r+s-m}(t/2).
+    Implemented with cos/sin(t/2) powers to avoid cot singularities.
+    """
+    from math import comb, factorial
+    s = -2
+    pre = (-1) ** m * np.sqrt(factorial(l + m) * factorial(l - m) * (2 * l + 1)
+                             / (4 * PI * factorial(l + s) * factorial(l - s)))
+    c, sn = np.cos(theta / 2.0), np.sin(theta / 2.0)
+    tot = 0.0
+    for r in range(0, l - s + 1):
+        k = r + s - m
+        if k < 0 or k > l + s:
+            continue
+        p = 2 * r + s - m  # power of cot
+        # sin^{2l} cot^p = cos^p sin^{2l-p}
+        tot = tot + comb(l - s, r) * comb(l + s, k) * (-1) ** (l - r - s) * c ** p * sn ** (2 * l - p)
+    return pre * tot
+
+
+def harmonic_factors(mode, iota, phi_ref):
+    """(K+, Kx) for mode (l, m>0) at lalsimulation azimuth pi/2 - phi_ref."""
+    l, m = mode
+    az = 0.5 * PI - phi_ref
+    y = sw_harmonic_theta(l, m, iota)
+    ym = sw_harmonic_theta(l, -m, iota)
+    e = np.exp(-1j * m * az)
+    sgn = (-1) ** l
+    kp = 0.5 * (y + sgn * ym) * e
+    kc = 0.5j * (sgn * ym - y) * e
+    return kp, kc
+
+
+def waveform_direct(params, f, modes=((2, 2),), f_ref=0.0):
+    """Reference-style waveform container at frequencies f (Hz), t_c = 0.
+
+    params: chirp_mass, mass_ratio, chi_1, chi_2, luminosity_distance [Mpc],
+            reference_phase, inclination.
+    Returns {mode: {"plus","cross","tf"}} (or the single dict if one mode).
+    """
+    hm = PhenomHM(params["chirp_mass"], params["mass_ratio"], params["chi_1"], params["chi_2"], f_ref, modes)
+    dist = params["luminosity_distance"] * 1e6 * PC_SI
+    f = np.asarray(f, dtype=np.float64)
+    out = {}
+    for md in modes:
+        Mf = du.D(f * hm.Msec, hm.Msec)  # d/df
+        A, P = hm.amp_phase(md, Mf)
+        keep = f * hm.Msec <= hm.mode_cut(md)
+        kp, kc = harmonic_factors(md, params["inclination"], params["reference_phase"])
+        c = mode_factor(md) / dist
+        h = np.where(keep, A.v * np.exp(-1j * P.v), 0.0)
+        out[f"{md[0]}{md[1]}"] = {"plus": kp * c * h, "cross": kc * c * h,
+                                  "tf": P.d / (2.0 * PI)}
+    if len(modes) == 1:
+        return out[f"{modes[0][0]}{modes[0][1]}"]
+    return out
+
+
+# --- node tables and cubic-Hermite interpolation ----------------------------------------
+def node_grid(fmin, fmax, n_nodes):
+    """Uniform grid in u = ln f: u_j = u0 + j*du, j = 0..n-1."""
+    u0 = np.log(fmin)
+    du_ = (np.log(fmax) - u0) / (n_nodes - 1)
+    return u0, du_
+
+
+def bin_to_interval(f, u0, du_, n_nodes):
+    """Interval index j in [0, n-2] and local coordinate s = (ln f - u0)/du - j."""
+    x = (np.log(np.asarray(f, dtype=np.float64)) - u0) / du_
+    j = np.clip(np.floor(x), 0, n_nodes - 2).astype(np.int32)
+    return j, x - j
+
+
+def node_values(hm, mode, u0, du_, n_nodes):
+    """A, dA/du, Psi, dPsi/du at the nodes (exact derivatives via duals)."""
+    u = u0 + du_ * np.arange(n_nodes)
+    f = np.exp(u)
+    Mf = du.D(f * hm.Msec, f * hm.Msec)  # dMf/du = Mf
+    A, P = hm.amp_phase(mode, Mf)
+    return A.v, A.d, P.v, P.d
+
+
+def hermite_coeffs(v, dv, du_):
+    """Per-interval cubic coefficients c0..c3 in s for value v and slope dv (d/du)."""
+    P0, P1 = v[:-1], v[1:]
+    M0, M1 = dv[:-1] * du_, dv[1:] * du_
+    c2 = 3.0 * (P1 - P0) - 2.0 * M0 - M1
+    c3 = 2.0 * (P0 - P1) + M0 + M1
+    return np.stack([P0, M0, c2, c3], axis=-1)
+
+
+def waveform_interpolated(params, f, modes, f_ref, fmin_nodes, fmax_nodes, n_nodes):
+    """Waveform container from cubic-Hermite interpolation of node tables."""
+    hm = PhenomHM(params["chirp_mass"], params["mass_ratio"], params["chi_1"], params["chi_2"], f_ref, modes)
+    dist = params["luminosity_distance"] * 1e6 * PC_SI
+    f = np.asarray(f, dtype=np.float64)
+    u0, du_ = node_grid(fmin_nodes, fmax_nodes, n_nodes)
+    j, s = bin_to_interval(f, u0, du_, n_nodes)
+    out = {}
+    for md in modes:
+        Av, Ad, Pv, Pd = node_values(hm, md, u0, du_, n_nodes)
+        ca = hermite_coeffs(Av, Ad, du_)[j]
+        cp = hermite_coeffs(Pv, Pd, du_)[j]
+        A = ca[:, 0] + s * (ca[:, 1] + s * (ca[:, 2] + s * ca[:, 3]))
+        P = cp[:, 0] + s * (cp[:, 1] + s * (cp[:, 2] + s * cp[:, 3]))
+        dPds = cp[:, 1] + s * (2.0 * cp[:, 2] + s * (3.0 * cp[:, 3]))
+        tf = dPds / (2.0 * PI * du_ * f)
+        keep = f * hm.Msec <= hm.mode_cut(md)
+        kp, kc = harmonic_factors(md, params["inclination"], params["reference_phase"])
+        c = mode_factor(md) / dist
+        h = np.where(keep, A * np.exp(-1j * P), 0.0)
+        out[f"{md[0]}{md[1]}"] = {"plus": kp * c * h, "cross": kc * c * h, "tf": tf}
+    return out

@@ -1,0 +1,149 @@
+"""Thin host object over one C-ABI context: detectors in, batched lnL(+grad) out.
+
+Device memory and streams come from PyTorch (ROCm); the compute is the HIP
+library.  Parameter rows follow ``PARAM_NAMES``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import PespaceError, c_int_p, check, dptr, lib
+
+PARAM_NAMES = ("chirp_mass", "mass_ratio", "chi_1", "chi_2", "luminosity_distance", "reference_phase",
+               "inclination", "ecliptic_longitude", "ecliptic_latitude", "polarization", "coalescence_time")
+N_PARAMS = len(PARAM_NAMES)
+MODE_IDS = {"22": 0, "21": 1, "33": 2, "32": 3, "44": 4, "43": 5}
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
+def _mode_key(m):
+    if isinstance(m, str):
+        return m
+    return f"{m[0]}{m[1]}"
+
+
+class LikelihoodCore:
+    """One waveform model + interpolation grid + detectors, on one GPU."""
+
+    def __init__(self, modes=("22",), reference_frequency=0.0, n_nodes=1024, f_node_min=None, f_node_max=None,
+                 device=0):
+        if not torch.cuda.is_available():
+            raise PespaceError("pespace_amd's hip backend needs a GPU (torch.cuda.is_available() is False)")
+        self.modes = tuple(_mode_key(m) for m in modes)
+        for m in self.modes:
+            if m not in MODE_IDS:
+                raise ValueError(f"unsupported mode {m}; available {tuple(MODE_IDS)}")
+        self.reference_frequency = float(reference_frequency)
+        self.n_nodes = int(n_nodes)
+        self.f_node_min = f_node_min
+        self.f_node_max = f_node_max
+        self.device = int(device)
+        self._dets = []
+        self._ctx = None
+        self._n_bins = []
+
+    # -- detectors ---------------------------------------------------------------------
+    def add_detector(self, orbit_kind, arm_length, rotation_initial, revolution_initial, tdi_generation, orthogonal,
+                     frequencies, data, psd, channel_mask, df=None, weight=None):
+        """data: complex (3, N); psd: (3, N) (unused channels may hold ones)."""
+        if self._ctx is not None:
+            raise PespaceError("detectors must be added before the first evaluation")
+        f = np.ascontiguousarray(frequencies, dtype=np.float64)
+        data = np.asarray(data, dtype=np.complex128)
+        psd = np.ascontiguousarray(psd, dtype=np.float64)
+        if data.shape != (3, f.size) or psd.shape != (3, f.size):
+            raise ValueError("data and psd must have shape (3, n_bins)")
+        self._dets.append(dict(kind=int(orbit_kind), arm=float(arm_length), rot0=float(rotation_initial),
+                               rev0=float(revolution_initial), gen=int(tdi_generation), orth=int(bool(orthogonal)),
+                               f=f, dre=np.ascontiguousarray(data.real), dim=np.ascontiguousarray(data.imag),
+                               psd=psd, mask=np.asarray(channel_mask, dtype=np.int32),
+                               df=0.0 if df is None else float(df),
+                               weight=None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)))
+        return len(self._dets) - 1
+
+    def _build(self):
+        if self._ctx is not None:
+            return
+        L = lib()
+        if not self._dets:
+            raise PespaceError("no detector added")
+        fmin = min(float(d["f"].min()) for d in self._dets)
+        fmax = max(float(d["f"].max()) for d in self._dets)
+        f_lo = self.f_node_min if self.f_node_min is not None else fmin
+        f_hi = self.f_node_max if self.f_node_max is not None else fmax
+        ids = (ctypes.c_int * len(self.modes))(*[MODE_IDS[m] for m in self.modes])
+        h = ctypes.c_void_p()
+        check(L.pes_context_create(ctypes.byref(h), self.device, len(self.modes), ids, self.reference_frequency,
+                                   self.n_nodes, f_lo, f_hi))
+        try:
+            for d in self._dets:
+                mask = np.ascontiguousarray(d["mask"], dtype=np.int32)
+                w = d["weight"]
+                check(L.pes_context_add_detector(
+                    h, d["kind"], d["arm"], d["rot0"], d["rev0"], d["gen"], d["orth"], d["f"].size, dptr(d["f"]),
+                    dptr(w) if w is not None else None, d["df"], dptr(d["dre"]), dptr(d["dim"]), dptr(d["psd"]),
+                    mask.ctypes.data_as(c_int_p)))
+            check(L.pes_context_finalize(h))
+        except Exception:
+            L.pes_context_destroy(h)
+            raise
+        self._ctx = h
+        self._n_bins = [L.pes_detector_bins(h, i) for i in range(len(self._dets))]
+
+    def close(self):
+        if self._ctx is not None:
+            lib().pes_context_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- evaluation ----------------------------------------------------------------------
+    def _params(self, params):
+        p = torch.as_tensor(params, dtype=torch.float64)
+        if p.dim() == 1:
+            p = p.unsqueeze(0)
+        if p.dim() != 2 or p.shape[1] != N_PARAMS:
+            raise ValueError(f"params must have shape (P, {N_PARAMS})")
+        return p.to(device=f"cuda:{self.device}").contiguous()
+
+    def lnl(self, params, grad=False, precision="fp64", out=None, grad_out=None):
+        self._build()
+        p = self._params(params)
+        P = p.shape[0]
+        dev = p.device
+        lnl = out if out is not None else torch.empty(P, dtype=torch.float64, device=dev)
+        g = None
+        if grad:
+            g = grad_out if grad_out is not None else torch.empty(P, N_PARAMS, dtype=torch.float64, device=dev)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        check(lib().pes_lnl_batch(self._ctx, p.data_ptr(), P, lnl.data_ptr(), g.data_ptr() if g is not None else None,
+                                  PRECISION[precision], stream))
+        return (lnl, g) if grad else lnl
+
+    def response(self, det, params):
+        """Complex (P, 3, N) TDI response of detector det."""
+        self._build()
+        p = self._params(params)
+        n = self._n_bins[det]
+        out = torch.empty(p.shape[0], 3, n, 2, dtype=torch.float64, device=p.device)
+        check(lib().pes_response(self._ctx, det, p.data_ptr(), p.shape[0], out.data_ptr(),
+                                 torch.cuda.current_stream(p.device).cuda_stream))
+        return torch.view_as_complex(out)
+
+    def waveform(self, det, params):
+        """(P, n_modes, N, 5): plus re/im, cross re/im, tf on detector det's bins."""
+        self._build()
+        p = self._params(params)
+        n = self._n_bins[det]
+        out = torch.empty(p.shape[0], len(self.modes), n, 5, dtype=torch.float64, device=p.device)
+        check(lib().pes_waveform(self._ctx, det, p.data_ptr(), p.shape[0], out.data_ptr(),
+                                 torch.cuda.current_stream(p.device).cuda_stream))
+        return out

@@ -1,0 +1,791 @@
+// MI355X (gfx950) kernels and C-ABI for the FD lnL(+grad) hot path.
+//
+// Pipeline per batch of P parameter rows (one launch each, all on one stream):
+//   k_setup     one thread per row: PhenomD/HM coefficients (+ d/dtheta_int) and
+//               the extrinsic constants (polarization basis, harmonic factors)
+//   k_nodes     one thread per (row, mode, node): A, dA/du, Psi, dPsi/du on the
+//               log-spaced interpolation nodes (u = ln f)
+//   k_lnl_*     persistent fused kernel: lane <-> frequency bin, loop over the rows
+//               of a chunk; cubic-Hermite node -> bin interpolation, t_c shift,
+//               orbit, six single links, TDI, residual, FP64 reduction
+//   k_reduce    fixed-order sum of the per-(bin group, row) partials
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pespace_amd.h"
+#include "response.hpp"
+
+using namespace pes;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return fail(PES_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int TILE = 256;      // bins per tile = threads per block
+constexpr int CHUNK = 16;      // parameter rows per work item
+constexpr int TILES_PER_GROUP = 32;
+
+// ---- per-row constant block ------------------------------------------------------------
+enum PConst {
+  PC_TC = 0, PC_INVDL, PC_VALID, PC_DL,
+  PC_P = 4, PC_Q = 7, PC_K = 10,
+  PC_DLP = 13, PC_DLQ = 16, PC_DBP = 19, PC_DBQ = 22, PC_DLK = 25, PC_DBK = 28,
+  PC_MODE = 32,  // per mode: Fp re,im, Fc re,im, cut_hz, dFp/diota re,im, dFc/diota re,im
+  PC_MODE_STRIDE = 9,
+  PC_STRIDE = 32 + 6 * 9 + 2
+};
+enum { MF_FPR = 0, MF_FPI, MF_FCR, MF_FCI, MF_CUT, MF_DFPR, MF_DFPI, MF_DFCR, MF_DFCI };
+
+struct ModeList {
+  int n;
+  int id[6];
+};
+
+__device__ inline bool finite_d(double x) { return isfinite(x); }
+
+// unit complex factor of a mode: conj(phase of leading PN H_lm) * exp(-i (m-2) pi/8)
+__device__ inline Cplx mode_factor(int id) {
+  const double m = (double)kModeM[id];
+  double ph = -(m - 2.0) * PI / 8.0;
+  Cplx u;
+  switch (id) {
+    case 1: u = {0.0, -1.0}; break;   // (2,1): -i
+    case 2: u = {0.0, 1.0}; break;    // (3,3): +i
+    case 4: u = {-1.0, 0.0}; break;   // (4,4): -1
+    case 5: u = {0.0, -1.0}; break;   // (4,3): -i
+    default: u = {1.0, 0.0}; break;   // (2,2), (3,2)
+  }
+  double s, c;
+  sincos(ph, &s, &c);
+  return cmul(u, {c, s});
+}
+
+// ---- k_setup ---------------------------------------------------------------------------
+__global__ void k_setup(const double* __restrict__ params, int P, ModeList modes, double f_ref,
+                        double* __restrict__ setup, double* __restrict__ pconst) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const double* th = params + (size_t)p * PES_N_PARAMS;
+  double Mc = th[0], q = th[1], c1 = th[2], c2 = th[3], dL = th[4], phi = th[5], iota = th[6];
+  double lam = th[7], bet = th[8], psi = th[9], tc = th[10];
+  bool valid = true;
+  for (int i = 0; i < PES_N_PARAMS; ++i) valid = valid && finite_d(th[i]);
+  valid = valid && Mc > 0.0 && q > 0.0 && q <= 1.0 && fabs(c1) < 1.0 && fabs(c2) < 1.0 && dL > 0.0;
+  double* pc = pconst + (size_t)p * PC_STRIDE;
+  if (!valid) {  // harmless stand-in values; the row's lnL is set to -inf by k_reduce
+    Mc = 1.0e6; q = 1.0; c1 = 0.0; c2 = 0.0; dL = 1.0e4; phi = 0.0; iota = 0.0; lam = 0.0; bet = 0.0; psi = 0.0;
+    tc = 0.0;
+  }
+  typedef Dual<double, 4> D4;
+  D4 dMc = lift<D4>(Mc), dq = lift<D4>(q), dc1 = lift<D4>(c1), dc2 = lift<D4>(c2);
+  dMc.d[0] = 1.0; dq.d[1] = 1.0; dc1.d[2] = 1.0; dc2.d[3] = 1.0;
+  Phenom<D4> ph;
+  phenom_setup(ph, dMc, dq, dc1, dc2, f_ref);
+  double* su = setup + (size_t)p * N_SETUP * 5;
+  for (int i = 0; i < N_SETUP; ++i) {
+    su[i * 5 + 0] = ph.c[i].v;
+    for (int k = 0; k < 4; ++k) su[i * 5 + 1 + k] = ph.c[i].d[k];
+  }
+  pc[PC_TC] = tc;
+  pc[PC_DL] = dL;
+  pc[PC_INVDL] = 1.0 / dL;
+  pc[PC_VALID] = valid ? 1.0 : 0.0;
+  double sl, cl, sb, cb, sp, cp;
+  sincos(lam, &sl, &cl);
+  sincos(bet, &sb, &cb);
+  sincos(psi, &sp, &cp);
+  // antenna.py:70-83, 112-114 and their lambda / beta derivatives
+  double pv[3] = {sl * cp - cl * sb * sp, -cl * cp - sl * sb * sp, cb * sp};
+  double qv[3] = {-sl * sp - cl * sb * cp, cl * sp - sl * sb * cp, cb * cp};
+  double kv[3] = {-cb * cl, -cb * sl, -sb};
+  double dlp[3] = {cl * cp + sl * sb * sp, sl * cp - cl * sb * sp, 0.0};
+  double dbp[3] = {-cl * cb * sp, -sl * cb * sp, -sb * sp};
+  double dlq[3] = {-cl * sp + sl * sb * cp, -sl * sp - cl * sb * cp, 0.0};
+  double dbq[3] = {-cl * cb * cp, -sl * cb * cp, -sb * cp};
+  double dlk[3] = {cb * sl, -cb * cl, 0.0};
+  double dbk[3] = {sb * cl, sb * sl, -cb};
+  for (int i = 0; i < 3; ++i) {
+    pc[PC_P + i] = pv[i]; pc[PC_Q + i] = qv[i]; pc[PC_K + i] = kv[i];
+    pc[PC_DLP + i] = dlp[i]; pc[PC_DLQ + i] = dlq[i]; pc[PC_DBP + i] = dbp[i];
+    pc[PC_DBQ + i] = dbq[i]; pc[PC_DLK + i] = dlk[i]; pc[PC_DBK + i] = dbk[i];
+  }
+  double dist = dL * 1.0e6 * PC_SI;
+  double msec = ph.c[SU_MSEC].v;
+  double az = 0.5 * PI - phi;  // lalsimulation azimuth convention
+  typedef Dual<double, 1> D1;
+  D1 di = lift<D1>(iota);
+  di.d[0] = 1.0;
+  for (int mi = 0; mi < modes.n; ++mi) {
+    int id = modes.id[mi];
+    int l = kModeL[id], m = kModeM[id];
+    D1 y = sw_y(l, m, di);
+    D1 ym = sw_y(l, -m, di);
+    double sgn = (l % 2) ? -1.0 : 1.0;
+    double se, ce;
+    sincos(-(double)m * az, &se, &ce);
+    Cplx e = {ce, se};
+    Cplx u = mode_factor(id);
+    Cplx eu = cmul(e, u);
+    double kp = 0.5 * (y.v + sgn * ym.v), dkp = 0.5 * (y.d[0] + sgn * ym.d[0]);
+    double kc = 0.5 * (sgn * ym.v - y.v), dkc = 0.5 * (sgn * ym.d[0] - y.d[0]);  // times i
+    double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
+    Cplx Fp = cscale(eu, kp / dist);
+    Cplx Fc = cmul({0.0, kc / dist}, eu);
+    Cplx dFp = cscale(eu, dkp / dist);
+    Cplx dFc = cmul({0.0, dkc / dist}, eu);
+    mf[MF_FPR] = Fp.re; mf[MF_FPI] = Fp.im; mf[MF_FCR] = Fc.re; mf[MF_FCI] = Fc.im;
+    mf[MF_CUT] = ph.mode_cut(id) / msec;
+    mf[MF_DFPR] = dFp.re; mf[MF_DFPI] = dFp.im; mf[MF_DFCR] = dFc.re; mf[MF_DFCI] = dFc.im;
+  }
+}
+
+// ---- k_nodes ---------------------------------------------------------------------------
+// node table: [P][n_modes][NNS][4] = (A, dA/du, Psi, dPsi/du); NNS = n_nodes + 1 (padding)
+// grad table: [P][n_modes][NNS][16] = d/dtheta_i of the same four, i = Mc, q, chi1, chi2
+template <bool GRAD>
+__global__ void k_nodes(const double* __restrict__ setup, int P, ModeList modes, int n_nodes, double u0, double du,
+                        const double* __restrict__ node_f, double* __restrict__ nodes,
+                        double* __restrict__ nodes_grad) {
+  int jn = blockIdx.x * blockDim.x + threadIdx.x;
+  int pm = blockIdx.y;
+  int p = pm / modes.n, mi = pm % modes.n;
+  const int NNS = n_nodes + 1;
+  if (jn >= NNS || p >= P) return;
+  size_t row = ((size_t)p * modes.n + mi) * NNS + jn;
+  if (jn == n_nodes) {  // padding node (read as the right end of the last direct-mode bin)
+    for (int k = 0; k < 4; ++k) nodes[row * 4 + k] = 0.0;
+    if (GRAD)
+      for (int k = 0; k < 16; ++k) nodes_grad[row * 16 + k] = 0.0;
+    return;
+  }
+  double f = node_f ? node_f[jn] : exp(u0 + du * (double)jn);
+  const double* su = setup + (size_t)p * N_SETUP * 5;
+  int id = modes.id[mi];
+  if (!GRAD) {
+    typedef Dual<double, 1> S;
+    Phenom<S> ph;
+    for (int i = 0; i < N_SETUP; ++i) {
+      ph.c[i].v = su[i * 5];
+      ph.c[i].d[0] = 0.0;
+    }
+    S Mf;
+    Mf.v = f * su[SU_MSEC * 5];
+    Mf.d[0] = Mf.v;  // dMf/du
+    S A, Ps;
+    ph.amp_phase(id, Mf, A, Ps);
+    nodes[row * 4 + 0] = A.v;
+    nodes[row * 4 + 1] = A.d[0];
+    nodes[row * 4 + 2] = Ps.v;
+    nodes[row * 4 + 3] = Ps.d[0];
+  } else {
+    typedef Dual<double, 1> D1;
+    typedef Dual<D1, 4> S;
+    Phenom<S> ph;
+    for (int i = 0; i < N_SETUP; ++i) {
+      ph.c[i].v.v = su[i * 5];
+      ph.c[i].v.d[0] = 0.0;
+      for (int k = 0; k < 4; ++k) {
+        ph.c[i].d[k].v = su[i * 5 + 1 + k];
+        ph.c[i].d[k].d[0] = 0.0;
+      }
+    }
+    S Mf;
+    Mf.v.v = f * su[SU_MSEC * 5];
+    Mf.v.d[0] = Mf.v.v;
+    for (int k = 0; k < 4; ++k) {
+      Mf.d[k].v = f * su[SU_MSEC * 5 + 1 + k];
+      Mf.d[k].d[0] = Mf.d[k].v;
+    }
+    S A, Ps;
+    ph.amp_phase(id, Mf, A, Ps);
+    nodes[row * 4 + 0] = A.v.v;
+    nodes[row * 4 + 1] = A.v.d[0];
+    nodes[row * 4 + 2] = Ps.v.v;
+    nodes[row * 4 + 3] = Ps.v.d[0];
+    for (int k = 0; k < 4; ++k) {
+      nodes_grad[row * 16 + k * 4 + 0] = A.d[k].v;
+      nodes_grad[row * 16 + k * 4 + 1] = A.d[k].d[0];
+      nodes_grad[row * 16 + k * 4 + 2] = Ps.d[k].v;
+      nodes_grad[row * 16 + k * 4 + 3] = Ps.d[k].d[0];
+    }
+  }
+}
+
+// ---- Hermite interpolation between node j and j+1 ---------------------------------------
+struct Herm {
+  double v, ds;  // value and d/ds
+};
+__device__ inline Herm hermite(double P0, double D0, double P1, double D1, double du, double s) {
+  double M0 = D0 * du, M1 = D1 * du;
+  double c2 = 3.0 * (P1 - P0) - 2.0 * M0 - M1;
+  double c3 = 2.0 * (P0 - P1) + M0 + M1;
+  Herm h;
+  h.v = P0 + s * (M0 + s * (c2 + s * c3));
+  h.ds = M0 + s * (2.0 * c2 + s * (3.0 * c3));
+  return h;
+}
+
+struct BinCtx {
+  double f, s, du;
+  int j;
+};
+
+// Waveform of one mode at one bin: amplitude, phase and tf (t_c = 0).
+__device__ inline void mode_at_bin(const double* __restrict__ nodes, size_t base, const BinCtx& b, double& A,
+                                   double& Psi, double& tf) {
+  const double* n0 = nodes + (base + b.j) * 4;
+  Herm ha = hermite(n0[0], n0[1], n0[4], n0[5], b.du, b.s);
+  Herm hp = hermite(n0[2], n0[3], n0[6], n0[7], b.du, b.s);
+  A = ha.v;
+  Psi = hp.v;
+  tf = hp.ds / (2.0 * PI * b.du * b.f);
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+struct BinData {
+  double f, s, w[3];
+  Cplx d[3];
+  int j;
+};
+
+// full FP64 forward response of one row at one bin: the three TDI outputs
+__device__ inline void response_fp64(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
+                                     const double* __restrict__ nodes, size_t pbase, int NNS, const BinCtx& b,
+                                     Cplx out[3]) {
+  double tc = pc[PC_TC];
+  Vec3 pv = {pc[PC_P], pc[PC_P + 1], pc[PC_P + 2]};
+  Vec3 qv = {pc[PC_Q], pc[PC_Q + 1], pc[PC_Q + 2]};
+  Vec3 kv = {pc[PC_K], pc[PC_K + 1], pc[PC_K + 2]};
+  Cplx etc;
+  sincos(-2.0 * PI * b.f * tc, &etc.im, &etc.re);  // antenna.py:435
+  Links L = {};
+  for (int mi = 0; mi < modes.n; ++mi) {
+    const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
+    if (b.f > mf[MF_CUT]) continue;
+    double A, Psi, tf;
+    mode_at_bin(nodes, pbase + (size_t)mi * NNS, b, A, Psi, tf);
+    double sp, cp;
+    sincos(Psi, &sp, &cp);
+    Cplx h = cmul({A * cp, -A * sp}, etc);
+    Cplx hp = cmul({mf[MF_FPR], mf[MF_FPI]}, h);
+    Cplx hc = cmul({mf[MF_FCR], mf[MF_FCI]}, h);
+    add_mode_links(det, b.f, tf + tc, hp, hc, pv, qv, kv, L);
+  }
+  tdi_combine(det, b.f, L, out);
+}
+
+__device__ inline int tile_detector(const DetectorDev* dets, int n_det, int tile) {
+  int acc = 0;
+  for (int d = 0; d < n_det; ++d) {
+    acc += dets[d].n_tiles;
+    if (tile < acc) return d;
+  }
+  return n_det - 1;
+}
+
+// ---- k_lnl_fp64: persistent fused forward kernel ------------------------------------------
+__global__ __launch_bounds__(TILE) void k_lnl_fp64(const DetectorDev* __restrict__ dets, int n_det, int n_tiles,
+                                                   const double* __restrict__ bin_f, const int* __restrict__ bin_j,
+                                                   const double* __restrict__ bin_s, const double* __restrict__ bin_w,
+                                                   const double* __restrict__ dre, const double* __restrict__ dim,
+                                                   int NB, const double* __restrict__ pconst,
+                                                   const double* __restrict__ nodes, int NNS, double du, ModeList modes,
+                                                   int P, double* __restrict__ partial) {
+  __shared__ double acc[TILE / 64][CHUNK];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n_chunks = (P + CHUNK - 1) / CHUNK;
+  const int n_groups = (n_tiles + TILES_PER_GROUP - 1) / TILES_PER_GROUP;
+  const int n_work = n_chunks * n_groups;
+  for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
+    const int g = w / n_chunks, c = w % n_chunks;
+    const int p0 = c * CHUNK, p1 = min(P, p0 + CHUNK);
+    if (threadIdx.x < (TILE / 64) * CHUNK) (&acc[0][0])[threadIdx.x] = 0.0;
+    __syncthreads();
+    const int t_end = min(n_tiles, (g + 1) * TILES_PER_GROUP);
+    for (int tile = g * TILES_PER_GROUP; tile < t_end; ++tile) {
+      const int di = tile_detector(dets, n_det, tile);
+      const DetectorDev det = dets[di];
+      const int bi = tile * TILE + threadIdx.x;
+      BinCtx b;
+      b.f = bin_f[bi];
+      b.s = bin_s[bi];
+      b.j = bin_j[bi];
+      b.du = du;
+      double wc[3];
+      Cplx dc[3];
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        wc[ch] = bin_w[ch * NB + bi];
+        dc[ch] = {dre[ch * NB + bi], dim[ch * NB + bi]};
+      }
+      for (int p = p0; p < p1; ++p) {
+        const double* pc = pconst + (size_t)p * PC_STRIDE;
+        double v = 0.0;
+        if (pc[PC_VALID] != 0.0) {
+          Cplx out[3];
+          response_fp64(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, out);
+#pragma unroll
+          for (int ch = 0; ch < 3; ++ch) {
+            double rr = dc[ch].re - out[ch].re, ri = dc[ch].im - out[ch].im;
+            v -= wc[ch] * (rr * rr + ri * ri);
+          }
+        }
+        v = wave_sum(v);
+        if (lane == 0) acc[wave][p - p0] += v;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < p1 - p0) {
+      double s = 0.0;
+      for (int k = 0; k < TILE / 64; ++k) s += acc[k][threadIdx.x];
+      partial[(size_t)g * P + p0 + threadIdx.x] = s;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_reduce(const double* __restrict__ partial, int n_groups, int P, int nout,
+                         const double* __restrict__ pconst, double* __restrict__ lnl, double* __restrict__ grad) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  bool valid = pconst[(size_t)p * PC_STRIDE + PC_VALID] != 0.0;
+  for (int o = 0; o < nout; ++o) {
+    double s = 0.0;
+    for (int g = 0; g < n_groups; ++g) s += partial[((size_t)g * nout + o) * P + p];
+    if (o == 0) {
+      lnl[p] = valid ? s : -INFINITY;
+    } else if (grad) {
+      grad[(size_t)p * PES_N_PARAMS + (o - 1)] = valid ? s : 0.0;
+    }
+  }
+}
+
+// ---- response / waveform output kernels (parity checks) -------------------------------------
+__global__ void k_response(const DetectorDev* __restrict__ dets, int det_i, const double* __restrict__ bin_f,
+                           const int* __restrict__ bin_j, const double* __restrict__ bin_s,
+                           const double* __restrict__ pconst, const double* __restrict__ nodes, int NNS, double du,
+                           ModeList modes, int P, double* __restrict__ out) {
+  const DetectorDev det = dets[det_i];
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int p = blockIdx.y;
+  if (k >= det.n_bins || p >= P) return;
+  int bi = det.bin_offset + k;
+  BinCtx b;
+  b.f = bin_f[bi];
+  b.s = bin_s[bi];
+  b.j = bin_j[bi];
+  b.du = du;
+  const double* pc = pconst + (size_t)p * PC_STRIDE;
+  Cplx o[3];
+  response_fp64(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, o);
+  for (int ch = 0; ch < 3; ++ch) {
+    size_t idx = (((size_t)p * 3 + ch) * det.n_bins + k) * 2;
+    out[idx] = o[ch].re;
+    out[idx + 1] = o[ch].im;
+  }
+}
+
+__global__ void k_waveform(const DetectorDev* __restrict__ dets, int det_i, const double* __restrict__ bin_f,
+                           const int* __restrict__ bin_j, const double* __restrict__ bin_s,
+                           const double* __restrict__ pconst, const double* __restrict__ nodes, int NNS, double du,
+                           ModeList modes, int P, double* __restrict__ out) {
+  const DetectorDev det = dets[det_i];
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int p = blockIdx.y;
+  if (k >= det.n_bins || p >= P) return;
+  int bi = det.bin_offset + k;
+  BinCtx b;
+  b.f = bin_f[bi];
+  b.s = bin_s[bi];
+  b.j = bin_j[bi];
+  b.du = du;
+  const double* pc = pconst + (size_t)p * PC_STRIDE;
+  for (int mi = 0; mi < modes.n; ++mi) {
+    const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
+    double A, Psi, tf;
+    mode_at_bin(nodes, (size_t)p * modes.n * NNS + (size_t)mi * NNS, b, A, Psi, tf);
+    if (b.f > mf[MF_CUT]) A = 0.0;
+    double sp, cp;
+    sincos(Psi, &sp, &cp);
+    Cplx h = {A * cp, -A * sp};
+    Cplx hp = cmul({mf[MF_FPR], mf[MF_FPI]}, h);
+    Cplx hc = cmul({mf[MF_FCR], mf[MF_FCI]}, h);
+    size_t idx = (((size_t)p * modes.n + mi) * det.n_bins + k) * 5;
+    out[idx] = hp.re;
+    out[idx + 1] = hp.im;
+    out[idx + 2] = hc.re;
+    out[idx + 3] = hc.im;
+    out[idx + 4] = tf;
+  }
+}
+
+// gradient / mixed-precision launchers (defined below the forward kernels)
+int launch_lnl_grad(const DetectorDev*, int, int, const double*, const int*, const double*, const double*,
+                    const double*, const double*, int, const double*, const double*, const double*, int, double,
+                    ModeList, int, double*, int, int, hipStream_t) {
+  return -1;
+}
+int launch_lnl_mixed(const DetectorDev*, int, int, const double*, const int*, const double*, const double*,
+                     const double*, const double*, int, const double*, const double*, int, double, ModeList, int,
+                     double*, int, hipStream_t) {
+  return -1;
+}
+
+}  // namespace
+
+// ================================= host side ================================================
+struct pes_context {
+  int device = 0;
+  ModeList modes{};
+  double f_ref = 0.0;
+  bool direct = false;
+  int n_nodes = 0;
+  double f_node_min = 0.0, f_node_max = 0.0, u0 = 0.0, du = 1.0;
+  bool finalized = false;
+  struct HostDet {
+    DetectorDev d;
+    std::vector<double> f, w, dre, dim;  // w, dre, dim: [3][n]
+  };
+  std::vector<HostDet> hdets;
+  // device constant data
+  DetectorDev* d_dets = nullptr;
+  int n_tiles = 0, NB = 0;
+  double *d_bin_f = nullptr, *d_bin_s = nullptr, *d_bin_w = nullptr, *d_dre = nullptr, *d_dim = nullptr;
+  int* d_bin_j = nullptr;
+  double* d_node_f = nullptr;  // direct mode only
+  // scratch, grown on demand
+  int cap_P = 0;
+  bool cap_grad = false;
+  double *d_setup = nullptr, *d_pconst = nullptr, *d_nodes = nullptr, *d_nodes_grad = nullptr,
+         *d_partial = nullptr;
+  int n_cu = 256;
+};
+
+extern "C" const char* pes_last_error(void) { return g_err.c_str(); }
+
+extern "C" int pes_context_create(pes_context** out, int device, int n_modes, const int* mode_ids, double f_ref,
+                                  int n_nodes, double f_node_min, double f_node_max) {
+  if (!out) return fail(PES_ERR_INVALID, "out is NULL");
+  if (n_modes < 1 || n_modes > 6) return fail(PES_ERR_INVALID, "n_modes must be in [1, 6]");
+  if (n_nodes == 1 || n_nodes < 0) return fail(PES_ERR_INVALID, "n_nodes must be 0 (direct) or >= 2");
+  if (n_nodes >= 2 && !(f_node_min > 0.0 && f_node_max > f_node_min))
+    return fail(PES_ERR_INVALID, "node range must satisfy 0 < f_node_min < f_node_max");
+  pes_context* c = new pes_context();
+  c->device = device;
+  c->modes.n = n_modes;
+  for (int i = 0; i < n_modes; ++i) {
+    if (mode_ids[i] < 0 || mode_ids[i] > 5) {
+      delete c;
+      return fail(PES_ERR_INVALID, "mode id out of range [0, 5]");
+    }
+    for (int k = 0; k < i; ++k)
+      if (mode_ids[k] == mode_ids[i]) {
+        delete c;
+        return fail(PES_ERR_INVALID, "duplicate mode id");
+      }
+    c->modes.id[i] = mode_ids[i];
+  }
+  c->f_ref = f_ref;
+  c->direct = (n_nodes == 0);
+  c->n_nodes = n_nodes;
+  c->f_node_min = f_node_min;
+  c->f_node_max = f_node_max;
+  if (!c->direct) {
+    c->u0 = std::log(f_node_min);
+    c->du = (std::log(f_node_max) - c->u0) / (double)(n_nodes - 1);
+  }
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(PES_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+  *out = c;
+  return PES_OK;
+}
+
+extern "C" int pes_context_add_detector(pes_context* ctx, int orbit_kind, double arm_length_m,
+                                        double rotation_initial, double revolution_initial, int tdi_generation,
+                                        int orthogonal, int n_bins, const double* f, const double* weight, double df,
+                                        const double* data_re, const double* data_im, const double* psd,
+                                        const int* channel_mask) {
+  if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
+  if (ctx->finalized) return fail(PES_ERR_STATE, "context already finalized");
+  if (orbit_kind != 0 && orbit_kind != 1) return fail(PES_ERR_INVALID, "orbit_kind must be 0 or 1");
+  if (tdi_generation != 15 && tdi_generation != 20) return fail(PES_ERR_INVALID, "tdi_generation must be 15 or 20");
+  if (n_bins < 1 || !f || !data_re || !data_im || !psd || !channel_mask)
+    return fail(PES_ERR_INVALID, "detector arrays must be non-NULL and n_bins >= 1");
+  if (!(arm_length_m > 0.0)) return fail(PES_ERR_INVALID, "arm length must be positive");
+  pes_context::HostDet h;
+  DetectorDev& d = h.d;
+  std::memset(&d, 0, sizeof(d));
+  d.kind = orbit_kind;
+  d.tdi_gen = tdi_generation;
+  d.orthogonal = orthogonal ? 1 : 0;
+  d.n_bins = n_bins;
+  d.L = arm_length_m / C_SI;
+  d.rot0 = rotation_initial;
+  d.rev0 = revolution_initial;
+  if (orbit_kind == 0) {  // orbit.py:189-228
+    d.rp = d.L / (2.0 * std::sqrt(3.0));
+    double k1 = rotation_initial, k2 = k1 + 2.0 * PI / 3.0, k3 = k2 + 2.0 * PI / 3.0;
+    d.ck[0] = std::cos(k1); d.sk[0] = std::sin(k1);
+    d.ck[1] = std::cos(k2); d.sk[1] = std::sin(k2);
+    d.ck[2] = std::cos(k3); d.sk[2] = std::sin(k3);
+  } else {  // orbit.py:77-139, TianQin defaults
+    d.rp = d.L / std::sqrt(3.0);
+    double lr = 120.5 / 180.0 * PI, br = -4.7 / 180.0 * PI;
+    d.slam = std::sin(lr); d.clam = std::cos(lr); d.sbet = std::sin(br); d.cbet = std::cos(br);
+    d.omega_rot = 2.0 * PI / (3.65 * DAY_SI);
+  }
+  h.f.assign(f, f + n_bins);
+  h.w.assign(3 * (size_t)n_bins, 0.0);
+  h.dre.assign(data_re, data_re + 3 * (size_t)n_bins);
+  h.dim.assign(data_im, data_im + 3 * (size_t)n_bins);
+  for (int k = 0; k < n_bins; ++k) {
+    if (!(f[k] > 0.0) || !std::isfinite(f[k])) return fail(PES_ERR_INVALID, "frequencies must be positive");
+    double dfk = weight ? weight[k] : df;
+    for (int ch = 0; ch < 3; ++ch) {
+      if (!channel_mask[ch]) continue;
+      double s = psd[(size_t)ch * n_bins + k];
+      if (!(s > 0.0)) return fail(PES_ERR_INVALID, "PSD must be positive on used channels");
+      h.w[(size_t)ch * n_bins + k] = 2.0 * dfk / s;  // likelihood_ti.py:63-67: -2 df |d-h|^2 / S
+    }
+  }
+  ctx->hdets.push_back(std::move(h));
+  return PES_OK;
+}
+
+static void free_dev(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+extern "C" int pes_context_finalize(pes_context* ctx) {
+  if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
+  if (ctx->finalized) return fail(PES_ERR_STATE, "context already finalized");
+  if (ctx->hdets.empty()) return fail(PES_ERR_STATE, "no detector added");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int off = 0;
+  for (auto& h : ctx->hdets) {
+    h.d.bin_offset = off;
+    h.d.n_tiles = (h.d.n_bins + TILE - 1) / TILE;
+    off += h.d.n_tiles * TILE;
+  }
+  ctx->NB = off;
+  ctx->n_tiles = off / TILE;
+  const int NB = ctx->NB;
+  std::vector<double> bf(NB), bs(NB, 0.0), bw(3 * (size_t)NB, 0.0), dre(3 * (size_t)NB, 0.0), dim(3 * (size_t)NB, 0.0);
+  std::vector<int> bj(NB, 0);
+  for (auto& h : ctx->hdets) {
+    const int n = h.d.n_bins, o = h.d.bin_offset;
+    for (int k = 0; k < h.d.n_tiles * TILE; ++k) {
+      int src = k < n ? k : n - 1;  // padding lanes repeat the last bin with zero weight
+      bf[o + k] = h.f[src];
+      for (int ch = 0; ch < 3; ++ch) {
+        if (k < n) {
+          bw[(size_t)ch * NB + o + k] = h.w[(size_t)ch * n + k];
+          dre[(size_t)ch * NB + o + k] = h.dre[(size_t)ch * n + k];
+          dim[(size_t)ch * NB + o + k] = h.dim[(size_t)ch * n + k];
+        }
+      }
+    }
+  }
+  if (ctx->direct) {
+    ctx->n_nodes = NB;
+    ctx->du = 1.0;
+    for (int k = 0; k < NB; ++k) {
+      bj[k] = k;
+      bs[k] = 0.0;
+    }
+  } else {
+    for (int k = 0; k < NB; ++k) {
+      double x = (std::log(bf[k]) - ctx->u0) / ctx->du;
+      double jf = std::floor(x);
+      if (jf < 0.0) jf = 0.0;
+      if (jf > (double)(ctx->n_nodes - 2)) jf = (double)(ctx->n_nodes - 2);
+      bj[k] = (int)jf;
+      bs[k] = x - jf;
+    }
+  }
+  size_t nd = ctx->hdets.size();
+  std::vector<DetectorDev> dd(nd);
+  for (size_t i = 0; i < nd; ++i) dd[i] = ctx->hdets[i].d;
+  HIP_TRY(hipMalloc(&ctx->d_dets, nd * sizeof(DetectorDev)));
+  HIP_TRY(hipMemcpy(ctx->d_dets, dd.data(), nd * sizeof(DetectorDev), hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&ctx->d_bin_f, NB * sizeof(double)));
+  HIP_TRY(hipMalloc(&ctx->d_bin_s, NB * sizeof(double)));
+  HIP_TRY(hipMalloc(&ctx->d_bin_j, NB * sizeof(int)));
+  HIP_TRY(hipMalloc(&ctx->d_bin_w, 3 * (size_t)NB * sizeof(double)));
+  HIP_TRY(hipMalloc(&ctx->d_dre, 3 * (size_t)NB * sizeof(double)));
+  HIP_TRY(hipMalloc(&ctx->d_dim, 3 * (size_t)NB * sizeof(double)));
+  HIP_TRY(hipMemcpy(ctx->d_bin_f, bf.data(), NB * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->d_bin_s, bs.data(), NB * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->d_bin_j, bj.data(), NB * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->d_bin_w, bw.data(), 3 * (size_t)NB * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->d_dre, dre.data(), 3 * (size_t)NB * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->d_dim, dim.data(), 3 * (size_t)NB * sizeof(double), hipMemcpyHostToDevice));
+  if (ctx->direct) ctx->d_node_f = ctx->d_bin_f;
+  ctx->finalized = true;
+  return PES_OK;
+}
+
+extern "C" void pes_context_destroy(pes_context* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  free_dev(ctx->d_dets);
+  free_dev(ctx->d_bin_f);
+  free_dev(ctx->d_bin_s);
+  free_dev(ctx->d_bin_j);
+  free_dev(ctx->d_bin_w);
+  free_dev(ctx->d_dre);
+  free_dev(ctx->d_dim);
+  free_dev(ctx->d_setup);
+  free_dev(ctx->d_pconst);
+  free_dev(ctx->d_nodes);
+  free_dev(ctx->d_nodes_grad);
+  free_dev(ctx->d_partial);
+  delete ctx;
+}
+
+extern "C" int pes_detector_bins(const pes_context* ctx, int det) {
+  if (!ctx || det < 0 || det >= (int)ctx->hdets.size()) return -1;
+  return ctx->hdets[det].d.n_bins;
+}
+
+static int n_groups_of(const pes_context* ctx) { return (ctx->n_tiles + TILES_PER_GROUP - 1) / TILES_PER_GROUP; }
+
+// (re)allocate scratch for P rows; allocation happens only when P or the gradient flag grows
+static int ensure_scratch(pes_context* ctx, int P, bool grad) {
+  if (P <= ctx->cap_P && (!grad || ctx->cap_grad)) return PES_OK;
+  int cap = P > ctx->cap_P ? P : ctx->cap_P;
+  bool g = grad || ctx->cap_grad;
+  free_dev(ctx->d_setup);
+  free_dev(ctx->d_pconst);
+  free_dev(ctx->d_nodes);
+  free_dev(ctx->d_nodes_grad);
+  free_dev(ctx->d_partial);
+  ctx->d_setup = ctx->d_pconst = ctx->d_nodes = ctx->d_nodes_grad = ctx->d_partial = nullptr;
+  size_t NNS = (size_t)ctx->n_nodes + 1;
+  HIP_TRY(hipMalloc(&ctx->d_setup, (size_t)cap * N_SETUP * 5 * sizeof(double)));
+  HIP_TRY(hipMalloc(&ctx->d_pconst, (size_t)cap * PC_STRIDE * sizeof(double)));
+  HIP_TRY(hipMalloc(&ctx->d_nodes, (size_t)cap * ctx->modes.n * NNS * 4 * sizeof(double)));
+  if (g) HIP_TRY(hipMalloc(&ctx->d_nodes_grad, (size_t)cap * ctx->modes.n * NNS * 16 * sizeof(double)));
+  size_t nout = g ? 1 + PES_N_PARAMS : 1;
+  HIP_TRY(hipMalloc(&ctx->d_partial, (size_t)n_groups_of(ctx) * nout * cap * sizeof(double)));
+  ctx->cap_P = cap;
+  ctx->cap_grad = g;
+  return PES_OK;
+}
+
+static int prepare(pes_context* ctx, const double* d_params, int P, bool grad, hipStream_t st) {
+  if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
+  if (!ctx->finalized) return fail(PES_ERR_STATE, "context not finalized");
+  if (P < 0) return fail(PES_ERR_INVALID, "P must be >= 0");
+  if (P > 0 && !d_params) return fail(PES_ERR_INVALID, "params is NULL");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = ensure_scratch(ctx, P > 0 ? P : 1, grad);
+  if (rc != PES_OK) return rc;
+  if (P == 0) return PES_OK;
+  hipLaunchKernelGGL(k_setup, dim3((P + 63) / 64), dim3(64), 0, st, d_params, P, ctx->modes, ctx->f_ref,
+                     ctx->d_setup, ctx->d_pconst);
+  HIP_TRY(hipGetLastError());
+  const int NNS = ctx->n_nodes + 1;
+  dim3 g((NNS + 127) / 128, P * ctx->modes.n);
+  if (grad)
+    hipLaunchKernelGGL(k_nodes<true>, g, dim3(128), 0, st, ctx->d_setup, P, ctx->modes, ctx->n_nodes, ctx->u0,
+                       ctx->du, ctx->d_node_f, ctx->d_nodes, ctx->d_nodes_grad);
+  else
+    hipLaunchKernelGGL(k_nodes<false>, g, dim3(128), 0, st, ctx->d_setup, P, ctx->modes, ctx->n_nodes, ctx->u0,
+                       ctx->du, ctx->d_node_f, ctx->d_nodes, nullptr);
+  HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, double* d_lnl, double* d_grad,
+                             int precision, void* stream) {
+  if (precision != PES_PREC_FP64 && precision != PES_PREC_MIXED)
+    return fail(PES_ERR_INVALID, "precision must be PES_PREC_FP64 or PES_PREC_MIXED");
+  if (P > 0 && !d_lnl) return fail(PES_ERR_INVALID, "lnl output is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  bool grad = d_grad != nullptr;
+  int rc = prepare(ctx, d_params, P, grad, st);
+  if (rc != PES_OK || P == 0) return rc;
+  const int NNS = ctx->n_nodes + 1;
+  const int n_groups = n_groups_of(ctx);
+  const int n_chunks = (P + CHUNK - 1) / CHUNK;
+  const int n_work = n_groups * n_chunks;
+  int grid = ctx->n_cu * 8;
+  if (grid > n_work) grid = n_work;
+  int nout = 1;
+  if (grad) {
+    nout = 1 + PES_N_PARAMS;
+    rc = launch_lnl_grad(ctx->d_dets, (int)ctx->hdets.size(), ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j,
+                         ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre, ctx->d_dim, ctx->NB, ctx->d_pconst, ctx->d_nodes,
+                         ctx->d_nodes_grad, NNS, ctx->du, ctx->modes, P, ctx->d_partial, grid, precision, st);
+    if (rc != 0) return fail(PES_ERR_HIP, "gradient kernel launch failed");
+  } else if (precision == PES_PREC_MIXED) {
+    rc = launch_lnl_mixed(ctx->d_dets, (int)ctx->hdets.size(), ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j,
+                          ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre, ctx->d_dim, ctx->NB, ctx->d_pconst, ctx->d_nodes,
+                          NNS, ctx->du, ctx->modes, P, ctx->d_partial, grid, st);
+    if (rc != 0) return fail(PES_ERR_HIP, "mixed kernel launch failed");
+  } else {
+    hipLaunchKernelGGL(k_lnl_fp64, dim3(grid), dim3(TILE), 0, st, ctx->d_dets, (int)ctx->hdets.size(), ctx->n_tiles,
+                       ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre, ctx->d_dim, ctx->NB,
+                       ctx->d_pconst, ctx->d_nodes, NNS, ctx->du, ctx->modes, P, ctx->d_partial);
+    HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_reduce, dim3((P + 127) / 128), dim3(128), 0, st, ctx->d_partial, n_groups, P, nout,
+                     ctx->d_pconst, d_lnl, d_grad);
+  HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" int pes_response(pes_context* ctx, int det, const double* d_params, int P, double* d_out, void* stream) {
+  if (!ctx || det < 0 || det >= (int)ctx->hdets.size()) return fail(PES_ERR_INVALID, "bad detector index");
+  if (P > 0 && !d_out) return fail(PES_ERR_INVALID, "out is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = prepare(ctx, d_params, P, false, st);
+  if (rc != PES_OK || P == 0) return rc;
+  int n = ctx->hdets[det].d.n_bins;
+  hipLaunchKernelGGL(k_response, dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det, ctx->d_bin_f,
+                     ctx->d_bin_j, ctx->d_bin_s, ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1, ctx->du, ctx->modes,
+                     P, d_out);
+  HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" int pes_waveform(pes_context* ctx, int det, const double* d_params, int P, double* d_out, void* stream) {
+  if (!ctx || det < 0 || det >= (int)ctx->hdets.size()) return fail(PES_ERR_INVALID, "bad detector index");
+  if (P > 0 && !d_out) return fail(PES_ERR_INVALID, "out is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = prepare(ctx, d_params, P, false, st);
+  if (rc != PES_OK || P == 0) return rc;
+  int n = ctx->hdets[det].d.n_bins;
+  hipLaunchKernelGGL(k_waveform, dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det, ctx->d_bin_f,
+                     ctx->d_bin_j, ctx->d_bin_s, ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1, ctx->du, ctx->modes,
+                     P, d_out);
+  HIP_TRY(hipGetLastError());
+  return PES_OK;
+}

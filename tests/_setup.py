@@ -1,0 +1,56 @@
+"""Shared small workloads for the parity tests (oracle side and HIP side)."""
+
+import numpy as np
+
+from oracle import detector as odet
+from oracle import likelihood as olk
+
+INJ = dict(chirp_mass=1.2e6, mass_ratio=0.6, chi_1=0.75, chi_2=0.62, luminosity_distance=56000.0,
+           reference_phase=1.3, inclination=0.4, ecliptic_longitude=1.375, ecliptic_latitude=-1.2108,
+           polarization=2.659, coalescence_time=0.8 * 655360.0)
+
+ORBIT_KIND = {"helio": 0, "geo": 1}
+
+
+def inj_row(**over):
+    d = dict(INJ)
+    d.update(over)
+    return np.array([d[k] for k in olk.PARAM_NAMES], dtype=np.float64)
+
+
+def particle_cloud(n, seed=2026, rel=1e-3, center=None):
+    """Gaussian cloud around the injection; angles wrapped (fisher.py:297-310 semantics)."""
+    rng = np.random.default_rng(seed)
+    c = inj_row() if center is None else np.asarray(center, dtype=np.float64)
+    rows = c + rel * np.abs(c) * rng.standard_normal((n, c.size))
+    rows[:, 1] = np.clip(rows[:, 1], 1e-3, 1.0)
+    rows[:, 5] = np.mod(rows[:, 5], 2 * np.pi)      # reference phase
+    rows[:, 7] = np.mod(rows[:, 7], 2 * np.pi)      # longitude
+    rows[:, 9] = np.mod(rows[:, 9], np.pi)          # polarization
+    rows[:, 8] = np.clip(rows[:, 8], -np.pi / 2, np.pi / 2)
+    return rows
+
+
+def make_detector(orbit="LISA_analytic", psd_model="LISA_SciRDv1", gen="1.5", T=655360.0, dt=5.0,
+                  fmin=1e-4, fmax=0.05, seed=7, modes=((2, 2),), interp=(1e-4, 0.05, 1024), orthogonal=True,
+                  channels=("A", "E", "T"), inject=True):
+    f, mask, n_t, df, _ = odet.data_grid(T, dt, fmin, fmax)
+    labels = ("A", "E", "T") if orthogonal else ("X", "Y", "Z")
+    psd = odet.psd(psd_model, f, labels, False, gen)
+    noise = odet.noise_realization(seed, labels, n_t, mask, psd, df)
+    orb = odet.ORBITS[orbit]
+    d0 = olk.Detector(f, None, psd, df, orb, gen, channels, orthogonal)
+    data = dict(noise)
+    if inject:
+        resp = olk.response(INJ, d0, modes, 0.0, interp)
+        data = {c: noise[c] + resp[c] for c in labels}
+    return olk.Detector(f, data, psd, df, orb, gen, channels, orthogonal), labels
+
+
+def add_to_core(core, d, labels):
+    kind, arm, rot0, rev0 = d.orbit
+    data = np.stack([d.data[c] for c in labels])
+    psd = np.stack([d.psd[c] for c in labels])
+    mask = [1 if c in d.channels else 0 for c in labels]
+    gen = 15 if d.tdi_generation == "1.5" else 20
+    return core.add_detector(ORBIT_KIND[kind], arm, rot0, rev0, gen, d.orthogonal, d.f, data, psd, mask, df=d.df)
