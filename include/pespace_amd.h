@@ -89,13 +89,40 @@ int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, double* d_lnl
                   void* stream);
 
 /* Detector response h_c(f) for P rows on detector det: device out [P][3][n_bins][2]
- * (re, im).  Replaces update_detector_response + tdi_response_numpy
+ * (re, im), computed by the same device code as pes_lnl_batch at the given
+ * precision.  Replaces update_detector_response + tdi_response_numpy
  * (antenna.py:204-229, 305-313; tdi.py:1764-1793). */
-int pes_response(pes_context* ctx, int det, const double* d_params, int P, double* d_out, void* stream);
+int pes_response(pes_context* ctx, int det, const double* d_params, int P, double* d_out, int precision,
+                 void* stream);
 
 /* Waveform container on detector det's bins: device out [P][n_modes][n_bins][5] =
  * plus (re, im), cross (re, im), tf, with t_c = 0 (interface_ripple.py:42-51). */
 int pes_waveform(pes_context* ctx, int det, const double* d_params, int P, double* d_out, void* stream);
+
+/* Response of detector det to a caller-supplied waveform container (one parameter
+ * set): device d_wf [n_modes][n_bins][5] = plus (re, im), cross (re, im), tf with
+ * t_c = 0; device d_out [3][n_bins][2].  The drop-in for
+ * _FDResponseModelMarset2018Taichi.update_single_link_response followed by
+ * update_tdi_response (antenna.py:403-460, tdi.py:1764-1793). */
+int pes_response_from_waveform(pes_context* ctx, int det, int n_modes, const double* d_wf, double lam, double beta,
+                               double psi, double tc, double* d_out, void* stream);
+
+/* Standalone Whittle lnL = -2 df sum_c sum_k |d - h|^2 / S on device arrays
+ * (d_data, d_resp: [n_chan][n][2]; d_psd: [n_chan][n]); the result is written to
+ * device d_out[0].  d_work: caller-owned device workspace of at least
+ * PES_WHITTLE_WORK doubles.  Replaces compute_whittle_likelihood_ti
+ * (likelihood_ti.py:17-69) and noise_weighted_inner_product (likelihood_jax.py:21-42),
+ * with a fixed-order (deterministic) reduction instead of atomic_add. */
+#define PES_WHITTLE_WORK 1024
+int pes_whittle_lnl(const double* d_data, const double* d_resp, const double* d_psd, int n_chan, int n, double df,
+                    double* d_work, double* d_out, void* stream);
+
+/* Timing of the fused lnL kernel: when enabled, pes_lnl_batch records a HIP event
+ * pair around the fused kernel on its stream.  pes_profile_read waits for the
+ * recorded pairs, returns their count and summed duration (ms) and clears them.
+ * No counterpart in the reference (it has no per-stage timing). */
+int pes_context_set_profiling(pes_context* ctx, int enable);
+int pes_profile_read(pes_context* ctx, int* n_launches, double* total_ms);
 
 /* Number of bins of detector det (after finalize). */
 int pes_detector_bins(const pes_context* ctx, int det);

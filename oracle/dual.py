@@ -86,6 +86,12 @@ def sqrt(x):
     return np.sqrt(x)
 
 
+def cos(x):
+    if isinstance(x, D):
+        return D(np.cos(x.v), -np.sin(x.v) * x.d)
+    return np.cos(x)
+
+
 def atan(x):
     if isinstance(x, D):
         return D(np.arctan(x.v), x.d / (1.0 + x.v * x.v))

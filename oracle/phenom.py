@@ -37,7 +37,11 @@ The higher modes (2,1),(3,3),(3,2),(4,4),(4,3) follow London et al. 2018
 phase evaluated at a mapped frequency f22(f), scaled by the leading-order PN
 amplitude ratio, with the (l,m) QNM setting where the ringdown lands.  The map
 is f22 = 2f/m below f_i = (m/2)*0.014, f22 = (fRD22/fRDlm) f above fRDlm, and
-the C1 cubic Hermite between, so t_lm(f) is continuous.
+the C1 cubic Hermite between, so t_lm(f) is continuous.  The PN velocity in the
+amplitude ratio saturates smoothly above f_i, v = v_i (2 - exp(1 - v/v_i)), and
+every mode's amplitude is tapered to zero over the last 10% below its cut (f22 =
+0.2) by a C1 raised cosine, so that lnL is continuously differentiable in all
+parameters (no kinks or jumps move across bins when the masses change).
 
 Pins (tests/test_oracle_phenom.py): the PN part of dPsi/df equals the
 reference's get_PN_d_phase for non-spinning systems (where its 4PN terms
@@ -129,6 +133,7 @@ AMP_F_INS = 0.014  # amplitude inspiral -> intermediate
 PHI_F_INS = 0.018  # phase inspiral -> intermediate
 MF_CUT = 0.2  # PhenomD domain end
 HM_MAP_JOIN = 0.014  # 22-frequency where the HM frequency map leaves f22 = 2f/m
+TAPER = 0.1  # amplitude taper width, as a fraction of each mode's cut frequency
 MODES = ((2, 2), (2, 1), (3, 3), (3, 2), (4, 4), (4, 3))
 
 _QNM = None
@@ -417,8 +422,10 @@ class PhenomHM:
         pd = self.pd
         fi = 0.5 * m * HM_MAP_JOIN
         fv = du.val(Mf)
-        Mfe = du.where(fv < fi, Mf, du.D(np.full_like(fv, fi), 0.0))
-        v = (2.0 * PI * Mfe / m) ** (1.0 / 3.0)
+        vraw = (2.0 * PI * Mf / m) ** (1.0 / 3.0)
+        vi = (2.0 * PI * fi / m) ** (1.0 / 3.0)
+        # C1 saturation above f_i: v = vi (2 - exp(1 - vraw/vi)) (slope 1 at the joint, -> 2 vi)
+        v = du.where(fv < fi, vraw, vi * (2.0 - du.exp(1.0 - vraw / vi)))
         d, eta = pd.delta, pd.eta
         if mode == (2, 1):
             return np.sqrt(2.0) / 3.0 * (d * v - 1.5 * (pd.chia + d * pd.chis) * v * v)
@@ -439,12 +446,21 @@ class PhenomHM:
         return MF_CUT * self.qnm[mode][0] / self.qnm[(2, 2)][0]
 
     def amp_phase(self, mode, Mf):
-        """Real amplitude (without 1/dL, m s units) and phase of mode at Mf (dual-aware)."""
+        """Real amplitude (without 1/dL, m s units) and phase of mode at Mf (dual-aware).
+
+        The amplitude carries a C1 raised-cosine taper over the last TAPER of the mode's
+        range, x = Mf / Mf_cut in (1 - TAPER, 1], so that it reaches zero at the cut.
+        """
         l, m = mode
         f22 = self.fmap(mode, Mf)
         A = self.beta(mode, Mf) * self.pd.amplitude(f22) * self.amp_scale
+        x = Mf / self.mode_cut(mode)
+        xv = du.val(x)
+        x0 = 1.0 - TAPER
+        arg = (x - x0) * (PI / TAPER)
+        w = du.where(xv <= x0, 1.0, du.where(xv >= 1.0, 0.0, 0.5 * (1.0 + du.cos(arg))))
         P = (0.5 * m) * self.psi22(f22)
-        return A, P
+        return A * w, P
 
 
 # unit complex factor of each mode: conj(arg H_lm) * exp(-i (m-2) pi/8)

@@ -31,9 +31,18 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
       ctypes.c_void_p]),
     ("pes_response", ctypes.c_int,
-     [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+      ctypes.c_void_p]),
     ("pes_waveform", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    ("pes_response_from_waveform", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+      ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+    ("pes_whittle_lnl", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("pes_context_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("pes_profile_read", ctypes.c_int, [ctypes.c_void_p, c_int_p, c_double_p]),
     ("pes_detector_bins", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
 ]
 

@@ -128,13 +128,29 @@ class LikelihoodCore:
                                   PRECISION[precision], stream))
         return (lnl, g) if grad else lnl
 
-    def response(self, det, params):
+    def set_profiling(self, enable=True):
+        self._build()
+        check(lib().pes_context_set_profiling(self._ctx, int(bool(enable))))
+
+    def read_profile(self):
+        """(launches, total_ms) of the fused kernel since the last read."""
+        n = ctypes.c_int()
+        ms = ctypes.c_double()
+        check(lib().pes_profile_read(self._ctx, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    @property
+    def n_bins(self):
+        self._build()
+        return list(self._n_bins)
+
+    def response(self, det, params, precision="fp64"):
         """Complex (P, 3, N) TDI response of detector det."""
         self._build()
         p = self._params(params)
         n = self._n_bins[det]
         out = torch.empty(p.shape[0], 3, n, 2, dtype=torch.float64, device=p.device)
-        check(lib().pes_response(self._ctx, det, p.data_ptr(), p.shape[0], out.data_ptr(),
+        check(lib().pes_response(self._ctx, det, p.data_ptr(), p.shape[0], out.data_ptr(), PRECISION[precision],
                                  torch.cuda.current_stream(p.device).cuda_stream))
         return torch.view_as_complex(out)
 
@@ -147,3 +163,49 @@ class LikelihoodCore:
         check(lib().pes_waveform(self._ctx, det, p.data_ptr(), p.shape[0], out.data_ptr(),
                                  torch.cuda.current_stream(p.device).cuda_stream))
         return out
+
+    def response_from_container(self, det, waveform, lam, beta, psi, tc):
+        """Complex (3, N) response of detector det to a reference-style container.
+
+        waveform: {"plus","cross","tf"} or a mode-keyed dict of those (numpy or torch).
+        """
+        self._build()
+        n = self._n_bins[det]
+        modes = [waveform] if {"plus", "cross", "tf"} <= set(waveform.keys()) else list(waveform.values())
+        dev = torch.device(f"cuda:{self.device}")
+        wf = torch.empty(len(modes), n, 5, dtype=torch.float64, device=dev)
+        for i, m in enumerate(modes):
+            hp = torch.as_tensor(m["plus"], device=dev)
+            hc = torch.as_tensor(m["cross"], device=dev)
+            tf = torch.as_tensor(m["tf"], device=dev, dtype=torch.float64)
+            if hp.shape[-1] != n or hc.shape[-1] != n or tf.shape[-1] != n:
+                raise ValueError(f"waveform arrays must have length {n}")
+            hp = hp.to(torch.complex128)
+            hc = hc.to(torch.complex128)
+            wf[i, :, 0] = hp.real
+            wf[i, :, 1] = hp.imag
+            wf[i, :, 2] = hc.real
+            wf[i, :, 3] = hc.imag
+            wf[i, :, 4] = tf
+        out = torch.empty(3, n, 2, dtype=torch.float64, device=dev)
+        check(lib().pes_response_from_waveform(self._ctx, det, len(modes), wf.data_ptr(), float(lam), float(beta),
+                                               float(psi), float(tc), out.data_ptr(),
+                                               torch.cuda.current_stream(dev).cuda_stream))
+        return torch.view_as_complex(out)
+
+
+def whittle_lnl(data, response, psd, df):
+    """-2 df sum |d - h|^2 / S over channels and bins, on the GPU (deterministic reduction).
+
+    data, response: complex (n_chan, N) tensors; psd: real (n_chan, N).
+    """
+    dev = data.device
+    d = torch.view_as_real(data.to(torch.complex128).contiguous()).contiguous()
+    r = torch.view_as_real(response.to(device=dev, dtype=torch.complex128).contiguous()).contiguous()
+    s = psd.to(device=dev, dtype=torch.float64).contiguous()
+    n_chan, n = s.shape
+    work = torch.empty(1024, dtype=torch.float64, device=dev)
+    out = torch.empty(1, dtype=torch.float64, device=dev)
+    check(lib().pes_whittle_lnl(d.data_ptr(), r.data_ptr(), s.data_ptr(), n_chan, n, float(df), work.data_ptr(),
+                                out.data_ptr(), torch.cuda.current_stream(dev).cuda_stream))
+    return out

@@ -19,6 +19,7 @@
 
 #include "../../include/pespace_amd.h"
 #include "response.hpp"
+#include "mathfn.hpp"
 
 using namespace pes;
 
@@ -304,66 +305,7 @@ __device__ inline int tile_detector(const DetectorDev* dets, int n_det, int tile
   return n_det - 1;
 }
 
-// ---- k_lnl_fp64: persistent fused forward kernel ------------------------------------------
-__global__ __launch_bounds__(TILE) void k_lnl_fp64(const DetectorDev* __restrict__ dets, int n_det, int n_tiles,
-                                                   const double* __restrict__ bin_f, const int* __restrict__ bin_j,
-                                                   const double* __restrict__ bin_s, const double* __restrict__ bin_w,
-                                                   const double* __restrict__ dre, const double* __restrict__ dim,
-                                                   int NB, const double* __restrict__ pconst,
-                                                   const double* __restrict__ nodes, int NNS, double du, ModeList modes,
-                                                   int P, double* __restrict__ partial) {
-  __shared__ double acc[TILE / 64][CHUNK];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int n_chunks = (P + CHUNK - 1) / CHUNK;
-  const int n_groups = (n_tiles + TILES_PER_GROUP - 1) / TILES_PER_GROUP;
-  const int n_work = n_chunks * n_groups;
-  for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
-    const int g = w / n_chunks, c = w % n_chunks;
-    const int p0 = c * CHUNK, p1 = min(P, p0 + CHUNK);
-    if (threadIdx.x < (TILE / 64) * CHUNK) (&acc[0][0])[threadIdx.x] = 0.0;
-    __syncthreads();
-    const int t_end = min(n_tiles, (g + 1) * TILES_PER_GROUP);
-    for (int tile = g * TILES_PER_GROUP; tile < t_end; ++tile) {
-      const int di = tile_detector(dets, n_det, tile);
-      const DetectorDev det = dets[di];
-      const int bi = tile * TILE + threadIdx.x;
-      BinCtx b;
-      b.f = bin_f[bi];
-      b.s = bin_s[bi];
-      b.j = bin_j[bi];
-      b.du = du;
-      double wc[3];
-      Cplx dc[3];
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch) {
-        wc[ch] = bin_w[ch * NB + bi];
-        dc[ch] = {dre[ch * NB + bi], dim[ch * NB + bi]};
-      }
-      for (int p = p0; p < p1; ++p) {
-        const double* pc = pconst + (size_t)p * PC_STRIDE;
-        double v = 0.0;
-        if (pc[PC_VALID] != 0.0) {
-          Cplx out[3];
-          response_fp64(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, out);
-#pragma unroll
-          for (int ch = 0; ch < 3; ++ch) {
-            double rr = dc[ch].re - out[ch].re, ri = dc[ch].im - out[ch].im;
-            v -= wc[ch] * (rr * rr + ri * ri);
-          }
-        }
-        v = wave_sum(v);
-        if (lane == 0) acc[wave][p - p0] += v;
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x < p1 - p0) {
-      double s = 0.0;
-      for (int k = 0; k < TILE / 64; ++k) s += acc[k][threadIdx.x];
-      partial[(size_t)g * P + p0 + threadIdx.x] = s;
-    }
-    __syncthreads();
-  }
-}
+#include "hotpath.hpp"
 
 __global__ void k_reduce(const double* __restrict__ partial, int n_groups, int P, int nout,
                          const double* __restrict__ pconst, double* __restrict__ lnl, double* __restrict__ grad) {
@@ -382,30 +324,6 @@ __global__ void k_reduce(const double* __restrict__ partial, int n_groups, int P
 }
 
 // ---- response / waveform output kernels (parity checks) -------------------------------------
-__global__ void k_response(const DetectorDev* __restrict__ dets, int det_i, const double* __restrict__ bin_f,
-                           const int* __restrict__ bin_j, const double* __restrict__ bin_s,
-                           const double* __restrict__ pconst, const double* __restrict__ nodes, int NNS, double du,
-                           ModeList modes, int P, double* __restrict__ out) {
-  const DetectorDev det = dets[det_i];
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int p = blockIdx.y;
-  if (k >= det.n_bins || p >= P) return;
-  int bi = det.bin_offset + k;
-  BinCtx b;
-  b.f = bin_f[bi];
-  b.s = bin_s[bi];
-  b.j = bin_j[bi];
-  b.du = du;
-  const double* pc = pconst + (size_t)p * PC_STRIDE;
-  Cplx o[3];
-  response_fp64(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, o);
-  for (int ch = 0; ch < 3; ++ch) {
-    size_t idx = (((size_t)p * 3 + ch) * det.n_bins + k) * 2;
-    out[idx] = o[ch].re;
-    out[idx + 1] = o[ch].im;
-  }
-}
-
 __global__ void k_waveform(const DetectorDev* __restrict__ dets, int det_i, const double* __restrict__ bin_f,
                            const int* __restrict__ bin_j, const double* __restrict__ bin_s,
                            const double* __restrict__ pconst, const double* __restrict__ nodes, int NNS, double du,
@@ -440,16 +358,66 @@ __global__ void k_waveform(const DetectorDev* __restrict__ dets, int det_i, cons
   }
 }
 
-// gradient / mixed-precision launchers (defined below the forward kernels)
-int launch_lnl_grad(const DetectorDev*, int, int, const double*, const int*, const double*, const double*,
-                    const double*, const double*, int, const double*, const double*, const double*, int, double,
-                    ModeList, int, double*, int, int, hipStream_t) {
-  return -1;
+
+// ---- response from a caller-supplied waveform container ------------------------------------
+// wf: [n_modes][n_bins][5] = plus (re,im), cross (re,im), tf with t_c = 0 (the reference's
+// container contract, interface_ripple.py:42-51); antenna.py:429-460 + tdi.py:1764-1793.
+struct Extrinsic {
+  Vec3 p, q, k;
+  double tc;
+};
+__global__ void k_response_container(const DetectorDev* __restrict__ dets, int det_i, const double* __restrict__ bin_f,
+                                     int n_modes, const double* __restrict__ wf, Extrinsic ex,
+                                     double* __restrict__ out) {
+  const DetectorDev det = dets[det_i];
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= det.n_bins) return;
+  double f = bin_f[det.bin_offset + k];
+  Cplx etc;
+  sincos(-2.0 * PI * f * ex.tc, &etc.im, &etc.re);
+  Links L = {};
+  for (int m = 0; m < n_modes; ++m) {
+    const double* w = wf + ((size_t)m * det.n_bins + k) * 5;
+    Cplx hp = cmul({w[0], w[1]}, etc);
+    Cplx hc = cmul({w[2], w[3]}, etc);
+    add_mode_links(det, f, w[4] + ex.tc, hp, hc, ex.p, ex.q, ex.k, L);
+  }
+  Cplx o[3];
+  tdi_combine(det, f, L, o);
+  for (int ch = 0; ch < 3; ++ch) {
+    out[((size_t)ch * det.n_bins + k) * 2] = o[ch].re;
+    out[((size_t)ch * det.n_bins + k) * 2 + 1] = o[ch].im;
+  }
 }
-int launch_lnl_mixed(const DetectorDev*, int, int, const double*, const int*, const double*, const double*,
-                     const double*, const double*, int, const double*, const double*, int, double, ModeList, int,
-                     double*, int, hipStream_t) {
-  return -1;
+
+// ---- standalone Whittle lnL: -2 df sum_c sum_k |d - h|^2 / S (likelihood_ti.py:58-67) --------
+constexpr int WHITTLE_BLOCKS = 1024;
+__global__ void k_whittle_partial(const double* __restrict__ data, const double* __restrict__ resp,
+                                  const double* __restrict__ psd, int n_chan, int n, double* __restrict__ work) {
+  __shared__ double red[TILE / 64];
+  double v = 0.0;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    for (int c = 0; c < n_chan; ++c) {
+      size_t i = (size_t)c * n + k;
+      double rr = data[2 * i] - resp[2 * i], ri = data[2 * i + 1] - resp[2 * i + 1];
+      v += (rr * rr + ri * ri) / psd[i];
+    }
+  }
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < TILE / 64; ++w) s += red[w];
+    work[blockIdx.x] = s;
+  }
+}
+__global__ void k_whittle_final(const double* __restrict__ work, int nb, double df, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += work[b];
+    out[0] = -2.0 * df * s;
+  }
 }
 
 }  // namespace
@@ -480,6 +448,9 @@ struct pes_context {
   double *d_setup = nullptr, *d_pconst = nullptr, *d_nodes = nullptr, *d_nodes_grad = nullptr,
          *d_partial = nullptr;
   int n_cu = 256;
+  // optional timing of the fused kernel (HIP events on the launch stream)
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_start, ev_stop;
 };
 
 extern "C" const char* pes_last_error(void) { return g_err.c_str(); }
@@ -666,6 +637,8 @@ extern "C" void pes_context_destroy(pes_context* ctx) {
   free_dev(ctx->d_nodes);
   free_dev(ctx->d_nodes_grad);
   free_dev(ctx->d_partial);
+  for (auto e : ctx->ev_start) (void)hipEventDestroy(e);
+  for (auto e : ctx->ev_stop) (void)hipEventDestroy(e);
   delete ctx;
 }
 
@@ -739,22 +712,35 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
   int grid = ctx->n_cu * 8;
   if (grid > n_work) grid = n_work;
   int nout = 1;
-  if (grad) {
-    nout = 1 + PES_N_PARAMS;
-    rc = launch_lnl_grad(ctx->d_dets, (int)ctx->hdets.size(), ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j,
-                         ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre, ctx->d_dim, ctx->NB, ctx->d_pconst, ctx->d_nodes,
-                         ctx->d_nodes_grad, NNS, ctx->du, ctx->modes, P, ctx->d_partial, grid, precision, st);
-    if (rc != 0) return fail(PES_ERR_HIP, "gradient kernel launch failed");
-  } else if (precision == PES_PREC_MIXED) {
-    rc = launch_lnl_mixed(ctx->d_dets, (int)ctx->hdets.size(), ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j,
-                          ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre, ctx->d_dim, ctx->NB, ctx->d_pconst, ctx->d_nodes,
-                          NNS, ctx->du, ctx->modes, P, ctx->d_partial, grid, st);
-    if (rc != 0) return fail(PES_ERR_HIP, "mixed kernel launch failed");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ctx->profiling) {
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, st));
+  }
+  const int n_det = (int)ctx->hdets.size();
+  if (grad) nout = 1 + PES_N_PARAMS;
+#define PES_LNL_ARGS                                                                                         \
+  ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre,       \
+      ctx->d_dim, ctx->NB, ctx->d_pconst, ctx->d_nodes, ctx->d_nodes_grad, NNS, ctx->du, ctx->modes, P,       \
+      ctx->d_partial
+  if (precision == PES_PREC_FP64) {
+    if (grad)
+      hipLaunchKernelGGL((k_lnl<double, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+    else
+      hipLaunchKernelGGL((k_lnl<double, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
   } else {
-    hipLaunchKernelGGL(k_lnl_fp64, dim3(grid), dim3(TILE), 0, st, ctx->d_dets, (int)ctx->hdets.size(), ctx->n_tiles,
-                       ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre, ctx->d_dim, ctx->NB,
-                       ctx->d_pconst, ctx->d_nodes, NNS, ctx->du, ctx->modes, P, ctx->d_partial);
-    HIP_TRY(hipGetLastError());
+    if (grad)
+      hipLaunchKernelGGL((k_lnl<float, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+    else
+      hipLaunchKernelGGL((k_lnl<float, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+  }
+#undef PES_LNL_ARGS
+  HIP_TRY(hipGetLastError());
+  if (ctx->profiling) {
+    HIP_TRY(hipEventRecord(e1, st));
+    ctx->ev_start.push_back(e0);
+    ctx->ev_stop.push_back(e1);
   }
   hipLaunchKernelGGL(k_reduce, dim3((P + 127) / 128), dim3(128), 0, st, ctx->d_partial, n_groups, P, nout,
                      ctx->d_pconst, d_lnl, d_grad);
@@ -762,16 +748,24 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
   return PES_OK;
 }
 
-extern "C" int pes_response(pes_context* ctx, int det, const double* d_params, int P, double* d_out, void* stream) {
+extern "C" int pes_response(pes_context* ctx, int det, const double* d_params, int P, double* d_out, int precision,
+                            void* stream) {
+  if (precision != PES_PREC_FP64 && precision != PES_PREC_MIXED)
+    return fail(PES_ERR_INVALID, "precision must be PES_PREC_FP64 or PES_PREC_MIXED");
   if (!ctx || det < 0 || det >= (int)ctx->hdets.size()) return fail(PES_ERR_INVALID, "bad detector index");
   if (P > 0 && !d_out) return fail(PES_ERR_INVALID, "out is NULL");
   hipStream_t st = (hipStream_t)stream;
   int rc = prepare(ctx, d_params, P, false, st);
   if (rc != PES_OK || P == 0) return rc;
   int n = ctx->hdets[det].d.n_bins;
-  hipLaunchKernelGGL(k_response, dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det, ctx->d_bin_f,
-                     ctx->d_bin_j, ctx->d_bin_s, ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1, ctx->du, ctx->modes,
-                     P, d_out);
+  if (precision == PES_PREC_FP64)
+    hipLaunchKernelGGL(k_response_t<double>, dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det,
+                       ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1,
+                       ctx->du, ctx->modes, P, d_out);
+  else
+    hipLaunchKernelGGL(k_response_t<float>, dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det,
+                       ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1,
+                       ctx->du, ctx->modes, P, d_out);
   HIP_TRY(hipGetLastError());
   return PES_OK;
 }
@@ -787,5 +781,64 @@ extern "C" int pes_waveform(pes_context* ctx, int det, const double* d_params, i
                      ctx->d_bin_j, ctx->d_bin_s, ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1, ctx->du, ctx->modes,
                      P, d_out);
   HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" int pes_response_from_waveform(pes_context* ctx, int det, int n_modes, const double* d_wf, double lam,
+                                          double beta, double psi, double tc, double* d_out, void* stream) {
+  if (!ctx || !ctx->finalized) return fail(PES_ERR_STATE, "context missing or not finalized");
+  if (det < 0 || det >= (int)ctx->hdets.size()) return fail(PES_ERR_INVALID, "bad detector index");
+  if (n_modes < 1 || !d_wf || !d_out) return fail(PES_ERR_INVALID, "waveform/out must be non-NULL, n_modes >= 1");
+  HIP_TRY(hipSetDevice(ctx->device));
+  Extrinsic ex;
+  double sl = std::sin(lam), cl = std::cos(lam), sb = std::sin(beta), cb = std::cos(beta);
+  double sp = std::sin(psi), cp = std::cos(psi);
+  ex.p = {sl * cp - cl * sb * sp, -cl * cp - sl * sb * sp, cb * sp};   // antenna.py:70-76
+  ex.q = {-sl * sp - cl * sb * cp, cl * sp - sl * sb * cp, cb * cp};   // antenna.py:77-83
+  ex.k = {-cb * cl, -cb * sl, -sb};                                    // antenna.py:112-114
+  ex.tc = tc;
+  int n = ctx->hdets[det].d.n_bins;
+  hipLaunchKernelGGL(k_response_container, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream, ctx->d_dets,
+                     det, ctx->d_bin_f, n_modes, d_wf, ex, d_out);
+  HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" int pes_whittle_lnl(const double* d_data, const double* d_resp, const double* d_psd, int n_chan, int n,
+                               double df, double* d_work, double* d_out, void* stream) {
+  if (n_chan < 1 || n < 1 || !d_data || !d_resp || !d_psd || !d_work || !d_out)
+    return fail(PES_ERR_INVALID, "whittle: arrays must be non-NULL, n_chan >= 1, n >= 1");
+  int nb = (n + TILE - 1) / TILE;
+  if (nb > WHITTLE_BLOCKS) nb = WHITTLE_BLOCKS;
+  hipLaunchKernelGGL(k_whittle_partial, dim3(nb), dim3(TILE), 0, (hipStream_t)stream, d_data, d_resp, d_psd, n_chan,
+                     n, d_work);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_whittle_final, dim3(1), dim3(64), 0, (hipStream_t)stream, d_work, nb, df, d_out);
+  HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" int pes_context_set_profiling(pes_context* ctx, int enable) {
+  if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
+  ctx->profiling = enable != 0;
+  return PES_OK;
+}
+
+extern "C" int pes_profile_read(pes_context* ctx, int* n_launches, double* total_ms) {
+  if (!ctx || !n_launches || !total_ms) return fail(PES_ERR_INVALID, "NULL argument");
+  HIP_TRY(hipSetDevice(ctx->device));
+  double tot = 0.0;
+  for (size_t i = 0; i < ctx->ev_start.size(); ++i) {
+    HIP_TRY(hipEventSynchronize(ctx->ev_stop[i]));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_start[i], ctx->ev_stop[i]));
+    tot += ms;
+    (void)hipEventDestroy(ctx->ev_start[i]);
+    (void)hipEventDestroy(ctx->ev_stop[i]);
+  }
+  *n_launches = (int)ctx->ev_start.size();
+  *total_ms = tot;
+  ctx->ev_start.clear();
+  ctx->ev_stop.clear();
   return PES_OK;
 }

@@ -34,6 +34,7 @@ constexpr double AMP_F_INS = 0.014;
 constexpr double PHI_F_INS = 0.018;
 constexpr double MF_CUT = 0.2;
 constexpr double HM_MAP_JOIN = 0.014;
+constexpr double TAPER = 0.1;
 
 // ---- calibrated coefficients (Khan et al. 2016 table V) --------------------------------
 // order: l00, l10, (xi) l01 l11 l21, (xi^2) l02 l12 l22, (xi^3) l03 l13 l23
@@ -233,8 +234,10 @@ struct Phenom {
     if (mode == 0) return lift<S>(1.0);
     const double m = (double)kModeM[mode];
     const double fi = 0.5 * m * HM_MAP_JOIN;
-    S fe = value_of(f) < fi ? f : lift<S>(fi);
-    S v = dcbrt(fe * (2.0 * PI / m));
+    S vraw = dcbrt(f * (2.0 * PI / m));
+    const double vi = cbrt(fi * (2.0 * PI / m));
+    // C1 saturation above f_i: v = vi (2 - exp(1 - vraw/vi))
+    S v = value_of(f) < fi ? vraw : vi * (2.0 - dexp(1.0 - vraw * (1.0 / vi)));
     const S& d = c[SU_DELTA];
     const S& eta = c[SU_ETA];
     switch (mode) {
@@ -246,10 +249,25 @@ struct Phenom {
     }
   }
   // amplitude (m s, no 1/dL) and phase of a mode at Mf
+  // amplitude with a C1 raised-cosine taper over the last TAPER of the mode's range
   PES_HD void amp_phase(int mode, const S& Mf, S& A, S& P) const {
     S f22 = fmap(mode, Mf);
     A = beta(mode, Mf) * amplitude(f22) * c[SU_AMPSCALE];
+    S x = Mf / mode_cut_s(mode);
+    const double x0 = 1.0 - TAPER;
+    double xv = value_of(x);
+    if (xv >= 1.0) {
+      A = A * 0.0;
+    } else if (xv > x0) {
+      S w = 0.5 * (1.0 + dcos((x - x0) * (PI / TAPER)));
+      A = A * w;
+    }
     P = (0.5 * (double)kModeM[mode]) * psi22(f22);
+  }
+  // mode cut (Mf) as a differentiable quantity
+  PES_HD S mode_cut_s(int mode) const {
+    if (mode == 0) return lift<S>(MF_CUT);
+    return MF_CUT * frd_mode(mode) / c[SU_FRD];
   }
   // Mf beyond which the mode is zero (f22 = 0.2)
   PES_HD double mode_cut(int mode) const {

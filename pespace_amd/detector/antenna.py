@@ -1,0 +1,138 @@
+"""Detector facade and the Marsat-Baker FD response model, hip backend.
+
+Mirrors pespace/detector/antenna.py:117-379: ``InterferometerAntenna`` owns the
+TDI data, orbit, response model and TDI combination; ``update_detector_response``
+computes the A/E/T (or X/Y/Z) response of a waveform on the GPU and
+``inject_signal`` adds it to the data.  The waveform may be a hip waveform model
+(``pespace_amd.waveform.IMRPhenomD/IMRPhenomHM``, evaluated inside the fused
+kernel) or any reference-style container ``{"plus","cross","tf"}`` / mode-keyed
+dict of those (antenna.py:438-458).
+"""
+
+from __future__ import annotations
+
+from collections import namedtuple
+
+import numpy as np
+import torch
+
+from .orbit import OrbitModelBase, available_orbit_models
+
+BACKENDS = ("hip",)
+
+
+class FDResponseModelMarset2018:
+    """Marsat et al. 2018 (arXiv:1806.10734) FD single-link response, fused with TDI on the device."""
+
+    def __init__(self, backend="hip"):
+        if backend not in BACKENDS:
+            raise ValueError(f"Invalid backend: {backend}")
+        self.backend = backend
+        self.detector = None
+
+    def init_single_link_response_model(self, detector):
+        self.detector = detector
+
+    def update_single_link_response(self, waveform, lam, beta, psi, tc):
+        self.detector._update_response(waveform, lam, beta, psi, tc)
+
+
+class InterferometerAntenna:
+    """antenna.py:117-323 with backend='hip'."""
+
+    def __init__(self, name, tdi_data, orbit_model, response_model, tdi_combination, backend="hip"):
+        if backend not in BACKENDS:
+            raise ValueError(f"Unknown backend: {backend}")
+        if isinstance(orbit_model, str):
+            models = available_orbit_models(backend)
+            if orbit_model not in models:
+                raise ValueError(f"{orbit_model} is not a implemented orbit model. \n"
+                                 f"Current available models are {[*models.keys()]}")
+            orbit_model = models[orbit_model]
+        elif not isinstance(orbit_model, OrbitModelBase):
+            raise TypeError(f"Expected OrbitModelBase or str for orbit_model, but got {type(orbit_model).__name__}.")
+        for component in (tdi_data, orbit_model, response_model, tdi_combination):
+            comp_backend = getattr(component, "backend")
+            if comp_backend != backend:
+                raise ValueError(f"Backend mismatch: the passed class {component.__class__.__name__} uses "
+                                 f"{comp_backend} as its backend. But current backend for InterferometerAntenna "
+                                 f"is setting to {backend}.")
+        self.name = name
+        self.tdi_data = tdi_data
+        self.orbit_model = orbit_model
+        self.response_model = response_model
+        self.tdi_combination = tdi_combination
+        self.backend = backend
+        self.tdi_response = None
+        self._core = None
+        self._core_key = None
+        self.tdi_combination.init_tdi_combination_model(self)
+        self.response_model.init_single_link_response_model(self)
+
+    # -- HIP context holding this detector's grid (data enter only through the likelihood) --
+    def _response_core(self, modes, reference_frequency, n_nodes, f_node_min, f_node_max):
+        from ..core import LikelihoodCore
+        key = (tuple(modes), reference_frequency, n_nodes, f_node_min, f_node_max)
+        if self._core is None or self._core_key != key:
+            core = LikelihoodCore(modes=modes, reference_frequency=reference_frequency, n_nodes=n_nodes,
+                                  f_node_min=f_node_min, f_node_max=f_node_max)
+            add_detector_to_core(core, self, data_channels=())
+            self._core = core
+            self._core_key = key
+        return self._core
+
+    def _update_response(self, waveform, lam, beta, psi, tc):
+        from ..waveform.phenom import PhenomWaveformBase
+        labels = self.tdi_combination.labels
+        if isinstance(waveform, PhenomWaveformBase):
+            row = waveform.parameter_row(lam, beta, psi, tc)
+            core = self._response_core(waveform.modes, waveform.reference_frequency, waveform.n_nodes,
+                                       waveform.f_node_min, waveform.f_node_max)
+            out = core.response(0, row[None, :])[0]
+        else:
+            core = self._response_core(("22",), 0.0, 2, None, None)
+            out = core.response_from_container(0, waveform, lam, beta, psi, tc)
+        DataContent = namedtuple("DataContent", labels)
+        self.tdi_response = DataContent(*[out[i] for i in range(3)])
+
+    def update_detector_response(self, waveform, lam, beta, psi, tc):
+        """antenna.py:204-229."""
+        self.response_model.update_single_link_response(waveform, lam, beta, psi, tc)
+        self.tdi_combination.update_tdi_response()
+
+    def inject_signal(self, waveform, lam, beta, psi, tc):
+        """antenna.py:231-271."""
+        self.update_detector_response(waveform, lam, beta, psi, tc)
+        chans = self.tdi_data.data_info.channels
+        DataContent = namedtuple("DataContent", chans)
+        resp = self.tdi_response_numpy
+        self.tdi_data.add_into_fd_data(DataContent(**{c: getattr(resp, c) for c in chans}))
+
+    @property
+    def tdi_response_numpy(self):
+        labels = self.tdi_combination.labels
+        DataContent = namedtuple("DataContent", labels)
+        return DataContent(*[getattr(self.tdi_response, c).detach().cpu().numpy() for c in labels])
+
+
+def add_detector_to_core(core, det, data_channels):
+    """Register an InterferometerAntenna's grid/data/PSD with a LikelihoodCore."""
+    labels = det.tdi_combination.labels
+    td = det.tdi_data
+    f = np.asarray(td.frequency_samples, dtype=np.float64)
+    n = f.size
+    data = np.zeros((3, n), np.complex128)
+    psd = np.ones((3, n), np.float64)
+    mask = [0, 0, 0]
+    for i, c in enumerate(labels):
+        if c in data_channels:
+            data[i] = np.asarray(getattr(td.fd_data, c))
+            psd[i] = np.asarray(getattr(td.fd_noise_power_density, c))
+            mask[i] = 1
+    orb = det.orbit_model
+    weight = td.frequency_weights()
+    uniform = td.data_info.delta_frequency is not None
+    return core.add_detector(orb.kind, orb.arm_length, orb.rotation_initial, orb.revolution_initial,
+                             det.tdi_combination.generation_code, det.tdi_combination.orthogonal, f, data, psd, mask,
+                             df=td.data_info.delta_frequency if uniform else 0.0,
+                             weight=None if uniform else weight)

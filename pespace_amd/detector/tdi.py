@@ -1,0 +1,271 @@
+"""TDI data container and the FD Michelson TDI model, hip backend.
+
+Mirrors pespace/detector/tdi.py: ``DataInformation`` (tdi.py:64-160),
+``TDIChannelData`` (tdi.py:162-1376; the FD parts) and
+``FDMichelsonConstantEqualArm`` (tdi.py:1603-1647).  Grid, PSD and noise are
+setup-time host work (numpy, as in the reference); the per-evaluation response,
+TDI combination and likelihood run in the HIP library.
+"""
+
+from __future__ import annotations
+
+import warnings
+from collections import namedtuple
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .noise import FrequencyDomainNoiseModel, available_noise_models
+
+BACKENDS = ("hip",)
+
+
+@dataclass(frozen=True)
+class DataInformation:
+    """tdi.py:64-160: rfft grid of a (duration, delta_time) series, masked to [fmin, fmax]."""
+
+    channels: tuple
+    duration: float
+    delta_time: float
+    start_time: float
+    minimum_frequency: float
+    maximum_frequency: float
+
+    sampling_frequency: float = field(init=False)
+    delta_frequency: float = field(init=False)
+    time_series_length: int = field(init=False)
+    full_frequency_series_length: int = field(init=False)
+    full_frequency_samples_array: np.ndarray = field(init=False, repr=False)
+    frequency_mask_array: np.ndarray = field(init=False, repr=False)
+    frequency_samples_array: np.ndarray = field(init=False, repr=False)
+    frequency_series_length: int = field(init=False)
+
+    def __post_init__(self):
+        sf = 1 / self.delta_time
+        df = 1 / self.duration  # tdi.py:130
+        n_t = int(self.duration // self.delta_time + 1)  # tdi.py:131
+        full = np.fft.rfftfreq(n_t, self.delta_time)  # tdi.py:136
+        mask = (full >= self.minimum_frequency) * (full <= self.maximum_frequency)
+        f = full[mask]
+        for k, v in (("sampling_frequency", sf), ("delta_frequency", df), ("time_series_length", n_t),
+                     ("full_frequency_series_length", int(full.size)), ("full_frequency_samples_array", full),
+                     ("frequency_mask_array", mask), ("frequency_samples_array", f),
+                     ("frequency_series_length", int(f.size))):
+            object.__setattr__(self, k, v)
+
+    @property
+    def time_samples_array(self):
+        return np.arange(self.time_series_length) * self.delta_time + self.start_time
+
+
+@dataclass(frozen=True)
+class CustomGridInformation:
+    """An explicit (e.g. log-spaced) frequency grid with per-bin weights df_k.
+
+    Not in the reference, whose grids are always rfft grids; used for
+    lnL = -2 sum_k df_k sum_c |d-h|^2/S_c on non-uniform grids, which reduces to
+    tdi.py's df = 1/T form when df_k is constant.
+    """
+
+    channels: tuple
+    frequency_samples_array: np.ndarray
+    frequency_weights_array: np.ndarray
+
+    @property
+    def frequency_series_length(self):
+        return int(self.frequency_samples_array.size)
+
+    @property
+    def delta_frequency(self):
+        return None
+
+
+class TDIChannelData:
+    """FD TDI data, noise PSD and noise realizations (tdi.py:162-1376, hip backend)."""
+
+    def __init__(self, scaling: bool = False, backend: str = "hip"):
+        if backend not in BACKENDS:
+            raise ValueError(f"Invalid backend: {backend}")
+        self.scaling = scaling
+        self.backend = backend
+        self._reset()
+
+    def _reset(self):
+        self.frequency_samples = None
+        self.fd_data = None
+        self.fd_noise_power_density = None
+        self._data_info = None
+        self._reset_flag = False
+
+    @property
+    def data_info(self):
+        return self._data_info
+
+    def set_data_info(self, channels, duration, delta_time, start_time, minimum_frequency, maximum_frequency):
+        if self._reset_flag:
+            warnings.warn("You are setting `data_info`, but you have set TDI data of current instance previously.")
+        self._data_info = DataInformation(tuple(channels), duration, delta_time, start_time, minimum_frequency,
+                                          maximum_frequency)
+
+    def _store(self, data_array):
+        chans = self._data_info.channels
+        DataContent = namedtuple("DataContent", chans)
+        self.frequency_samples = np.asarray(self._data_info.frequency_samples_array, dtype=np.float64)
+        self.fd_data = DataContent(*[np.asarray(a, dtype=np.complex128) for a in data_array])
+        self._reset_flag = True
+
+    def set_fd_data_from_zero(self, channels, duration, delta_time, start_time=0.0, minimum_frequency=1e-5,
+                              maximum_frequency=0.1):
+        """tdi.py:585-649."""
+        if self._reset_flag:
+            warnings.warn("You are setting `fd_data` with zero value, but you have set TDI data previously; "
+                          "the instance is reset.")
+            self._reset()
+        self.set_data_info(channels, duration, delta_time, start_time, minimum_frequency, maximum_frequency)
+        n = self._data_info.frequency_series_length
+        self._store(np.zeros((len(channels), n), dtype=np.complex128))
+
+    def set_fd_data_from_input(self, channels, duration, delta_time, tdi_data_array, start_time=0.0,
+                               minimum_frequency=1e-5, maximum_frequency=0.1):
+        """tdi.py:416-519."""
+        if self._reset_flag:
+            warnings.warn("You are setting `fd_data` with input array, but you have set TDI data previously; "
+                          "the instance is reset.")
+            self._reset()
+        self.set_data_info(channels, duration, delta_time, start_time, minimum_frequency, maximum_frequency)
+        arr = np.asarray(tdi_data_array)
+        nc, ns = arr.shape
+        if nc != len(channels):
+            raise ValueError(f"You set channels with {channels}, while the length of first dimension of input "
+                             f"tdi_data_array is {nc}.")
+        if ns != self._data_info.frequency_series_length:
+            raise ValueError(f"The length of second dimension of input array is {ns}, different from "
+                             f"frequency_series_length={self._data_info.frequency_series_length}.")
+        self._store(arr)
+
+    def set_fd_data_on_grid(self, channels, frequencies, weights, tdi_data_array=None):
+        """Explicit frequency grid with per-bin weights (log-spaced grids)."""
+        f = np.asarray(frequencies, dtype=np.float64)
+        w = np.asarray(weights, dtype=np.float64)
+        if f.shape != w.shape or f.ndim != 1:
+            raise ValueError("frequencies and weights must be 1-D arrays of equal length")
+        self._reset()
+        self._data_info = CustomGridInformation(tuple(channels), f, w)
+        arr = np.zeros((len(channels), f.size), np.complex128) if tdi_data_array is None else tdi_data_array
+        self.frequency_samples = f
+        DataContent = namedtuple("DataContent", channels)
+        self.fd_data = DataContent(*[np.asarray(a, dtype=np.complex128) for a in arr])
+        self._reset_flag = True
+
+    def set_fd_noise_power_density_from_model(self, noise_model, **model_kwards):
+        """tdi.py:771-837 (``tdi_generation`` is passed through the keyword arguments)."""
+        if isinstance(noise_model, str):
+            if noise_model not in available_noise_models:
+                raise ValueError(f"{noise_model} is not an implemented noise model. \n"
+                                 f"Current available noise models include {[*available_noise_models.keys()]}.")
+            psd = available_noise_models[noise_model]
+        elif isinstance(noise_model, FrequencyDomainNoiseModel):
+            psd = noise_model
+        else:
+            raise TypeError(f"unsupported noise model {type(noise_model).__name__}")
+        if self._data_info is None:
+            raise ValueError("The `data_info` has not yet set.")
+        if self.fd_noise_power_density is not None:
+            warnings.warn("You are setting `fd_noise_power_density` for current instance, but it has been set "
+                          "previously. It will be reset and updated.")
+        out = psd(self._data_info.frequency_samples_array, self._data_info.channels, self.scaling, **model_kwards)
+        DataContent = namedtuple("DataContent", self._data_info.channels)
+        self.fd_noise_power_density = DataContent(**{c: np.asarray(out[c], np.float64)
+                                                     for c in self._data_info.channels})
+
+    def get_fd_noise_realization(self, seed=None, output_type="numpy"):
+        """tdi.py:849-930: same numpy RNG call sequence (re, im per channel, DC/Nyquist zeroed)."""
+        if self.fd_noise_power_density is None:
+            raise ValueError("Setting `fd_noise_power_density` before generating noise realization.")
+        if output_type not in ("numpy", "hip"):
+            raise ValueError(f"Unknown output_type: {output_type}")
+        info = self._data_info
+        rng = np.random.default_rng(seed=seed)
+        noise = {}
+        if isinstance(info, DataInformation):
+            var = 0.5 / info.delta_frequency ** 0.5
+            for chan in info.channels:
+                re, im = rng.normal(0, var, (2, info.full_frequency_series_length))
+                re[0] = 0.0
+                im[0] = 0.0
+                if np.mod(info.time_series_length, 2) == 0:
+                    im[-1] = 0.0
+                    re[-1] = 0.0
+                psd_chan = np.asarray(getattr(self.fd_noise_power_density, chan))
+                noise[chan] = (re + 1j * im)[info.frequency_mask_array] * psd_chan ** 0.5
+        else:  # explicit grid: per-bin variance 0.5 / sqrt(df_k)
+            var = 0.5 / info.frequency_weights_array ** 0.5
+            for chan in info.channels:
+                re, im = rng.normal(0, 1.0, (2, info.frequency_series_length))
+                psd_chan = np.asarray(getattr(self.fd_noise_power_density, chan))
+                noise[chan] = (re + 1j * im) * var * psd_chan ** 0.5
+        DataContent = namedtuple("DataContent", info.channels)
+        return DataContent(**noise)
+
+    def add_into_fd_data(self, input):
+        """tdi.py:942-1010: add a per-channel NamedTuple (numpy or torch) into fd_data."""
+        if not (isinstance(input, tuple) and hasattr(input, "_fields")):
+            raise TypeError(f"Expected input to be a NamedTuple instance, but got {type(input).__name__}.")
+        n = self._data_info.frequency_series_length
+        vals = {}
+        for c in input._fields:
+            a = getattr(input, c)
+            if hasattr(a, "detach"):
+                a = a.detach().cpu().numpy()
+            vals[c] = np.asarray(a)
+        if not all(v.shape == (n,) for v in vals.values()):
+            raise ValueError("Cannot add the input NamedTuple into the `fd_data`, since there is at least one "
+                             "array in the input having different length with the TDI data.")
+        if set(input._fields) != set(self._data_info.channels):
+            raise ValueError("Cannot add the input NamedTuple of array into the `fd_data`, since the channnels "
+                             "contained by input is different with the TDI data.")
+        DataContent = namedtuple("DataContent", self._data_info.channels)
+        self.fd_data = DataContent(**{c: getattr(self.fd_data, c) + vals[c] for c in self._data_info.channels})
+
+    @property
+    def fd_data_numpy(self):
+        return self.fd_data
+
+    @property
+    def fd_noise_power_density_numpy(self):
+        return self.fd_noise_power_density
+
+    def frequency_weights(self):
+        """Per-bin df_k (constant 1/T on rfft grids)."""
+        info = self._data_info
+        if isinstance(info, DataInformation):
+            return np.full(info.frequency_series_length, info.delta_frequency)
+        return info.frequency_weights_array
+
+
+class FDMichelsonConstantEqualArm:
+    """FD Michelson TDI 1.5 / 2.0, A/E/T or X/Y/Z (tdi.py:1603-1647); evaluated in the HIP kernels."""
+
+    domain = "fd"
+
+    def __init__(self, generation="1.5", orthogonal=True, backend="hip"):
+        self.generation = str(generation)
+        if self.generation not in ("1.5", "2.0"):
+            raise ValueError(f"Unsupported generation {self.generation}.")
+        self.orthogonal = bool(orthogonal)
+        self.labels = ("A", "E", "T") if self.orthogonal else ("X", "Y", "Z")
+        if backend not in BACKENDS:
+            raise ValueError(f"Unknown backend: {backend}")
+        self.backend = backend
+        self.detector = None
+
+    @property
+    def generation_code(self):
+        return 15 if self.generation == "1.5" else 20
+
+    def init_tdi_combination_model(self, detector):
+        self.detector = detector
+
+    def update_tdi_response(self):
+        """The TDI combination is fused into the response kernel; nothing to do here."""
+        return None

@@ -36,6 +36,8 @@ def make_detector(orbit="LISA_analytic", psd_model="LISA_SciRDv1", gen="1.5", T=
                   channels=("A", "E", "T"), inject=True):
     f, mask, n_t, df, _ = odet.data_grid(T, dt, fmin, fmax)
     labels = ("A", "E", "T") if orthogonal else ("X", "Y", "Z")
+    if not orthogonal and channels == ("A", "E", "T"):
+        channels = labels
     psd = odet.psd(psd_model, f, labels, False, gen)
     noise = odet.noise_realization(seed, labels, n_t, mask, psd, df)
     orb = odet.ORBITS[orbit]

@@ -1,0 +1,49 @@
+"""Mixed precision (FP32 bulk, FP64 phases and reductions) vs the FP64 path.
+
+Near the likelihood peak (where samplers spend their time) the FP32 rounding of
+the response enters lnL through <d - h, dh> ~ SNR * eps and must stay below 1e-3.
+Far from the peak the error grows like SNR^2 * eps; those rows are reported, not
+asserted.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import phenom as oph
+from tests._setup import add_to_core, inj_row, make_detector, particle_cloud
+
+pytestmark = pytest.mark.gpu
+
+
+def _core(modes, d, labels):
+    from pespace_amd.core import LikelihoodCore
+    core = LikelihoodCore(modes=[f"{l}{m}" for l, m in modes], n_nodes=1024, f_node_min=1e-4, f_node_max=0.05)
+    add_to_core(core, d, labels)
+    return core
+
+
+@pytest.mark.parametrize("modes,gen", [(((2, 2),), "1.5"), (oph.MODES, "1.5"), (oph.MODES, "2.0")])
+def test_mixed_near_peak(modes, gen):
+    d, labels = make_detector(modes=modes, gen=gen)
+    core = _core(modes, d, labels)
+    near = np.vstack([inj_row(), particle_cloud(63, seed=5, rel=1e-8)])
+    far = particle_cloud(16, seed=6, rel=1e-4)
+    rows = np.vstack([near, far])
+    l64, g64 = core.lnl(rows, grad=True, precision="fp64")
+    lmx, gmx = core.lnl(rows, grad=True, precision="mixed")
+    l64, lmx, g64, gmx = (x.cpu().numpy() for x in (l64, lmx, g64, gmx))
+    dl = np.abs(lmx - l64)
+    n = near.shape[0]
+    scale = np.abs(g64[:n]).max(axis=0)
+    rel = np.abs(gmx[:n] - g64[:n]) / scale
+    print(f"near: dlnl max {dl[:n].max():.2e} p99 {np.percentile(dl[:n], 99):.2e}; "
+          f"far: dlnl max {dl[n:].max():.2e}; lnl-range near {l64[:n].min():.1f}..{l64[:n].max():.1f}; "
+          f"grad rel (to max |g|) max {rel.max():.2e}")
+    assert dl[:n].max() < 1e-3
+    # near the peak the gradient is O(SNR) while its FP32 error is O(eps SNR^2): ~1e-4 relative
+    assert rel.max() < 5e-3
+    r64 = core.response(0, rows[:2], precision="fp64").cpu().numpy()
+    rmx = core.response(0, rows[:2], precision="mixed").cpu().numpy()
+    err = np.linalg.norm(rmx - r64) / np.linalg.norm(r64)
+    print("response rel err", err)
+    assert err < 2e-6
