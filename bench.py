@@ -63,7 +63,7 @@ def build_network(workload, seed=7):
     for i, name in enumerate(wl["detectors"]):
         orbit, psd, fmin, fmax = DETECTORS[name]
         td = TDIChannelData(backend="hip")
-        td.set_fd_data_from_zero(("A", "E", "T"), YEAR_SI, 10.0, 0.0, fmin, fmax)
+        td.set_fd_data_from_zero(("A", "E", "T"), YEAR_SI, 5.0, 0.0, fmin, fmax)
         td.set_fd_noise_power_density_from_model(psd, tdi_generation="1.5")
         det = InterferometerAntenna(name, td, available_orbit_models()[orbit], FDResponseModelMarset2018(),
                                     FDMichelsonConstantEqualArm("1.5", True), backend="hip")
