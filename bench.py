@@ -147,6 +147,17 @@ def cpu_baseline(dets, wf, rows, budget_s=15.0):
                        f"{dt * 1e3:.1f} ms per sampled eval")
 
 
+def copy_calibration(n_bytes=1 << 30, reps=3):
+    """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel calibrate the counters."""
+    import torch
+    a = torch.ones(n_bytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n_bytes, "copy_bytes_written": n_bytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,7 +168,12 @@ def main():
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--precision", default="both", choices=("both", "fp64", "mixed"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the path shards over particles with "
               "one all-gather per SMC level, which this build does not run.", file=sys.stderr)

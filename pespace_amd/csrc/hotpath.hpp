@@ -1,368 +1,346 @@
-// Fused per-(bin, row) forward response and adjoint, generic in the bulk real type R.
+// Fused per-(bin, row) forward response and adjoint.
 //
-//   R = double : the FP64 path.  The t_c shift is applied as a separate factor
-//                exp(-2 pi i f t_c) exactly as the reference does (antenna.py:435-441).
-//   R = float  : the mixed path.  FP64 is kept for the phase Psi(f) + 2 pi f t_c,
-//                the orbit angle and the barycentric delay 2 pi f k.x0, each reduced
-//                mod pi/2 in FP64 before an FP32 sine/cosine; amplitudes, link
-//                projections, transfer functions and the TDI combination run in FP32;
-//                the residual, lnL and gradient accumulations are FP64.
+// Arithmetic is FP64 in both precision modes; they differ only in where sines, cosines
+// and reciprocals come from:
+//   MIXED = false (FP64 path): sincos_d / sincos_small_d / rcp_d, and the t_c shift and the
+//     barycentric delay are applied as separate factors exp(-2 pi i f t_c),
+//     exp(-2 pi i f k.x0) exactly as the reference does (antenna.py:435, 530-533).
+//   MIXED = true: the transfer-function sines/cosines and reciprocals, and the phasor of the
+//     total phase Psi + 2 pi f (t_c + k.x0) (summed and reduced mod 2 pi in FP64), come from
+//     the FP32 hardware units (v_sin_f32, v_cos_f32, v_rcp_f32).  The orbit angle keeps the
+//     FP64 sincos_d: its error is multiplied by 2 pi f AU/c (~150 rad) in the delay phase.
+// Without packed FP32, an FP32 FMA issues at the FP64 rate on gfx950, so FP32 pays only in
+// the transcendental units; FP64 algebra also keeps per-row constants (polarization
+// factors, 1/d_L, node values) from carrying an FP32 rounding that would bias every bin
+// of a row the same way.
 //
 // Algebraic rearrangements relative to antenna.py:496-543 (same values):
-//   * sinc(xL(1 +- k.n)): sin(xL +- a) = sin xL cos a +- cos xL sin a, a = xL k.n, with
-//     sin xL, cos xL per bin and one sincos per link pair;
-//   * exp(-i pi f (L + k.(xi + xj))) = exp(-i pi f L) exp(-2 pi i f k.x0)
-//     exp(-i pi f k.xi_det) exp(-i pi f k.xj_det): one large-argument sincos per
-//     (bin, mode, row), three small ones, and exp(-i pi f L) per bin;
+//   * the constellation offsets sum to zero, so k.(x_i + x_j) = 2 k.x0 - k.xd_a for link
+//     pair a with nodes (i, j) and third node a: exp(-i pi f (L + k.(x_i + x_j))) =
+//     exp(-i pi f L) exp(-2 pi i f k.x0) exp(+i pi f k.xd_a), one small sincos per pair;
+//   * sinc(xL(1 +- k.n)): sin(xL +- a) = sin xL cos a +- cos xL sin a, a = xL k.n;
+//   * n_a.v = (xd_i.v - xd_j.v)/L for v in {p, q, k}: nine scalar products per mode;
 //   * n.e+.n = (n.p)^2 - (n.q)^2, n.ex.n = 2 (n.p)(n.q) with e+ = pp - qq, ex = pq + qp.
 // This header is included inside the anonymous namespace of pespace_amd.hip.
 
-template <class R>
 struct Cx {
-  R re, im;
+  double re, im;
 };
-template <class R>
-__device__ inline Cx<R> mul(Cx<R> a, Cx<R> b) {
-  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-template <class R>
-__device__ inline Cx<R> add(Cx<R> a, Cx<R> b) {
-  return {a.re + b.re, a.im + b.im};
-}
-template <class R>
-__device__ inline Cx<R> sub(Cx<R> a, Cx<R> b) {
-  return {a.re - b.re, a.im - b.im};
-}
-template <class R>
-__device__ inline Cx<R> scale(Cx<R> a, R s) {
-  return {a.re * s, a.im * s};
-}
-template <class R>
-__device__ inline Cx<R> conjx(Cx<R> a) {
-  return {a.re, -a.im};
-}
-template <class R>
+__device__ inline Cx mul(Cx a, Cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ inline Cx add(Cx a, Cx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ inline Cx sub(Cx a, Cx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ inline Cx scale(Cx a, double s) { return {a.re * s, a.im * s}; }
+__device__ inline Cx conjx(Cx a) { return {a.re, -a.im}; }
+__device__ inline Cx axpy(double s, Cx a, Cx y) { return {fma(s, a.re, y.re), fma(s, a.im, y.im)}; }
+
 struct V3 {
-  R x, y, z;
+  double x, y, z;
 };
-template <class R>
-__device__ inline R dot3(const V3<R>& a, const V3<R>& b) {
-  return a.x * b.x + a.y * b.y + a.z * b.z;
+__device__ inline double dot3(const V3& a, const V3& b) { return fma(a.x, b.x, fma(a.y, b.y, a.z * b.z)); }
+__device__ inline void axpy3(double s, const V3& a, V3& y) {
+  y.x = fma(s, a.x, y.x);
+  y.y = fma(s, a.y, y.y);
+  y.z = fma(s, a.z, y.z);
 }
 
-// sin/cos of a large FP64 argument, result in R.  FP32: reduced to revolutions in FP64,
-// then the hardware v_sin_f32 / v_cos_f32 (which take revolutions).
-__device__ inline void sc_big(double x, double* s, double* c) { sincos_d(x, s, c); }
-__device__ inline void sc_big(double x, float* s, float* c) {
-  double r = x * 0.15915494309189533577;  // 1/(2 pi)
-  r -= rint(r);
-  float rf = (float)r;
-  *s = __builtin_amdgcn_sinf(rf);
-  *c = __builtin_amdgcn_cosf(rf);
-}
-// sin/cos of a modest argument held in R
-__device__ inline void sc_mod(double x, double* s, double* c) { sincos_d(x, s, c); }
-__device__ inline void sc_mod(float x, float* s, float* c) {
-  float r = x * 0.15915494309189533577f;
-  r -= rintf(r);
-  *s = __builtin_amdgcn_sinf(r);
-  *c = __builtin_amdgcn_cosf(r);
-}
-__device__ inline double rcp(double x) { return 1.0 / x; }
-__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+template <bool MIXED>
+struct Trig {
+  // any argument
+  __device__ static void big(double x, double* s, double* c) {
+    if constexpr (MIXED)
+      sincos_rev_f(x, s, c);
+    else
+      sincos_d(x, s, c);
+  }
+  // |x| <= 4 (the host checks pi f_max L <= 4 per detector)
+  __device__ static void small(double x, double* s, double* c) {
+    if constexpr (MIXED)
+      sincos_rev_small_f(x, s, c);
+    else
+      sincos_small_d(x, s, c);
+  }
+  __device__ static double rcp(double x) {
+    if constexpr (MIXED)
+      return rcp_f(x);
+    else
+      return rcp_d(x);
+  }
+};
 
 // per-bin quantities shared by every row and mode of a tile lane
-template <class R>
 struct BinPre {
-  double f, s, du, inv2pidf, pif;
+  double f, pif, xL, sxL, cxL;
+  Cx eL;       // exp(-i pi f L)
+  Cx D, pf;    // TDI delay exp(-2 pi i f L) and generation prefactor
+  Cx pfT;      // T-channel factor pf (1 - D) / sqrt(3)
+  // cubic-Hermite weights on [node j, node j+1] at this bin, u = ln f, s = (u - u_j)/du:
+  //   value = v0 + hb1 (v1 - v0) + hb2 v0' + hb3 v1'      (v' = dv/du)
+  //   t_f   = he1 (P1 - P0) + he2 P0' + he3 P1'            (dPsi/df / 2 pi)
+  double hb1, hb2, hb3, he1, he2, he3;
   int j;
-  R xL, sxL, cxL;
-  Cx<R> eL;        // exp(-i pi f L)
-  Cx<R> D, pf;     // TDI delay exp(-2 pi i f L) and generation prefactor
-  Cx<R> pfT;       // T-channel factor pf (1 - D) / sqrt(3)
 };
 
-// TDI constants are formed in FP64 (1 - D^n cancels at low f) and then rounded to R
-template <class R>
-__device__ inline void bin_prepare(const DetectorDev& det, double f, double s, int j, double du, BinPre<R>& b) {
+// TDI constants are formed in FP64 from their product forms (1 - D^n cancels at low f)
+__device__ inline void bin_prepare(const DetectorDev& det, double f, double s, int j, double du, BinPre& b) {
   b.f = f;
-  b.s = s;
   b.j = j;
-  b.du = du;
-  b.inv2pidf = 1.0 / (2.0 * PI * du * f);
   b.pif = PI * f;
+  const double inv2pidf = 1.0 / (2.0 * PI * du * f);
+  const double s1 = 1.0 - s;
+  b.hb1 = s * s * (3.0 - 2.0 * s);
+  b.hb2 = s * s1 * s1 * du;
+  b.hb3 = -s * s * s1 * du;
+  b.he1 = 6.0 * s * s1 * inv2pidf;
+  b.he2 = s1 * (1.0 - 3.0 * s) * du * inv2pidf;
+  b.he3 = s * (3.0 * s - 2.0) * du * inv2pidf;
   const double xL = PI * f * det.L;
   double sx, cx;
   sincos_d(xL, &sx, &cx);
-  b.xL = (R)xL;
-  b.sxL = (R)sx;
-  b.cxL = (R)cx;
-  b.eL = {(R)cx, (R)-sx};
+  b.xL = xL;
+  b.sxL = sx;
+  b.cxL = cx;
+  b.eL = {cx, -sx};
   // D = exp(-2 i xL);  1 - D^n = 2 i sin(n xL) exp(-i n xL)
   const double s2 = 2.0 * sx * cx, c2 = cx * cx - sx * sx;
-  b.D = {(R)c2, (R)-s2};
-  double pfr, pfi;  // prefactor, from its product form
-  {
-    // 1 - D^2 = 2 i sin(2 xL) exp(-2 i xL) = 2 sin(2xL) (sin(2xL) + i cos(2xL))
-    double a_re = 2.0 * s2 * s2, a_im = 2.0 * s2 * c2;
-    pfr = a_re;
-    pfi = a_im;
-    if (det.tdi_gen == 20) {  // (1 - D^2)(1 - D^4), 1 - D^4 = 2 i sin(4 xL) exp(-4 i xL)
-      double s8, c8;
-      sincos_d(4.0 * xL, &s8, &c8);
-      double b_re = 2.0 * s8 * s8, b_im = 2.0 * s8 * c8;
-      double r = pfr * b_re - pfi * b_im, i = pfr * b_im + pfi * b_re;
-      pfr = r;
-      pfi = i;
-    }
+  b.D = {c2, -s2};
+  // 1 - D^2 = 2 i sin(2xL) exp(-2 i xL) = 2 sin(2xL) (sin(2xL) + i cos(2xL))
+  double pfr = 2.0 * s2 * s2, pfi = 2.0 * s2 * c2;
+  if (det.tdi_gen == 20) {  // (1 - D^2)(1 - D^4)
+    double s8, c8;
+    sincos_d(4.0 * xL, &s8, &c8);
+    const double b_re = 2.0 * s8 * s8, b_im = 2.0 * s8 * c8;
+    const double r = pfr * b_re - pfi * b_im, i = pfr * b_im + pfi * b_re;
+    pfr = r;
+    pfi = i;
   }
-  b.pf = {(R)pfr, (R)pfi};
+  b.pf = {pfr, pfi};
   // pf (1 - D) / sqrt(3),  1 - D = 2 i sin(xL) exp(-i xL)
-  {
-    double o_re = 2.0 * sx * sx, o_im = 2.0 * sx * cx;
-    const double r3 = 0.57735026918962576451;
-    b.pfT = {(R)((pfr * o_re - pfi * o_im) * r3), (R)((pfr * o_im + pfi * o_re) * r3)};
-  }
+  const double o_re = 2.0 * sx * sx, o_im = 2.0 * sx * cx;
+  const double r3 = 0.57735026918962576451;
+  b.pfT = {(pfr * o_re - pfi * o_im) * r3, (pfr * o_im + pfi * o_re) * r3};
 }
 
-// orbit at time t: link vectors n_a and detector-frame node positions (R); the barycentre
-// term k.x0 and its derivatives in FP64; with DT also the time derivatives
-struct Bary {
-  double kx0, kx0_t, sa, ca;
+// one row's extrinsic constants
+struct RowC {
+  double tc;
+  V3 p, q, k;
 };
-template <class R, bool DT>
-__device__ inline void orbit_eval(const DetectorDev& d, double t, const V3<R>& k, V3<R> n[3], V3<R> xd[3],
-                                  V3<R> nd[3], V3<R> xdd[3], Bary& bc) {
-  const double omega = 2.0 * PI / YEAR_SI;
-  double sa, ca;
-  sincos_d(omega * t + d.rev0, &sa, &ca);
-  bc.sa = sa;
-  bc.ca = ca;
-  bc.kx0 = AU_SEC * ((double)k.x * ca + (double)k.y * sa);
-  bc.kx0_t = AU_SEC * omega * (-(double)k.x * sa + (double)k.y * ca);
-  const R saf = (R)sa, caf = (R)ca;
-  if (d.kind == 0) {
-    const R rp = (R)d.rp;
-    R sc = saf * caf, sa2 = (R)1 + saf * saf, ca2 = (R)1 + caf * caf, c2a = caf * caf - saf * saf;
+__device__ inline RowC load_row(const double* __restrict__ pc) {
+  RowC r;
+  r.tc = pc[PC_TC];
+  r.p = {pc[PC_P], pc[PC_P + 1], pc[PC_P + 2]};
+  r.q = {pc[PC_Q], pc[PC_Q + 1], pc[PC_Q + 2]};
+  r.k = {pc[PC_K], pc[PC_K + 1], pc[PC_K + 2]};
+  return r;
+}
+
+// One mode at one bin: waveform, orbit and the nine projections of the constellation
+struct ModeG {
+  double A, t;      // amplitude, t_f + t_c
+  Cx u, hm, hp, hc; // unit phasor, A u, F+ A u, Fx A u
+  double sa, ca;    // orbit angle (heliocentric revolution)
+  double sr, cr;    // geocentric rotation angle of node 1
+  double kx0;
+  V3 xd[3];
+  double px[3], qx[3], kx[3];
+};
+
+constexpr double kOmegaYear = 2.0 * PI / YEAR_SI;
+
+template <bool MIXED>
+__device__ inline void mode_geom(const DetectorDev& det, const double* __restrict__ mf,
+                                 const double* __restrict__ n0, const BinPre& b, const RowC& rc, Cx etc, ModeG& g) {
+  const double A0 = n0[0], dA0 = n0[1], P0 = n0[2], dP0 = n0[3];
+  const double A1 = n0[4], dA1 = n0[5], P1 = n0[6], dP1 = n0[7];
+  g.A = A0 + fma(b.hb1, A1 - A0, fma(b.hb2, dA0, b.hb3 * dA1));
+  const double dP = P1 - P0;
+  const double Psi = P0 + fma(b.hb1, dP, fma(b.hb2, dP0, b.hb3 * dP1));
+  g.t = fma(b.he1, dP, fma(b.he2, dP0, b.he3 * dP1)) + rc.tc;
+  sincos_d(fma(kOmegaYear, g.t, det.rev0), &g.sa, &g.ca);
+  const double sa = g.sa, ca = g.ca;
+  g.kx0 = AU_SEC * fma(rc.k.x, ca, rc.k.y * sa);
+  if (det.kind == 0) {
+    const double sc = sa * ca, sa2 = fma(sa, sa, 1.0), ca2 = fma(ca, ca, 1.0);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const R ck = (R)d.ck[i], sk = (R)d.sk[i];
-      xd[i].x = rp * (sc * sk - sa2 * ck);
-      xd[i].y = rp * (sc * ck - ca2 * sk);
-      xd[i].z = rp * ((R)(-SQRT3) * (caf * ck + saf * sk));
-      if (DT) {
-        const R rw = (R)(d.rp * omega);
-        xdd[i].x = rw * (c2a * sk - (R)2 * sc * ck);
-        xdd[i].y = rw * (c2a * ck + (R)2 * sc * sk);
-        xdd[i].z = rw * ((R)(-SQRT3) * (caf * sk - saf * ck));
-      }
+      g.xd[i].x = fma(sc, det.hsk[i], -sa2 * det.hck[i]);
+      g.xd[i].y = fma(sc, det.hck[i], -ca2 * det.hsk[i]);
+      g.xd[i].z = fma(ca, det.hzc[i], sa * det.hzs[i]);
     }
+    g.sr = 0.0;
+    g.cr = 0.0;
   } else {
-    R s1, c1;
-    sc_big(d.omega_rot * t + d.rot0, &s1, &c1);
-    const R cr = (R)-0.5, sr = (R)(0.5 * SQRT3);
-    R sk[3], ck[3];
+    double s1, c1;
+    Trig<MIXED>::big(fma(det.omega_rot, g.t, det.rot0), &s1, &c1);
+    g.sr = s1;
+    g.cr = c1;
+    const double cr = -0.5, sr = 0.5 * SQRT3;
+    double sk[3], ck[3];
     sk[0] = s1;
     ck[0] = c1;
-    sk[1] = sk[0] * cr + ck[0] * sr;
-    ck[1] = ck[0] * cr - sk[0] * sr;
-    sk[2] = sk[1] * cr + ck[1] * sr;
-    ck[2] = ck[1] * cr - sk[1] * sr;
-    const R rp = (R)d.rp, sb = (R)d.sbet, cb = (R)d.cbet, sl = (R)d.slam, cl = (R)d.clam;
+    sk[1] = fma(sk[0], cr, ck[0] * sr);
+    ck[1] = fma(ck[0], cr, -sk[0] * sr);
+    sk[2] = fma(sk[1], cr, ck[1] * sr);
+    ck[2] = fma(ck[1], cr, -sk[1] * sr);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      xd[i].x = rp * (sb * cl * sk[i] + sl * ck[i]);
-      xd[i].y = rp * (sb * sl * sk[i] - cl * ck[i]);
-      xd[i].z = rp * (-cb * sk[i]);
-      if (DT) {
-        const R rw = (R)(d.rp * d.omega_rot);
-        xdd[i].x = rw * (sb * cl * ck[i] - sl * sk[i]);
-        xdd[i].y = rw * (sb * sl * ck[i] + cl * sk[i]);
-        xdd[i].z = rw * (-cb * ck[i]);
-      }
+      g.xd[i].x = fma(det.gxs, sk[i], det.gxc * ck[i]);
+      g.xd[i].y = fma(det.gys, sk[i], det.gyc * ck[i]);
+      g.xd[i].z = det.gzs * sk[i];
     }
   }
-  const R invL = (R)(1.0 / d.L);
-  const int ia[3] = {2, 0, 1}, ib[3] = {1, 2, 0};  // n1 = x3 - x2, n2 = x1 - x3, n3 = x2 - x1
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    n[a] = {(xd[ia[a]].x - xd[ib[a]].x) * invL, (xd[ia[a]].y - xd[ib[a]].y) * invL,
-            (xd[ia[a]].z - xd[ib[a]].z) * invL};
-    if (DT)
-      nd[a] = {(xdd[ia[a]].x - xdd[ib[a]].x) * invL, (xdd[ia[a]].y - xdd[ib[a]].y) * invL,
-               (xdd[ia[a]].z - xdd[ib[a]].z) * invL};
+  for (int i = 0; i < 3; ++i) {
+    g.px[i] = dot3(rc.p, g.xd[i]);
+    g.qx[i] = dot3(rc.q, g.xd[i]);
+    g.kx[i] = dot3(rc.k, g.xd[i]);
+  }
+  if constexpr (MIXED) {
+    double sp, cp;
+    Trig<true>::big(Psi + 2.0 * b.pif * (rc.tc + g.kx0), &sp, &cp);
+    g.u = {cp, -sp};
+  } else {
+    double sp, cp, s0, c0;
+    sincos_d(Psi, &sp, &cp);
+    sincos_d(-2.0 * b.pif * g.kx0, &s0, &c0);
+    g.u = mul(mul(Cx{cp, -sp}, etc), Cx{c0, s0});
+  }
+  g.hm = scale(g.u, g.A);
+  g.hp = mul(Cx{mf[MF_FPR], mf[MF_FPI]}, g.hm);
+  g.hc = mul(Cx{mf[MF_FCR], mf[MF_FCI]}, g.hm);
+}
+
+// time derivatives of the constellation offsets
+__device__ inline void xd_dot(const DetectorDev& det, const ModeG& g, V3 xt[3]) {
+  if (det.kind == 0) {
+    const double sa = g.sa, ca = g.ca;
+    const double w = kOmegaYear;
+    const double sc2 = 2.0 * sa * ca, c2a = (ca - sa) * (ca + sa);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      xt[i].x = w * fma(c2a, det.hsk[i], -sc2 * det.hck[i]);
+      xt[i].y = w * fma(c2a, det.hck[i], sc2 * det.hsk[i]);
+      xt[i].z = w * fma(ca, det.hzs[i], -sa * det.hzc[i]);
+    }
+  } else {
+    const double cr = -0.5, sr = 0.5 * SQRT3;
+    double sk[3], ck[3];
+    sk[0] = g.sr;
+    ck[0] = g.cr;
+    sk[1] = fma(sk[0], cr, ck[0] * sr);
+    ck[1] = fma(ck[0], cr, -sk[0] * sr);
+    sk[2] = fma(sk[1], cr, ck[1] * sr);
+    ck[2] = fma(ck[1], cr, -sk[1] * sr);
+    const double w = det.omega_rot;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      xt[i].x = w * fma(det.gxs, ck[i], -det.gxc * sk[i]);
+      xt[i].y = w * fma(det.gys, ck[i], -det.gyc * sk[i]);
+      xt[i].z = w * det.gzs * ck[i];
+    }
   }
 }
 
-// sinc(x +- a) at x = xL, a = xL k.n, their derivatives, and delta = sinc(x-a) - sinc(x+a).
-// delta is the T-channel weight; in FP32 it is taken from
+// sinc(xL +- a) with a = xL k.n, their derivatives, and delta = sinc(xL-a) - sinc(xL+a).
+// In the mixed path delta is taken for xL < 0.3 from
 //   delta = 2 x a sum_{m,n>=0} (-1)^{m+n} X^m A^n / ((2m+1)! (2n+1)! (2m+2n+3)),  X = x^2, A = a^2
-// (from sinc y = int_0^1 cos(y t) dt, so delta = 2 int_0^1 sin(x t) sin(a t) dt), which has no
-// cancellation; used for x < 0.3 where the direct difference loses precision.
-template <class R>
-__device__ inline void sinc_pair(R xL, R sxL, R cxL, R kn, R& sp, R& sm, R& dsp, R& dsm, R& delta) {
-  R a = xL * kn;
-  R sa, ca;
-  sc_mod(a, &sa, &ca);
-  R xp = xL + a, xm = xL - a;
-  R sinp = sxL * ca + cxL * sa, cosp = cxL * ca - sxL * sa;
-  R sinm = sxL * ca - cxL * sa, cosm = cxL * ca + sxL * sa;
-  const R thr = (R)1e-2;
-  if (fabs(xp) < thr) {
-    R x2 = xp * xp;
-    sp = (R)1 - x2 * ((R)(1.0 / 6.0) - x2 * ((R)(1.0 / 120.0) - x2 * (R)(1.0 / 5040.0)));
-    dsp = -xp * ((R)(1.0 / 3.0) - x2 * ((R)(1.0 / 30.0) - x2 * (R)(1.0 / 840.0)));
-  } else {
-    R ip = rcp(xp);
+// (sinc y = int_0^1 cos(y t) dt, so delta = 2 int_0^1 sin(x t) sin(a t) dt), which has no
+// cancellation; the direct difference would lose the FP32 sines' accuracy there.
+template <bool MIXED>
+__device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, double& sp, double& sm, double& dsp,
+                                 double& dsm, double& delta) {
+  const double a = xL * kn;
+  double sa, ca;
+  Trig<MIXED>::small(a, &sa, &ca);
+  const double xp = xL + a, xm = xL - a;
+  const double sinp = fma(sxL, ca, cxL * sa), cosp = fma(cxL, ca, -sxL * sa);
+  const double sinm = fma(sxL, ca, -cxL * sa), cosm = fma(cxL, ca, sxL * sa);
+  const double thr = 1e-2;
+  const bool small_p = fabs(xp) < thr, small_m = fabs(xm) < thr;
+  if (!small_p && !small_m) {
+    const double inv = Trig<MIXED>::rcp(xp * xm);
+    const double ip = xm * inv, im = xp * inv;
     sp = sinp * ip;
     dsp = (cosp - sp) * ip;
-  }
-  if (fabs(xm) < thr) {
-    R x2 = xm * xm;
-    sm = (R)1 - x2 * ((R)(1.0 / 6.0) - x2 * ((R)(1.0 / 120.0) - x2 * (R)(1.0 / 5040.0)));
-    dsm = -xm * ((R)(1.0 / 3.0) - x2 * ((R)(1.0 / 30.0) - x2 * (R)(1.0 / 840.0)));
-  } else {
-    R im = rcp(xm);
     sm = sinm * im;
     dsm = (cosm - sm) * im;
+  } else {
+    if (small_p) {
+      const double x2 = xp * xp;
+      sp = 1.0 - x2 * (1.0 / 6.0 - x2 * (1.0 / 120.0 - x2 * (1.0 / 5040.0)));
+      dsp = -xp * (1.0 / 3.0 - x2 * (1.0 / 30.0 - x2 * (1.0 / 840.0)));
+    } else {
+      const double ip = Trig<MIXED>::rcp(xp);
+      sp = sinp * ip;
+      dsp = (cosp - sp) * ip;
+    }
+    if (small_m) {
+      const double x2 = xm * xm;
+      sm = 1.0 - x2 * (1.0 / 6.0 - x2 * (1.0 / 120.0 - x2 * (1.0 / 5040.0)));
+      dsm = -xm * (1.0 / 3.0 - x2 * (1.0 / 30.0 - x2 * (1.0 / 840.0)));
+    } else {
+      const double im = Trig<MIXED>::rcp(xm);
+      sm = sinm * im;
+      dsm = (cosm - sm) * im;
+    }
   }
-  if (sizeof(R) == sizeof(float) && xL < (R)0.3) {
-    const R X = xL * xL, A = a * a;
+  if (MIXED && xL < 0.3) {
+    const double X = xL * xL, A = a * a;
     // q_m(A) = sum_n c_mn A^n, total degree <= 3
-    const R q0 = (R)(1.0 / 3.0) + A * ((R)(-1.0 / 30.0) + A * ((R)(1.0 / 840.0) + A * (R)(-1.0 / 45360.0)));
-    const R q1 = (R)(-1.0 / 30.0) + A * ((R)(1.0 / 252.0) + A * (R)(-1.0 / 6480.0));
-    const R q2 = (R)(1.0 / 840.0) + A * (R)(-1.0 / 6480.0);
-    const R q3 = (R)(-1.0 / 45360.0);
-    delta = (R)2 * xL * a * (q0 + X * (q1 + X * (q2 + X * q3)));
+    const double q0 = 1.0 / 3.0 + A * (-1.0 / 30.0 + A * (1.0 / 840.0 + A * (-1.0 / 45360.0)));
+    const double q1 = -1.0 / 30.0 + A * (1.0 / 252.0 + A * (-1.0 / 6480.0));
+    const double q2 = 1.0 / 840.0 + A * (-1.0 / 6480.0);
+    const double q3 = -1.0 / 45360.0;
+    delta = 2.0 * xL * a * (q0 + X * (q1 + X * (q2 + X * q3)));
   } else {
     delta = sm - sp;
   }
 }
 
-template <class R>
+// link pair a: link vector n_a = (xd[ia] - xd[ib]) / L with (ia, ib) = (2,1), (0,2), (1,0),
+// third node a; links 0:12 1:21 2:23 3:32 4:31 5:13; the (+) link lp = 2, 4, 0 gets
+// sinc(xL(1 + k.n_a)), the (-) link lp + 1 gets sinc(xL(1 - k.n_a))
+
+struct PairV {
+  double np, nq, kn, Pp, Pc;
+  Cx E, C;
+  double sp, sm, dsp, dsm, dl;
+};
+
+template <bool MIXED>
+__device__ inline void pair_eval(const DetectorDev& det, const BinPre& b, const ModeG& g, int a, PairV& v) {
+  const int ia = a == 0 ? 2 : a - 1;
+  const int ib = a == 0 ? 1 : (a == 1 ? 2 : 0);
+  v.np = (g.px[ia] - g.px[ib]) * det.invL;
+  v.nq = (g.qx[ia] - g.qx[ib]) * det.invL;
+  v.kn = (g.kx[ia] - g.kx[ib]) * det.invL;
+  v.Pp = (v.np - v.nq) * (v.np + v.nq);
+  v.Pc = 2.0 * v.np * v.nq;
+  const Cx nhn = {fma(v.Pp, g.hp.re, v.Pc * g.hc.re), fma(v.Pp, g.hp.im, v.Pc * g.hc.im)};
+  double s_, c_;
+  Trig<MIXED>::small(b.pif * g.kx[a], &s_, &c_);
+  v.E = mul(b.eL, Cx{c_, s_});
+  const Cx T = mul(nhn, v.E);
+  v.C = {b.xL * T.im, -b.xL * T.re};  // (-i xL) nhn E
+  sinc_pair<MIXED>(b.xL, b.sxL, b.cxL, v.kn, v.sp, v.sm, v.dsp, v.dsm, v.dl);
+}
+
 struct LinksT {
-  Cx<R> l[6];  // 12, 21, 23, 32, 31, 13
-  Cx<R> tl;    // sum over pairs of delta_a C_a (T channel, before pf (1-D)/sqrt3)
+  Cx l[6];
+  Cx tl;  // sum over pairs of delta_a C_a (T channel, before pf (1-D)/sqrt3)
 };
-
-// rows' constants in R
-template <class R>
-struct RowC {
-  double tc;
-  V3<R> p, q, k;
-};
-
-template <class R>
-__device__ inline RowC<R> load_row(const double* __restrict__ pc) {
-  RowC<R> r;
-  r.tc = pc[PC_TC];
-  r.p = {(R)pc[PC_P], (R)pc[PC_P + 1], (R)pc[PC_P + 2]};
-  r.q = {(R)pc[PC_Q], (R)pc[PC_Q + 1], (R)pc[PC_Q + 2]};
-  r.k = {(R)pc[PC_K], (R)pc[PC_K + 1], (R)pc[PC_K + 2]};
-  return r;
-}
-
-// One mode at one bin: amplitude A, time t = tf + t_c (FP64) and the phasor
-//   u = exp(-i Psi) exp(-2 pi i f t_c) exp(-2 pi i f k.x0(t))
-// (FP64: three separate factors as in antenna.py:435, 530-533; FP32: one FP64-reduced phase)
-template <class R>
-struct ModeState {
-  R A;
-  double t;
-  Cx<R> u;
-};
-
-template <class R>
-__device__ inline void mode_state(const double* __restrict__ nodes, size_t nbase, const BinPre<R>& b,
-                                  const RowC<R>& rc, Cx<R> etc, const Bary* bc_in, ModeState<R>& ms,
-                                  double& Psi_out) {
-  const double* n0 = nodes + (nbase + b.j) * 4;
-  {
-    R P0 = (R)n0[0], P1 = (R)n0[4], M0 = (R)(n0[1] * b.du), M1 = (R)(n0[5] * b.du), s = (R)b.s;
-    R c2 = (R)3 * (P1 - P0) - (R)2 * M0 - M1;
-    R c3 = (R)2 * (P0 - P1) + M0 + M1;
-    ms.A = P0 + s * (M0 + s * (c2 + s * c3));
-  }
-  double P0 = n0[2], P1 = n0[6], M0 = n0[3] * b.du, M1 = n0[7] * b.du, s = b.s;
-  double c2 = 3.0 * (P1 - P0) - 2.0 * M0 - M1;
-  double c3 = 2.0 * (P0 - P1) + M0 + M1;
-  Psi_out = P0 + s * (M0 + s * (c2 + s * c3));
-  double dPsids = M0 + s * (2.0 * c2 + s * (3.0 * c3));
-  ms.t = dPsids * b.inv2pidf + rc.tc;
-  (void)bc_in;
-  (void)etc;
-}
-
-template <class R>
-__device__ inline void mode_phasor(const BinPre<R>& b, const RowC<R>& rc, Cx<R> etc, double Psi, const Bary& bc,
-                                   ModeState<R>& ms) {
-  if (sizeof(R) == sizeof(double)) {
-    R sp, cp, s0, c0;
-    sc_big(Psi, &sp, &cp);
-    sc_big(-2.0 * b.pif * bc.kx0, &s0, &c0);
-    ms.u = mul(mul(Cx<R>{cp, -sp}, etc), Cx<R>{c0, s0});
-  } else {
-    R sp, cp;
-    sc_big(Psi + 2.0 * b.pif * (rc.tc + bc.kx0), &sp, &cp);
-    ms.u = {cp, -sp};
-  }
-}
-
-// forward: add one mode's links
-template <class R>
-__device__ inline void mode_links(const DetectorDev& det, const double* __restrict__ pc, int mi,
-                                  const double* __restrict__ nodes, size_t nbase, const BinPre<R>& b,
-                                  const RowC<R>& rc, Cx<R> etc, LinksT<R>& L) {
-  const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
-  ModeState<R> ms;
-  double Psi;
-  mode_state<R>(nodes, nbase, b, rc, etc, nullptr, ms, Psi);
-  V3<R> n[3], xd[3], nd_[3], xdd_[3];
-  Bary bc;
-  orbit_eval<R, false>(det, ms.t, rc.k, n, xd, nd_, xdd_, bc);
-  mode_phasor<R>(b, rc, etc, Psi, bc, ms);
-  Cx<R> hm = scale(ms.u, ms.A);
-  Cx<R> hp = mul(Cx<R>{(R)mf[MF_FPR], (R)mf[MF_FPI]}, hm);
-  Cx<R> hc = mul(Cx<R>{(R)mf[MF_FCR], (R)mf[MF_FCI]}, hm);
-  Cx<R> ei[3];
-  const R mpif = (R)(-b.pif);
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    R s_, c_;
-    sc_mod(mpif * dot3(rc.k, xd[i]), &s_, &c_);
-    ei[i] = {c_, s_};
-  }
-  // pair a: link vector n[a]; (+) and (-) links; nodes (i, j)
-  const int lp[3] = {2, 4, 0}, lm[3] = {3, 5, 1};
-  const int pi_[3] = {1, 2, 0}, pj_[3] = {2, 0, 1};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    R np = dot3(n[a], rc.p), nq = dot3(n[a], rc.q);
-    R Pp = np * np - nq * nq, Pc = (R)2 * np * nq;
-    Cx<R> nhn = {Pp * hp.re + Pc * hc.re, Pp * hp.im + Pc * hc.im};
-    R kn = dot3(rc.k, n[a]);
-    R sp, sm, dsp, dsm, dl;
-    sinc_pair(b.xL, b.sxL, b.cxL, kn, sp, sm, dsp, dsm, dl);
-    Cx<R> E = mul(b.eL, mul(ei[pi_[a]], ei[pj_[a]]));
-    Cx<R> C = mul(Cx<R>{b.xL * nhn.im, -b.xL * nhn.re}, E);  // (-i xL) nhn E
-    L.l[lp[a]] = add(L.l[lp[a]], scale(C, sp));
-    L.l[lm[a]] = add(L.l[lm[a]], scale(C, sm));
-    L.tl = add(L.tl, scale(C, dl));
-  }
-}
 
 // TDI outputs: A, E from X, Y, Z (tdi.py:1784-1785); T from the pair sum, which equals
 // (X + Y + Z)/sqrt3 = pf (1 - D) sum_a (sinc_- - sinc_+)_a C_a / sqrt3 without cancellation
-template <class R>
-__device__ inline void tdi_out(const DetectorDev& det, const BinPre<R>& b, const LinksT<R>& L, Cx<R> out[3]) {
-  const Cx<R>* l = L.l;
-  Cx<R> X = mul(b.pf, add(sub(l[5], l[0]), mul(b.D, sub(l[4], l[1]))));
-  Cx<R> Y = mul(b.pf, add(sub(l[1], l[2]), mul(b.D, sub(l[0], l[3]))));
-  Cx<R> Z = mul(b.pf, add(sub(l[3], l[4]), mul(b.D, sub(l[2], l[5]))));
+__device__ inline void tdi_out(const DetectorDev& det, const BinPre& b, const LinksT& L, Cx out[3]) {
+  const Cx* l = L.l;
+  const Cx X = mul(b.pf, add(sub(l[5], l[0]), mul(b.D, sub(l[4], l[1]))));
+  const Cx Y = mul(b.pf, add(sub(l[1], l[2]), mul(b.D, sub(l[0], l[3]))));
+  const Cx Z = mul(b.pf, add(sub(l[3], l[4]), mul(b.D, sub(l[2], l[5]))));
   if (det.orthogonal) {
-    const R r2 = (R)0.70710678118654752440, r6 = (R)0.40824829046386301637;
+    const double r2 = 0.70710678118654752440, r6 = 0.40824829046386301637;
     out[0] = scale(sub(Z, X), r2);
-    out[1] = scale(add(sub(X, scale(Y, (R)2)), Z), r6);
+    out[1] = scale(add(sub(X, scale(Y, 2.0)), Z), r6);
     out[2] = mul(b.pfT, L.tl);
   } else {
     out[0] = X;
@@ -371,152 +349,160 @@ __device__ inline void tdi_out(const DetectorDev& det, const BinPre<R>& b, const
   }
 }
 
-template <class R>
-__device__ inline Cx<R> row_etc(const BinPre<R>& b, double tc) {
-  Cx<R> etc = {(R)1, (R)0};
-  if (sizeof(R) == sizeof(double)) {
-    R s, c;
-    sc_big(-2.0 * b.pif * tc, &s, &c);
+template <bool MIXED>
+__device__ inline Cx row_etc(const BinPre& b, double tc) {
+  Cx etc = {1.0, 0.0};
+  if constexpr (!MIXED) {
+    double s, c;
+    sincos_d(-2.0 * b.pif * tc, &s, &c);
     etc = {c, s};
   }
   return etc;
 }
 
-template <class R>
+template <bool MIXED>
 __device__ inline void response_row(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
-                                    const double* __restrict__ nodes, size_t pbase, int NNS, const BinPre<R>& b,
-                                    Cx<R> out[3]) {
-  RowC<R> rc = load_row<R>(pc);
-  Cx<R> etc = row_etc(b, rc.tc);
-  LinksT<R> L;
+                                    const double* __restrict__ nodes, size_t pbase, int NNS, const BinPre& b,
+                                    const RowC& rc, Cx etc, Cx out[3]) {
+  LinksT L;
 #pragma unroll
-  for (int i = 0; i < 6; ++i) L.l[i] = {(R)0, (R)0};
-  L.tl = {(R)0, (R)0};
+  for (int i = 0; i < 6; ++i) L.l[i] = {0.0, 0.0};
+  L.tl = {0.0, 0.0};
   for (int mi = 0; mi < modes.n; ++mi) {
-    if (b.f > pc[PC_MODE + mi * PC_MODE_STRIDE + MF_CUT]) continue;
-    mode_links<R>(det, pc, mi, nodes, pbase + (size_t)mi * NNS, b, rc, etc, L);
+    const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
+    if (b.f > mf[MF_CUT]) continue;
+    ModeG g;
+    mode_geom<MIXED>(det, mf, nodes + (pbase + (size_t)mi * NNS + b.j) * 4, b, rc, etc, g);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      PairV v;
+      pair_eval<MIXED>(det, b, g, a, v);
+      const int lp = a == 0 ? 2 : (a == 1 ? 4 : 0), lm = lp + 1;
+      L.l[lp] = axpy(v.sp, v.C, L.l[lp]);
+      L.l[lm] = axpy(v.sm, v.C, L.l[lm]);
+      L.tl = axpy(v.dl, v.C, L.tl);
+    }
   }
   tdi_out(det, b, L, out);
 }
 
 // ------------------------------- adjoint ---------------------------------------------------
-// wl: link adjoints from the A,E (or X,Y,Z) channels; wT: adjoint of the T pair sum
-template <class R>
-__device__ inline void mode_adjoint(const DetectorDev& det, const double* __restrict__ pc, int mi, int mid,
-                                    const double* __restrict__ nodes, const double* __restrict__ nodes_grad,
-                                    size_t nbase, const BinPre<R>& b, const RowC<R>& rc, Cx<R> etc,
-                                    const Cx<R> wl[6], Cx<R> wT, double G[PES_N_PARAMS]) {
-  const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
-  ModeState<R> ms;
-  double Psi;
-  mode_state<R>(nodes, nbase, b, rc, etc, nullptr, ms, Psi);
-  V3<R> n[3], xd[3], nd[3], xdd[3];
-  Bary bc;
-  orbit_eval<R, true>(det, ms.t, rc.k, n, xd, nd, xdd, bc);
-  mode_phasor<R>(b, rc, etc, Psi, bc, ms);
-  Cx<R> hm = scale(ms.u, ms.A);
-  Cx<R> Fp = {(R)mf[MF_FPR], (R)mf[MF_FPI]}, Fc = {(R)mf[MF_FCR], (R)mf[MF_FCI]};
-  Cx<R> hp = mul(Fp, hm), hc = mul(Fc, hm);
-  Cx<R> ei[3];
-  const R mpif = (R)(-b.pif);
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    R s_, c_;
-    sc_mod(mpif * dot3(rc.k, xd[i]), &s_, &c_);
-    ei[i] = {c_, s_};
-  }
-  const V3<R> dlp = {(R)pc[PC_DLP], (R)pc[PC_DLP + 1], (R)pc[PC_DLP + 2]};
-  const V3<R> dlq = {(R)pc[PC_DLQ], (R)pc[PC_DLQ + 1], (R)pc[PC_DLQ + 2]};
-  const V3<R> dbp = {(R)pc[PC_DBP], (R)pc[PC_DBP + 1], (R)pc[PC_DBP + 2]};
-  const V3<R> dbq = {(R)pc[PC_DBQ], (R)pc[PC_DBQ + 1], (R)pc[PC_DBQ + 2]};
-  const V3<R> dlk = {(R)pc[PC_DLK], (R)pc[PC_DLK + 1], (R)pc[PC_DLK + 2]};
-  const V3<R> dbk = {(R)pc[PC_DBK], (R)pc[PC_DBK + 1], (R)pc[PC_DBK + 2]};
-  const int lp[3] = {2, 4, 0}, lm[3] = {3, 5, 1};
-  const int pi_[3] = {1, 2, 0}, pj_[3] = {2, 0, 1};
-  Cx<R> Hp = {(R)0, (R)0}, Hc = {(R)0, (R)0};
-  R tbar = (R)0, lbar = (R)0, bbar = (R)0, psbar = (R)0;
+// Per-(bin, row) adjoint accumulators.  pb, qb, kb: adjoints of the polarization vectors
+// and the propagation direction (contracted with their lambda/beta/psi derivatives once per
+// row); the rest are parameter adjoints.
+struct AdjAcc {
+  V3 pb, qb, kb;
+  double gint[4];  // Mc, q, chi1, chi2
+  double gh_re;    // sum Re(G h): d lnL / d d_L = -gh_re / d_L
+  double mphi;     // sum m phibar: d lnL / d phi_ref = -mphi
+  double iota, tc;
+};
+
+// wl: adjoints of the six links; wT: adjoint of the T pair sum; mid: mode id
+template <bool MIXED>
+__device__ inline void mode_adjoint(const DetectorDev& det, const double* __restrict__ mf, int mid,
+                                    const double* __restrict__ n0, const double* __restrict__ g0, const BinPre& b,
+                                    const RowC& rc, Cx etc, const Cx wl[6], Cx wT, AdjAcc& acc) {
+  ModeG g;
+  mode_geom<MIXED>(det, mf, n0, b, rc, etc, g);
+  double pxb[3] = {0.0, 0.0, 0.0}, qxb[3] = {0.0, 0.0, 0.0}, kxb[3] = {0.0, 0.0, 0.0};
+  Cx Hp = {0.0, 0.0}, Hc = {0.0, 0.0};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    R np = dot3(n[a], rc.p), nq = dot3(n[a], rc.q);
-    R Pp = np * np - nq * nq, Pc = (R)2 * np * nq;
-    Cx<R> nhn = {Pp * hp.re + Pc * hc.re, Pp * hp.im + Pc * hc.im};
-    R kn = dot3(rc.k, n[a]);
-    R sp, sm, dsp, dsm, dl;
-    sinc_pair(b.xL, b.sxL, b.cxL, kn, sp, sm, dsp, dsm, dl);
-    Cx<R> E = mul(b.eL, mul(ei[pi_[a]], ei[pj_[a]]));
-    Cx<R> C = mul(Cx<R>{b.xL * nhn.im, -b.xL * nhn.re}, E);
-    const Cx<R> wp = wl[lp[a]], wm = wl[lm[a]];
-    R gp = wp.re * C.re + wp.im * C.im;  // Re[conj(w) C]
-    R gm = wm.re * C.re + wm.im * C.im;
-    R gT = wT.re * C.re + wT.im * C.im;
-    R Sbar = b.xL * (dsp * gp - dsm * gm - (dsm + dsp) * gT);  // adjoint of k.n_a
-    Cx<R> W = {sp * wp.re + sm * wm.re + dl * wT.re, sp * wp.im + sm * wm.im + dl * wT.im};
-    Cx<R> cw = conjx(W);
-    Cx<R> V = mul(mul(cw, Cx<R>{(R)0, -b.xL}), E);  // conj(W) (-i xL) E
-    Cx<R> cwC = mul(cw, C);
-    R Tbar = (R)b.pif * cwC.im;  // adjoint of k.(xi_det + xj_det)
-    Cx<R> Vhp = mul(V, hp), Vhc = mul(V, hc);
-    R npbar = (R)2 * (np * Vhp.re + nq * Vhc.re);
-    R nqbar = (R)2 * (np * Vhc.re - nq * Vhp.re);
-    Hp.re += V.re * Pp;
-    Hp.im += V.im * Pp;
-    Hc.re += V.re * Pc;
-    Hc.im += V.im * Pc;
-    const V3<R>& xi = xd[pi_[a]];
-    const V3<R>& xj = xd[pj_[a]];
-    V3<R> X = {xi.x + xj.x, xi.y + xj.y, xi.z + xj.z};
-    V3<R> Xd = {xdd[pi_[a]].x + xdd[pj_[a]].x, xdd[pi_[a]].y + xdd[pj_[a]].y, xdd[pi_[a]].z + xdd[pj_[a]].z};
-    tbar += npbar * dot3(nd[a], rc.p) + nqbar * dot3(nd[a], rc.q) + Sbar * dot3(rc.k, nd[a]) +
-            Tbar * dot3(rc.k, Xd);
-    lbar += npbar * dot3(n[a], dlp) + nqbar * dot3(n[a], dlq) + Sbar * dot3(dlk, n[a]) + Tbar * dot3(dlk, X);
-    bbar += npbar * dot3(n[a], dbp) + nqbar * dot3(n[a], dbq) + Sbar * dot3(dbk, n[a]) + Tbar * dot3(dbk, X);
-    psbar += npbar * nq - nqbar * np;
+    PairV v;
+    pair_eval<MIXED>(det, b, g, a, v);
+    const int ia = a == 0 ? 2 : a - 1;
+    const int ib = a == 0 ? 1 : (a == 1 ? 2 : 0);
+    const int lp = a == 0 ? 2 : (a == 1 ? 4 : 0), lm = lp + 1;
+    const Cx wp = wl[lp], wm = wl[lm];
+    const double gp = fma(wp.re, v.C.re, wp.im * v.C.im);  // Re[conj(w) C]
+    const double gm = fma(wm.re, v.C.re, wm.im * v.C.im);
+    const double gT = fma(wT.re, v.C.re, wT.im * v.C.im);
+    const double Sbar = b.xL * (v.dsp * gp - v.dsm * gm - (v.dsm + v.dsp) * gT);  // adjoint of k.n_a
+    const Cx W = {fma(v.sp, wp.re, fma(v.sm, wm.re, v.dl * wT.re)), fma(v.sp, wp.im, fma(v.sm, wm.im, v.dl * wT.im))};
+    // adjoint of k.xd_a through exp(+i pi f k.xd_a): -pi f Im(conj(W) C)
+    kxb[a] -= b.pif * (W.re * v.C.im - W.im * v.C.re);
+    const Cx t1 = mul(conjx(W), v.E);
+    const Cx V = {b.xL * t1.im, -b.xL * t1.re};  // conj(W) (-i xL) E: adjoint of n.h.n
+    const double Ppb = V.re * g.hp.re - V.im * g.hp.im;  // Re(V h+)
+    const double Pcb = V.re * g.hc.re - V.im * g.hc.im;  // Re(V hx)
+    const double npb = 2.0 * fma(v.np, Ppb, v.nq * Pcb);
+    const double nqb = 2.0 * fma(v.np, Pcb, -v.nq * Ppb);
+    Hp = axpy(v.Pp, V, Hp);
+    Hc = axpy(v.Pc, V, Hc);
+    const double npl = npb * det.invL, nql = nqb * det.invL, knl = Sbar * det.invL;
+    pxb[ia] += npl;
+    pxb[ib] -= npl;
+    qxb[ia] += nql;
+    qxb[ib] -= nql;
+    kxb[ia] += knl;
+    kxb[ib] -= knl;
+  }
+  // constellation adjoints: p, q, k directly, and the time through the offsets
+  V3 xt[3];
+  xd_dot(det, g, xt);
+  double tgeo = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    axpy3(pxb[i], g.xd[i], acc.pb);
+    axpy3(qxb[i], g.xd[i], acc.qb);
+    axpy3(kxb[i], g.xd[i], acc.kb);
+    tgeo = fma(pxb[i], dot3(rc.p, xt[i]), tgeo);
+    tgeo = fma(qxb[i], dot3(rc.q, xt[i]), tgeo);
+    tgeo = fma(kxb[i], dot3(rc.k, xt[i]), tgeo);
   }
   // waveform-side adjoints: Gm = Hp Fp + Hc Fc, contributions Re[Gm dh_m]
-  Cx<R> Gm = add(mul(Hp, Fp), mul(Hc, Fc));
-  Cx<R> Gh = mul(Gm, hm);
-  const double phibar = (double)Gh.im;  // adjoint of the total phase Psi + 2 pi f (t_c + k.x0)
-  const double kx0bar = 2.0 * b.pif * phibar;
-  const double tb = (double)tbar + kx0bar * bc.kx0_t;
-  G[7] += (double)lbar + kx0bar * AU_SEC * ((double)dlk.x * bc.ca + (double)dlk.y * bc.sa);
-  G[8] += (double)bbar + kx0bar * AU_SEC * ((double)dbk.x * bc.ca + (double)dbk.y * bc.sa);
-  G[9] += (double)psbar;
-  const double m = (double)kModeM[mid];
-  G[4] += -(double)Gh.re * pc[PC_INVDL];
-  G[5] += -m * phibar;
+  const Cx Fp = {mf[MF_FPR], mf[MF_FPI]}, Fc = {mf[MF_FCR], mf[MF_FCI]};
+  const Cx Gm = add(mul(Hp, Fp), mul(Hc, Fc));
+  const Cx Gh = mul(Gm, g.hm);
+  const double phib = Gh.im;  // adjoint of the total phase Psi + 2 pi f (t_c + k.x0)
+  const double kx0b = 2.0 * b.pif * phib;
+  acc.kb.x = fma(kx0b * AU_SEC, g.ca, acc.kb.x);
+  acc.kb.y = fma(kx0b * AU_SEC, g.sa, acc.kb.y);
+  const double kx0_t = AU_SEC * kOmegaYear * fma(-rc.k.x, g.sa, rc.k.y * g.ca);
+  const double tb = fma(kx0b, kx0_t, tgeo);  // adjoint of t = t_f + t_c
+  acc.gh_re += Gh.re;
+  acc.mphi = fma((double)kModeM[mid], phib, acc.mphi);
   {
-    Cx<R> dFp = {(R)mf[MF_DFPR], (R)mf[MF_DFPI]}, dFc = {(R)mf[MF_DFCR], (R)mf[MF_DFCI]};
-    Cx<R> tt = mul(add(mul(Hp, dFp), mul(Hc, dFc)), hm);
-    G[6] += (double)tt.re;
+    const Cx dFp = {mf[MF_DFPR], mf[MF_DFPI]}, dFc = {mf[MF_DFCR], mf[MF_DFCI]};
+    const Cx tt = mul(add(mul(Hp, dFp), mul(Hc, dFc)), g.hm);
+    acc.iota += tt.re;
   }
-  G[10] += 2.0 * b.pif * phibar + tb;
-  Cx<R> Gu = mul(Gm, ms.u);
-  const R Abar = Gu.re, Psibar = (R)phibar;
-  const double* g0 = nodes_grad + (nbase + b.j) * 16;
-  const R s = (R)b.s;
-  const R rdu = (R)b.du;
-  const R tbi = (R)(tb * b.inv2pidf);
+  acc.tc += fma(2.0 * b.pif, phib, tb);
+  // intrinsic parameters through the node tables: weights of the eight node entries
+  const Cx Gu = mul(Gm, g.u);
+  const double Ab = Gu.re;
+  const double cA1 = Ab * b.hb1, cA0 = Ab - cA1, cdA0 = Ab * b.hb2, cdA1 = Ab * b.hb3;
+  const double tP = tb * b.he1;
+  const double cP1 = fma(phib, b.hb1, tP), cP0 = phib - cP1;
+  const double cdP0 = fma(phib, b.hb2, tb * b.he2), cdP1 = fma(phib, b.hb3, tb * b.he3);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    R a0 = (R)g0[4 * i + 0], a1 = (R)g0[16 + 4 * i + 0];
-    R am0 = (R)g0[4 * i + 1] * rdu, am1 = (R)g0[16 + 4 * i + 1] * rdu;
-    R p0 = (R)g0[4 * i + 2], p1 = (R)g0[16 + 4 * i + 2];
-    R pm0 = (R)g0[4 * i + 3] * rdu, pm1 = (R)g0[16 + 4 * i + 3] * rdu;
-    R ac2 = (R)3 * (a1 - a0) - (R)2 * am0 - am1, ac3 = (R)2 * (a0 - a1) + am0 + am1;
-    R pc2 = (R)3 * (p1 - p0) - (R)2 * pm0 - pm1, pc3 = (R)2 * (p0 - p1) + pm0 + pm1;
-    R dA = a0 + s * (am0 + s * (ac2 + s * ac3));
-    R dP = p0 + s * (pm0 + s * (pc2 + s * pc3));
-    R dPs = pm0 + s * ((R)2 * pc2 + s * ((R)3 * pc3));
-    G[i] += (double)(Abar * dA + Psibar * dP + tbi * dPs);
+    const double* q0 = g0 + 4 * i;
+    const double* q1 = g0 + 16 + 4 * i;
+    double s = cA0 * q0[0];
+    s = fma(cdA0, q0[1], s);
+    s = fma(cP0, q0[2], s);
+    s = fma(cdP0, q0[3], s);
+    s = fma(cA1, q1[0], s);
+    s = fma(cdA1, q1[1], s);
+    s = fma(cP1, q1[2], s);
+    s = fma(cdP1, q1[3], s);
+    acc.gint[i] += s;
   }
 }
 
 // ------------------------------- kernels ---------------------------------------------------
-template <class R, bool GRAD>
+// Persistent fused kernel.  Work item = (group of TILES_PER_GROUP tiles, chunk of CHUNK rows);
+// lane <-> bin.  A tile whose first frequency lies above every mode's cut of a row has a
+// zero response for that row: its lnL contribution is the precomputed -sum w |d|^2 of the
+// tile and its gradient contribution is zero, so the row skips it.
+template <bool MIXED, bool GRAD>
 __global__ __launch_bounds__(TILE) void k_lnl(const DetectorDev* __restrict__ dets, int n_det, int n_tiles,
                                               const double* __restrict__ bin_f, const int* __restrict__ bin_j,
                                               const double* __restrict__ bin_s, const double* __restrict__ bin_w,
                                               const double* __restrict__ dre, const double* __restrict__ dim, int NB,
+                                              const double* __restrict__ tile_f0, const double* __restrict__ tile_dd,
                                               const double* __restrict__ pconst, const double* __restrict__ nodes,
                                               const double* __restrict__ nodes_grad, int NNS, double du,
                                               ModeList modes, int P, double* __restrict__ partial) {
@@ -530,15 +516,25 @@ __global__ __launch_bounds__(TILE) void k_lnl(const DetectorDev* __restrict__ de
     const int g = w / n_chunks, c = w % n_chunks;
     const int p0 = c * CHUNK, p1 = min(P, p0 + CHUNK);
     for (int i = threadIdx.x; i < (TILE / 64) * CHUNK * NOUT; i += TILE) (&acc[0][0][0])[i] = 0.0;
+    double chunk_cut = 0.0;
+    for (int p = p0; p < p1; ++p) chunk_cut = fmax(chunk_cut, pconst[(size_t)p * PC_STRIDE + PC_FCUT]);
     __syncthreads();
     const int t_end = min(n_tiles, (g + 1) * TILES_PER_GROUP);
     for (int tile = g * TILES_PER_GROUP; tile < t_end; ++tile) {
+      const double f0 = tile_f0[tile];
+      if (f0 > chunk_cut) {  // no row of the chunk has a mode up here
+        if (threadIdx.x == 0) {
+          const double dd = tile_dd[tile];
+          for (int p = p0; p < p1; ++p) acc[0][p - p0][0] -= dd;
+        }
+        continue;
+      }
       const DetectorDev det = dets[tile_detector(dets, n_det, tile)];
       const int bi = tile * TILE + threadIdx.x;
-      BinPre<R> b;
-      bin_prepare<R>(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
+      BinPre b;
+      bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
       double wc[3];
-      Cplx dc[3];
+      Cx dc[3];
 #pragma unroll
       for (int ch = 0; ch < 3; ++ch) {
         wc[ch] = bin_w[ch * NB + bi];
@@ -546,27 +542,33 @@ __global__ __launch_bounds__(TILE) void k_lnl(const DetectorDev* __restrict__ de
       }
       for (int p = p0; p < p1; ++p) {
         const double* pc = pconst + (size_t)p * PC_STRIDE;
+        if (f0 > pc[PC_FCUT]) {
+          if (threadIdx.x == 0) acc[0][p - p0][0] -= tile_dd[tile];
+          continue;
+        }
         double G[PES_N_PARAMS];
 #pragma unroll
         for (int i = 0; i < PES_N_PARAMS; ++i) G[i] = 0.0;
         double v = 0.0;
         if (pc[PC_VALID] != 0.0) {
           const size_t pbase = (size_t)p * modes.n * NNS;
-          Cx<R> out[3];
-          response_row<R>(det, pc, modes, nodes, pbase, NNS, b, out);
-          Cx<R> wch[3];
+          const RowC rc = load_row(pc);
+          const Cx etc = row_etc<MIXED>(b, rc.tc);
+          Cx out[3];
+          response_row<MIXED>(det, pc, modes, nodes, pbase, NNS, b, rc, etc, out);
+          Cx wch[3];
 #pragma unroll
           for (int ch = 0; ch < 3; ++ch) {
-            double rr = dc[ch].re - (double)out[ch].re, ri = dc[ch].im - (double)out[ch].im;
-            v -= wc[ch] * (rr * rr + ri * ri);
-            wch[ch] = {(R)(2.0 * wc[ch] * rr), (R)(2.0 * wc[ch] * ri)};
+            const double rr = dc[ch].re - out[ch].re, ri = dc[ch].im - out[ch].im;
+            v -= wc[ch] * fma(rr, rr, ri * ri);
+            wch[ch] = {2.0 * wc[ch] * rr, 2.0 * wc[ch] * ri};
           }
           if (GRAD) {
-            Cx<R> wX, wY, wZ, wT = {(R)0, (R)0};
+            Cx wX, wY, wZ, wT = {0.0, 0.0};
             if (det.orthogonal) {
-              const R r2 = (R)0.70710678118654752440, r6 = (R)0.40824829046386301637;
+              const double r2 = 0.70710678118654752440, r6 = 0.40824829046386301637;
               wX = {-r2 * wch[0].re + r6 * wch[1].re, -r2 * wch[0].im + r6 * wch[1].im};
-              wY = {(R)-2 * r6 * wch[1].re, (R)-2 * r6 * wch[1].im};
+              wY = {-2.0 * r6 * wch[1].re, -2.0 * r6 * wch[1].im};
               wZ = {r2 * wch[0].re + r6 * wch[1].re, r2 * wch[0].im + r6 * wch[1].im};
               wT = mul(conjx(b.pfT), wch[2]);
             } else {
@@ -574,23 +576,45 @@ __global__ __launch_bounds__(TILE) void k_lnl(const DetectorDev* __restrict__ de
               wY = wch[1];
               wZ = wch[2];
             }
-            Cx<R> cpf = conjx(b.pf), cD = conjx(b.D);
-            Cx<R> WX = mul(cpf, wX), WY = mul(cpf, wY), WZ = mul(cpf, wZ);
-            Cx<R> DWX = mul(cD, WX), DWY = mul(cD, WY), DWZ = mul(cD, WZ);
-            Cx<R> wl[6];
+            const Cx cpf = conjx(b.pf), cD = conjx(b.D);
+            const Cx WX = mul(cpf, wX), WY = mul(cpf, wY), WZ = mul(cpf, wZ);
+            const Cx DWX = mul(cD, WX), DWY = mul(cD, WY), DWZ = mul(cD, WZ);
+            Cx wl[6];
             wl[0] = sub(DWY, WX);
             wl[1] = sub(WY, DWX);
             wl[2] = sub(DWZ, WY);
             wl[3] = sub(WZ, DWY);
             wl[4] = sub(DWX, WZ);
             wl[5] = sub(WX, DWZ);
-            RowC<R> rc = load_row<R>(pc);
-            Cx<R> etc = row_etc(b, rc.tc);
+            AdjAcc aa;
+            aa.pb = {0.0, 0.0, 0.0};
+            aa.qb = {0.0, 0.0, 0.0};
+            aa.kb = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) aa.gint[i] = 0.0;
+            aa.gh_re = aa.mphi = aa.iota = aa.tc = 0.0;
             for (int mi = 0; mi < modes.n; ++mi) {
-              if (b.f > pc[PC_MODE + mi * PC_MODE_STRIDE + MF_CUT]) continue;
-              mode_adjoint<R>(det, pc, mi, modes.id[mi], nodes, nodes_grad, pbase + (size_t)mi * NNS, b, rc, etc,
-                              wl, wT, G);
+              const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
+              if (b.f > mf[MF_CUT]) continue;
+              const size_t nrow = pbase + (size_t)mi * NNS + b.j;
+              mode_adjoint<MIXED>(det, mf, modes.id[mi], nodes + nrow * 4, nodes_grad + nrow * 16, b, rc, etc, wl,
+                                  wT, aa);
             }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) G[i] = aa.gint[i];
+            G[4] = -aa.gh_re * pc[PC_INVDL];
+            G[5] = -aa.mphi;
+            G[6] = aa.iota;
+            const V3 dlp = {pc[PC_DLP], pc[PC_DLP + 1], pc[PC_DLP + 2]};
+            const V3 dlq = {pc[PC_DLQ], pc[PC_DLQ + 1], pc[PC_DLQ + 2]};
+            const V3 dbp = {pc[PC_DBP], pc[PC_DBP + 1], pc[PC_DBP + 2]};
+            const V3 dbq = {pc[PC_DBQ], pc[PC_DBQ + 1], pc[PC_DBQ + 2]};
+            const V3 dlk = {pc[PC_DLK], pc[PC_DLK + 1], pc[PC_DLK + 2]};
+            const V3 dbk = {pc[PC_DBK], pc[PC_DBK + 1], pc[PC_DBK + 2]};
+            G[7] = dot3(aa.pb, dlp) + dot3(aa.qb, dlq) + dot3(aa.kb, dlk);
+            G[8] = dot3(aa.pb, dbp) + dot3(aa.qb, dbq) + dot3(aa.kb, dbk);
+            G[9] = dot3(aa.pb, rc.q) - dot3(aa.qb, rc.p);
+            G[10] = aa.tc;
           }
         }
         v = wave_sum(v);
@@ -598,7 +622,7 @@ __global__ __launch_bounds__(TILE) void k_lnl(const DetectorDev* __restrict__ de
         if (GRAD) {
 #pragma unroll
           for (int i = 0; i < PES_N_PARAMS; ++i) {
-            double s = wave_sum(G[i]);
+            const double s = wave_sum(G[i]);
             if (lane == 0) acc[wave][p - p0][1 + i] += s;
           }
         }
@@ -615,24 +639,26 @@ __global__ __launch_bounds__(TILE) void k_lnl(const DetectorDev* __restrict__ de
   }
 }
 
-template <class R>
-__global__ void k_response_t(const DetectorDev* __restrict__ dets, int det_i, const double* __restrict__ bin_f,
+template <bool MIXED>
+__global__ __launch_bounds__(128) void k_response_t(const DetectorDev* __restrict__ dets, int det_i, const double* __restrict__ bin_f,
                              const int* __restrict__ bin_j, const double* __restrict__ bin_s,
                              const double* __restrict__ pconst, const double* __restrict__ nodes, int NNS, double du,
                              ModeList modes, int P, double* __restrict__ out) {
   const DetectorDev det = dets[det_i];
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int p = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
   if (k >= det.n_bins || p >= P) return;
-  int bi = det.bin_offset + k;
-  BinPre<R> b;
-  bin_prepare<R>(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
+  const int bi = det.bin_offset + k;
+  BinPre b;
+  bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
   const double* pc = pconst + (size_t)p * PC_STRIDE;
-  Cx<R> o[3];
-  response_row<R>(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, o);
+  const RowC rc = load_row(pc);
+  const Cx etc = row_etc<MIXED>(b, rc.tc);
+  Cx o[3];
+  response_row<MIXED>(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, rc, etc, o);
   for (int ch = 0; ch < 3; ++ch) {
-    size_t idx = (((size_t)p * 3 + ch) * det.n_bins + k) * 2;
-    out[idx] = (double)o[ch].re;
-    out[idx + 1] = (double)o[ch].im;
+    const size_t idx = (((size_t)p * 3 + ch) * det.n_bins + k) * 2;
+    out[idx] = o[ch].re;
+    out[idx + 1] = o[ch].im;
   }
 }

@@ -1,23 +1,27 @@
-// Range-reduced sine/cosine for the hot kernels.
+// Sine/cosine and reciprocal for the hot kernels.
 //
-// Cody-Waite reduction by pi/2 (k = rint(2x/pi), three-part split of pi/2 whose
-// first two parts have 29 significant bits so k*C1, k*C2 are exact for |k| < 2^24)
-// followed by Taylor polynomials on [-pi/4, pi/4] (coefficients 1/n!), quadrant by
-// k mod 4.  The split constants are pi/2 rounded to 29 + 29 + 53 bits, computed with
-// mpmath.  Arguments beyond the reduction's exact range fall back to the library.
+// sincos_d: reduction by pi/2 with k = rint(2x/pi) in the FMA form
+//   r = fma(-k, P1, x); r = fma(-k, P2, r)
+// where P1 = pi/2 rounded to double and P2 = (pi/2 - P1) rounded.  The first FMA is exact:
+// x and k*P1 both lie on the 2^-52 grid once |x| >= 1 and their difference is below 1, so
+// no argument size needs a fallback (the error of r is ~ulp(r) + |k| 1.5e-33).  Then
+// Taylor polynomials on [-pi/4, pi/4] (coefficients 1/n!, truncation < 1e-19) and the
+// quadrant k mod 4.
+// sincos_small_d: |x| <= 4 without reduction: the same polynomials at x/2, then the
+// double-angle formulas (error < 1e-13 at |x| = 4, < 1e-19 for |x| <= 2).
+// rcp_d: hardware reciprocal estimate refined by two Newton steps (correctly rounded to
+// about one ulp).
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace pes {
 
 constexpr double kTwoOverPi = 0.6366197723675814;
-constexpr double kPio2_1 = 1.570796325802803;        // 29 bits
-constexpr double kPio2_2 = 9.920935774287987e-10;    // next 29 bits
-constexpr double kPio2_3 = 2.2517417741562176e-18;   // remainder
-constexpr double kPio2_f1 = 1.5707963109016418;      // 26 bits (float-output path)
-constexpr double kPio2_f2 = 1.5893254773528196e-08;  // remainder
+constexpr double kPio2Hi = 1.5707963267948966;     // pi/2 rounded to double
+constexpr double kPio2Lo = 6.123233995736766e-17;  // pi/2 - kPio2Hi, rounded
+constexpr double kInv2Pi = 0.15915494309189535;    // 1/(2 pi)
 
-// sin/cos on |r| <= pi/4, FP64 (truncation < 1e-19)
+// sin/cos on |r| <= 1 (Taylor to r^19 / r^20)
 __device__ inline void sincos_poly_d(double r, double* s, double* c) {
   const double r2 = r * r;
   double ps = 8.22063524662433e-18;
@@ -42,75 +46,53 @@ __device__ inline void sincos_poly_d(double r, double* s, double* c) {
   *c = fma(-r2, pc, 1.0);
 }
 
-__device__ inline void quadrant(int q, double sr, double cr, double* s, double* c) {
-  double ss = (q & 1) ? cr : sr;
-  double cc = (q & 1) ? sr : cr;
-  *s = (q & 2) ? -ss : ss;
-  *c = ((q + 1) & 2) ? -cc : cc;
-}
-
-// FP64 sin/cos, |x| < 2.6e7 on the fast path
+// FP64 sin/cos of any finite argument
 __device__ inline void sincos_d(double x, double* s, double* c) {
-  if (fabs(x) > 2.5e7) {
-    sincos(x, s, c);
-    return;
-  }
-  double k = rint(x * kTwoOverPi);
-  double r = fma(-k, kPio2_1, x);
-  r = fma(-k, kPio2_2, r);
-  r = fma(-k, kPio2_3, r);
+  const double k = rint(x * kTwoOverPi);
+  double r = fma(-k, kPio2Hi, x);
+  r = fma(-k, kPio2Lo, r);
   double sr, cr;
   sincos_poly_d(r, &sr, &cr);
-  quadrant(((int)k) & 3, sr, cr, s, c);
-}
-
-// FP32 sin/cos on |r| <= pi/4 (truncation < 2e-9)
-__device__ inline void sincos_poly_f(float r, float* s, float* c) {
-  const float r2 = r * r;
-  float ps = 2.7557319223985893e-06f;
-  ps = fmaf(ps, r2, -0.0001984126984126984f);
-  ps = fmaf(ps, r2, 0.008333333333333333f);
-  ps = fmaf(ps, r2, -0.16666666666666666f);
-  *s = fmaf(r * r2, ps, r);
-  float pc = 2.08767569878681e-09f;
-  pc = fmaf(pc, r2, -2.755731922398589e-07f);
-  pc = fmaf(pc, r2, 2.48015873015873e-05f);
-  pc = fmaf(pc, r2, -0.001388888888888889f);
-  pc = fmaf(pc, r2, 0.041666666666666664f);
-  pc = fmaf(pc, r2, -0.5f);
-  *c = fmaf(r2, pc, 1.0f);
-}
-
-// FP32 result for an FP64 argument: the reduction is done in FP64
-__device__ inline void sincos_df(double x, float* s, float* c) {
-  double k = rint(x * kTwoOverPi);
-  double r = fma(-k, kPio2_f1, x);
-  r = fma(-k, kPio2_f2, r);
-  float sr, cr;
-  sincos_poly_f((float)r, &sr, &cr);
-  int q = ((int)k) & 3;
-  float ss = (q & 1) ? cr : sr;
-  float cc = (q & 1) ? sr : cr;
+  const int q = (int)fma(-4.0, floor(k * 0.25), k);  // k mod 4, exact for |k| < 2^53
+  const double ss = (q & 1) ? cr : sr;
+  const double cc = (q & 1) ? sr : cr;
   *s = (q & 2) ? -ss : ss;
   *c = ((q + 1) & 2) ? -cc : cc;
 }
 
-// FP32 argument of modest size (|x| < ~1e3): reduce in FP32 with a two-part pi/2
-__device__ inline void sincos_f(float x, float* s, float* c) {
-  float k = rintf(x * 0.63661977236758134f);
-  float r = fmaf(-k, 1.5707963705062866f, x);
-  r = fmaf(-k, -4.3711388286737929e-08f, r);
-  float sr, cr;
-  sincos_poly_f(r, &sr, &cr);
-  int q = ((int)k) & 3;
-  float ss = (q & 1) ? cr : sr;
-  float cc = (q & 1) ? sr : cr;
-  *s = (q & 2) ? -ss : ss;
-  *c = ((q + 1) & 2) ? -cc : cc;
+// FP64 sin/cos for |x| <= 4
+__device__ inline void sincos_small_d(double x, double* s, double* c) {
+  double sh, ch;
+  sincos_poly_d(0.5 * x, &sh, &ch);
+  *s = 2.0 * sh * ch;
+  *c = (ch - sh) * (ch + sh);
 }
 
-// precision-generic front ends
-__device__ inline void sc_small(double x, double* s, double* c) { sincos_d(x, s, c); }
-__device__ inline void sc_small(float x, float* s, float* c) { sincos_f(x, s, c); }
+__device__ inline double rcp_d(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// FP64 argument reduced to revolutions in FP64, then the hardware FP32 sine/cosine
+// (v_sin_f32 / v_cos_f32 take revolutions); absolute error ~1e-7
+__device__ inline void sincos_rev_f(double x, double* s, double* c) {
+  double r = x * kInv2Pi;
+  r -= rint(r);
+  const float rf = (float)r;
+  *s = (double)__builtin_amdgcn_sinf(rf);
+  *c = (double)__builtin_amdgcn_cosf(rf);
+}
+
+// the same for |x| <= 4 (|x / 2 pi| < 0.64 revolutions: no reduction needed)
+__device__ inline void sincos_rev_small_f(double x, double* s, double* c) {
+  const float rf = (float)(x * kInv2Pi);
+  *s = (double)__builtin_amdgcn_sinf(rf);
+  *c = (double)__builtin_amdgcn_cosf(rf);
+}
+
+__device__ inline double rcp_f(double x) { return (double)__builtin_amdgcn_rcpf((float)x); }
 
 }  // namespace pes

@@ -49,6 +49,7 @@ enum PConst {
   PC_DLP = 13, PC_DLQ = 16, PC_DBP = 19, PC_DBQ = 22, PC_DLK = 25, PC_DBK = 28,
   PC_MODE = 32,  // per mode: Fp re,im, Fc re,im, cut_hz, dFp/diota re,im, dFc/diota re,im
   PC_MODE_STRIDE = 9,
+  PC_FCUT = 32 + 6 * 9,  // largest mode cut [Hz]: the response is zero above it
   PC_STRIDE = 32 + 6 * 9 + 2
 };
 enum { MF_FPR = 0, MF_FPI, MF_FCR, MF_FCI, MF_CUT, MF_DFPR, MF_DFPI, MF_DFCR, MF_DFCI };
@@ -78,7 +79,7 @@ __device__ inline Cplx mode_factor(int id) {
 }
 
 // ---- k_setup ---------------------------------------------------------------------------
-__global__ void k_setup(const double* __restrict__ params, int P, ModeList modes, double f_ref,
+__global__ __launch_bounds__(64) void k_setup(const double* __restrict__ params, int P, ModeList modes, double f_ref,
                         double* __restrict__ setup, double* __restrict__ pconst) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
@@ -132,6 +133,7 @@ __global__ void k_setup(const double* __restrict__ params, int P, ModeList modes
   typedef Dual<double, 1> D1;
   D1 di = lift<D1>(iota);
   di.d[0] = 1.0;
+  double fcut = 0.0;
   for (int mi = 0; mi < modes.n; ++mi) {
     int id = modes.id[mi];
     int l = kModeL[id], m = kModeM[id];
@@ -152,15 +154,17 @@ __global__ void k_setup(const double* __restrict__ params, int P, ModeList modes
     Cplx dFc = cmul({0.0, dkc / dist}, eu);
     mf[MF_FPR] = Fp.re; mf[MF_FPI] = Fp.im; mf[MF_FCR] = Fc.re; mf[MF_FCI] = Fc.im;
     mf[MF_CUT] = ph.mode_cut(id) / msec;
+    fcut = fmax(fcut, mf[MF_CUT]);
     mf[MF_DFPR] = dFp.re; mf[MF_DFPI] = dFp.im; mf[MF_DFCR] = dFc.re; mf[MF_DFCI] = dFc.im;
   }
+  pc[PC_FCUT] = fcut;
 }
 
 // ---- k_nodes ---------------------------------------------------------------------------
 // node table: [P][n_modes][NNS][4] = (A, dA/du, Psi, dPsi/du); NNS = n_nodes + 1 (padding)
 // grad table: [P][n_modes][NNS][16] = d/dtheta_i of the same four, i = Mc, q, chi1, chi2
 template <bool GRAD>
-__global__ void k_nodes(const double* __restrict__ setup, int P, ModeList modes, int n_nodes, double u0, double du,
+__global__ __launch_bounds__(128) void k_nodes(const double* __restrict__ setup, int P, ModeList modes, int n_nodes, double u0, double du,
                         const double* __restrict__ node_f, double* __restrict__ nodes,
                         double* __restrict__ nodes_grad) {
   int jn = blockIdx.x * blockDim.x + threadIdx.x;
@@ -270,32 +274,6 @@ struct BinData {
   int j;
 };
 
-// full FP64 forward response of one row at one bin: the three TDI outputs
-__device__ inline void response_fp64(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
-                                     const double* __restrict__ nodes, size_t pbase, int NNS, const BinCtx& b,
-                                     Cplx out[3]) {
-  double tc = pc[PC_TC];
-  Vec3 pv = {pc[PC_P], pc[PC_P + 1], pc[PC_P + 2]};
-  Vec3 qv = {pc[PC_Q], pc[PC_Q + 1], pc[PC_Q + 2]};
-  Vec3 kv = {pc[PC_K], pc[PC_K + 1], pc[PC_K + 2]};
-  Cplx etc;
-  sincos(-2.0 * PI * b.f * tc, &etc.im, &etc.re);  // antenna.py:435
-  Links L = {};
-  for (int mi = 0; mi < modes.n; ++mi) {
-    const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
-    if (b.f > mf[MF_CUT]) continue;
-    double A, Psi, tf;
-    mode_at_bin(nodes, pbase + (size_t)mi * NNS, b, A, Psi, tf);
-    double sp, cp;
-    sincos(Psi, &sp, &cp);
-    Cplx h = cmul({A * cp, -A * sp}, etc);
-    Cplx hp = cmul({mf[MF_FPR], mf[MF_FPI]}, h);
-    Cplx hc = cmul({mf[MF_FCR], mf[MF_FCI]}, h);
-    add_mode_links(det, b.f, tf + tc, hp, hc, pv, qv, kv, L);
-  }
-  tdi_combine(det, b.f, L, out);
-}
-
 __device__ inline int tile_detector(const DetectorDev* dets, int n_det, int tile) {
   int acc = 0;
   for (int d = 0; d < n_det; ++d) {
@@ -307,7 +285,7 @@ __device__ inline int tile_detector(const DetectorDev* dets, int n_det, int tile
 
 #include "hotpath.hpp"
 
-__global__ void k_reduce(const double* __restrict__ partial, int n_groups, int P, int nout,
+__global__ __launch_bounds__(128) void k_reduce(const double* __restrict__ partial, int n_groups, int P, int nout,
                          const double* __restrict__ pconst, double* __restrict__ lnl, double* __restrict__ grad) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
@@ -324,7 +302,7 @@ __global__ void k_reduce(const double* __restrict__ partial, int n_groups, int P
 }
 
 // ---- response / waveform output kernels (parity checks) -------------------------------------
-__global__ void k_waveform(const DetectorDev* __restrict__ dets, int det_i, const double* __restrict__ bin_f,
+__global__ __launch_bounds__(128) void k_waveform(const DetectorDev* __restrict__ dets, int det_i, const double* __restrict__ bin_f,
                            const int* __restrict__ bin_j, const double* __restrict__ bin_s,
                            const double* __restrict__ pconst, const double* __restrict__ nodes, int NNS, double du,
                            ModeList modes, int P, double* __restrict__ out) {
@@ -366,7 +344,7 @@ struct Extrinsic {
   Vec3 p, q, k;
   double tc;
 };
-__global__ void k_response_container(const DetectorDev* __restrict__ dets, int det_i, const double* __restrict__ bin_f,
+__global__ __launch_bounds__(128) void k_response_container(const DetectorDev* __restrict__ dets, int det_i, const double* __restrict__ bin_f,
                                      int n_modes, const double* __restrict__ wf, Extrinsic ex,
                                      double* __restrict__ out) {
   const DetectorDev det = dets[det_i];
@@ -392,7 +370,7 @@ __global__ void k_response_container(const DetectorDev* __restrict__ dets, int d
 
 // ---- standalone Whittle lnL: -2 df sum_c sum_k |d - h|^2 / S (likelihood_ti.py:58-67) --------
 constexpr int WHITTLE_BLOCKS = 1024;
-__global__ void k_whittle_partial(const double* __restrict__ data, const double* __restrict__ resp,
+__global__ __launch_bounds__(TILE) void k_whittle_partial(const double* __restrict__ data, const double* __restrict__ resp,
                                   const double* __restrict__ psd, int n_chan, int n, double* __restrict__ work) {
   __shared__ double red[TILE / 64];
   double v = 0.0;
@@ -441,6 +419,7 @@ struct pes_context {
   int n_tiles = 0, NB = 0;
   double *d_bin_f = nullptr, *d_bin_s = nullptr, *d_bin_w = nullptr, *d_dre = nullptr, *d_dim = nullptr;
   int* d_bin_j = nullptr;
+  double *d_tile_f0 = nullptr, *d_tile_dd = nullptr;  // first frequency and sum w |d|^2 per tile
   double* d_node_f = nullptr;  // direct mode only
   // scratch, grown on demand
   int cap_P = 0;
@@ -519,17 +498,29 @@ extern "C" int pes_context_add_detector(pes_context* ctx, int orbit_kind, double
   d.L = arm_length_m / C_SI;
   d.rot0 = rotation_initial;
   d.rev0 = revolution_initial;
+  d.invL = 1.0 / d.L;
   if (orbit_kind == 0) {  // orbit.py:189-228
     d.rp = d.L / (2.0 * std::sqrt(3.0));
     double k1 = rotation_initial, k2 = k1 + 2.0 * PI / 3.0, k3 = k2 + 2.0 * PI / 3.0;
     d.ck[0] = std::cos(k1); d.sk[0] = std::sin(k1);
     d.ck[1] = std::cos(k2); d.sk[1] = std::sin(k2);
     d.ck[2] = std::cos(k3); d.sk[2] = std::sin(k3);
+    for (int i = 0; i < 3; ++i) {
+      d.hck[i] = d.rp * d.ck[i];
+      d.hsk[i] = d.rp * d.sk[i];
+      d.hzc[i] = -SQRT3 * d.rp * d.ck[i];
+      d.hzs[i] = -SQRT3 * d.rp * d.sk[i];
+    }
   } else {  // orbit.py:77-139, TianQin defaults
     d.rp = d.L / std::sqrt(3.0);
     double lr = 120.5 / 180.0 * PI, br = -4.7 / 180.0 * PI;
     d.slam = std::sin(lr); d.clam = std::cos(lr); d.sbet = std::sin(br); d.cbet = std::cos(br);
     d.omega_rot = 2.0 * PI / (3.65 * DAY_SI);
+    d.gxs = d.rp * d.sbet * d.clam;
+    d.gxc = d.rp * d.slam;
+    d.gys = d.rp * d.sbet * d.slam;
+    d.gyc = -d.rp * d.clam;
+    d.gzs = -d.rp * d.cbet;
   }
   h.f.assign(f, f + n_bins);
   h.w.assign(3 * (size_t)n_bins, 0.0);
@@ -537,6 +528,10 @@ extern "C" int pes_context_add_detector(pes_context* ctx, int orbit_kind, double
   h.dim.assign(data_im, data_im + 3 * (size_t)n_bins);
   for (int k = 0; k < n_bins; ++k) {
     if (!(f[k] > 0.0) || !std::isfinite(f[k])) return fail(PES_ERR_INVALID, "frequencies must be positive");
+    if (k > 0 && !(f[k] > f[k - 1])) return fail(PES_ERR_INVALID, "frequencies must be strictly increasing");
+    // the transfer-function sines take arguments up to pi f L (no range reduction above 4)
+    if (PI * f[k] * d.L > 4.0)
+      return fail(PES_ERR_INVALID, "pi f L must not exceed 4 (grid too high for this arm length)");
     double dfk = weight ? weight[k] : df;
     for (int ch = 0; ch < 3; ++ch) {
       if (!channel_mask[ch]) continue;
@@ -617,6 +612,24 @@ extern "C" int pes_context_finalize(pes_context* ctx) {
   HIP_TRY(hipMemcpy(ctx->d_bin_w, bw.data(), 3 * (size_t)NB * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ctx->d_dre, dre.data(), 3 * (size_t)NB * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ctx->d_dim, dim.data(), 3 * (size_t)NB * sizeof(double), hipMemcpyHostToDevice));
+  {
+    const int nt = ctx->n_tiles;
+    std::vector<double> tf0(nt), tdd(nt, 0.0);
+    for (int t = 0; t < nt; ++t) {
+      tf0[t] = bf[(size_t)t * TILE];
+      double s = 0.0;
+      for (int ch = 0; ch < 3; ++ch)
+        for (int k = 0; k < TILE; ++k) {
+          const size_t i = (size_t)ch * NB + (size_t)t * TILE + k;
+          s += bw[i] * (dre[i] * dre[i] + dim[i] * dim[i]);
+        }
+      tdd[t] = s;
+    }
+    HIP_TRY(hipMalloc(&ctx->d_tile_f0, nt * sizeof(double)));
+    HIP_TRY(hipMalloc(&ctx->d_tile_dd, nt * sizeof(double)));
+    HIP_TRY(hipMemcpy(ctx->d_tile_f0, tf0.data(), nt * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->d_tile_dd, tdd.data(), nt * sizeof(double), hipMemcpyHostToDevice));
+  }
   if (ctx->direct) ctx->d_node_f = ctx->d_bin_f;
   ctx->finalized = true;
   return PES_OK;
@@ -632,6 +645,8 @@ extern "C" void pes_context_destroy(pes_context* ctx) {
   free_dev(ctx->d_bin_w);
   free_dev(ctx->d_dre);
   free_dev(ctx->d_dim);
+  free_dev(ctx->d_tile_f0);
+  free_dev(ctx->d_tile_dd);
   free_dev(ctx->d_setup);
   free_dev(ctx->d_pconst);
   free_dev(ctx->d_nodes);
@@ -722,18 +737,18 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
   if (grad) nout = 1 + PES_N_PARAMS;
 #define PES_LNL_ARGS                                                                                         \
   ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre,       \
-      ctx->d_dim, ctx->NB, ctx->d_pconst, ctx->d_nodes, ctx->d_nodes_grad, NNS, ctx->du, ctx->modes, P,       \
-      ctx->d_partial
+      ctx->d_dim, ctx->NB, ctx->d_tile_f0, ctx->d_tile_dd, ctx->d_pconst, ctx->d_nodes, ctx->d_nodes_grad, NNS, \
+      ctx->du, ctx->modes, P, ctx->d_partial
   if (precision == PES_PREC_FP64) {
     if (grad)
-      hipLaunchKernelGGL((k_lnl<double, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+      hipLaunchKernelGGL((k_lnl<false, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
     else
-      hipLaunchKernelGGL((k_lnl<double, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+      hipLaunchKernelGGL((k_lnl<false, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
   } else {
     if (grad)
-      hipLaunchKernelGGL((k_lnl<float, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+      hipLaunchKernelGGL((k_lnl<true, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
     else
-      hipLaunchKernelGGL((k_lnl<float, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+      hipLaunchKernelGGL((k_lnl<true, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
   }
 #undef PES_LNL_ARGS
   HIP_TRY(hipGetLastError());
@@ -759,11 +774,11 @@ extern "C" int pes_response(pes_context* ctx, int det, const double* d_params, i
   if (rc != PES_OK || P == 0) return rc;
   int n = ctx->hdets[det].d.n_bins;
   if (precision == PES_PREC_FP64)
-    hipLaunchKernelGGL(k_response_t<double>, dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det,
+    hipLaunchKernelGGL(k_response_t<false>, dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det,
                        ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1,
                        ctx->du, ctx->modes, P, d_out);
   else
-    hipLaunchKernelGGL(k_response_t<float>, dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det,
+    hipLaunchKernelGGL(k_response_t<true>, dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det,
                        ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1,
                        ctx->du, ctx->modes, P, d_out);
   HIP_TRY(hipGetLastError());
