@@ -67,12 +67,10 @@ struct Trig {
   }
 };
 
-// per-bin quantities shared by every row and mode of a tile lane
+// per-bin quantities shared by every row and mode of a tile lane (kept small: they stay in
+// registers for all rows of the tile)
 struct BinPre {
-  double f, pif, xL, sxL, cxL;
-  Cx eL;       // exp(-i pi f L)
-  Cx D, pf;    // TDI delay exp(-2 pi i f L) and generation prefactor
-  Cx pfT;      // T-channel factor pf (1 - D) / sqrt(3)
+  double f, pif, xL, sxL, cxL;  // f, pi f, pi f L, sin and cos of pi f L
   // cubic-Hermite weights on [node j, node j+1] at this bin, u = ln f, s = (u - u_j)/du:
   //   value = v0 + hb1 (v1 - v0) + hb2 v0' + hb3 v1'      (v' = dv/du)
   //   t_f   = he1 (P1 - P0) + he2 P0' + he3 P1'            (dPsi/df / 2 pi)
@@ -80,7 +78,6 @@ struct BinPre {
   int j;
 };
 
-// TDI constants are formed in FP64 from their product forms (1 - D^n cancels at low f)
 __device__ inline void bin_prepare(const DetectorDev& det, double f, double s, int j, double du, BinPre& b) {
   b.f = f;
   b.j = j;
@@ -99,25 +96,33 @@ __device__ inline void bin_prepare(const DetectorDev& det, double f, double s, i
   b.xL = xL;
   b.sxL = sx;
   b.cxL = cx;
-  b.eL = {cx, -sx};
+}
+
+// TDI constants of one bin, formed in FP64 from their product forms (1 - D^n cancels at
+// low f); recomputed per row from sin, cos of xL rather than held in registers
+struct TdiC {
+  Cx D, pf;  // delay exp(-2 i xL) and generation prefactor
+  Cx pfT;    // T-channel factor pf (1 - D) / sqrt(3)
+};
+__device__ inline void tdi_consts(const DetectorDev& det, const BinPre& b, TdiC& t) {
+  const double sx = b.sxL, cx = b.cxL;
   // D = exp(-2 i xL);  1 - D^n = 2 i sin(n xL) exp(-i n xL)
-  const double s2 = 2.0 * sx * cx, c2 = cx * cx - sx * sx;
-  b.D = {c2, -s2};
+  const double s2 = 2.0 * sx * cx, c2 = (cx - sx) * (cx + sx);
+  t.D = {c2, -s2};
   // 1 - D^2 = 2 i sin(2xL) exp(-2 i xL) = 2 sin(2xL) (sin(2xL) + i cos(2xL))
   double pfr = 2.0 * s2 * s2, pfi = 2.0 * s2 * c2;
-  if (det.tdi_gen == 20) {  // (1 - D^2)(1 - D^4)
-    double s8, c8;
-    sincos_d(4.0 * xL, &s8, &c8);
-    const double b_re = 2.0 * s8 * s8, b_im = 2.0 * s8 * c8;
+  if (det.tdi_gen == 20) {  // (1 - D^2)(1 - D^4), 1 - D^4 = 2 sin(4xL) (sin(4xL) + i cos(4xL))
+    const double s4 = 2.0 * s2 * c2, c4 = (c2 - s2) * (c2 + s2);
+    const double b_re = 2.0 * s4 * s4, b_im = 2.0 * s4 * c4;
     const double r = pfr * b_re - pfi * b_im, i = pfr * b_im + pfi * b_re;
     pfr = r;
     pfi = i;
   }
-  b.pf = {pfr, pfi};
+  t.pf = {pfr, pfi};
   // pf (1 - D) / sqrt(3),  1 - D = 2 i sin(xL) exp(-i xL)
   const double o_re = 2.0 * sx * sx, o_im = 2.0 * sx * cx;
   const double r3 = 0.57735026918962576451;
-  b.pfT = {(pfr * o_re - pfi * o_im) * r3, (pfr * o_im + pfi * o_re) * r3};
+  t.pfT = {(pfr * o_re - pfi * o_im) * r3, (pfr * o_im + pfi * o_re) * r3};
 }
 
 // one row's extrinsic constants
@@ -165,7 +170,7 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
     for (int i = 0; i < 3; ++i) {
       g.xd[i].x = fma(sc, det.hsk[i], -sa2 * det.hck[i]);
       g.xd[i].y = fma(sc, det.hck[i], -ca2 * det.hsk[i]);
-      g.xd[i].z = fma(ca, det.hzc[i], sa * det.hzs[i]);
+      g.xd[i].z = -SQRT3 * fma(ca, det.hck[i], sa * det.hsk[i]);
     }
     g.sr = 0.0;
     g.cr = 0.0;
@@ -220,7 +225,7 @@ __device__ inline void xd_dot(const DetectorDev& det, const ModeG& g, V3 xt[3]) 
     for (int i = 0; i < 3; ++i) {
       xt[i].x = w * fma(c2a, det.hsk[i], -sc2 * det.hck[i]);
       xt[i].y = w * fma(c2a, det.hck[i], sc2 * det.hsk[i]);
-      xt[i].z = w * fma(ca, det.hzs[i], -sa * det.hzc[i]);
+      xt[i].z = (-SQRT3 * w) * fma(ca, det.hsk[i], -sa * det.hck[i]);
     }
   } else {
     const double cr = -0.5, sr = 0.5 * SQRT3;
@@ -319,7 +324,7 @@ __device__ inline void pair_eval(const DetectorDev& det, const BinPre& b, const 
   const Cx nhn = {fma(v.Pp, g.hp.re, v.Pc * g.hc.re), fma(v.Pp, g.hp.im, v.Pc * g.hc.im)};
   double s_, c_;
   Trig<MIXED>::small(b.pif * g.kx[a], &s_, &c_);
-  v.E = mul(b.eL, Cx{c_, s_});
+  v.E = mul(Cx{b.cxL, -b.sxL}, Cx{c_, s_});  // exp(-i pi f L) exp(+i pi f k.xd_a)
   const Cx T = mul(nhn, v.E);
   v.C = {b.xL * T.im, -b.xL * T.re};  // (-i xL) nhn E
   sinc_pair<MIXED>(b.xL, b.sxL, b.cxL, v.kn, v.sp, v.sm, v.dsp, v.dsm, v.dl);
@@ -332,16 +337,16 @@ struct LinksT {
 
 // TDI outputs: A, E from X, Y, Z (tdi.py:1784-1785); T from the pair sum, which equals
 // (X + Y + Z)/sqrt3 = pf (1 - D) sum_a (sinc_- - sinc_+)_a C_a / sqrt3 without cancellation
-__device__ inline void tdi_out(const DetectorDev& det, const BinPre& b, const LinksT& L, Cx out[3]) {
+__device__ inline void tdi_out(const DetectorDev& det, const TdiC& t, const LinksT& L, Cx out[3]) {
   const Cx* l = L.l;
-  const Cx X = mul(b.pf, add(sub(l[5], l[0]), mul(b.D, sub(l[4], l[1]))));
-  const Cx Y = mul(b.pf, add(sub(l[1], l[2]), mul(b.D, sub(l[0], l[3]))));
-  const Cx Z = mul(b.pf, add(sub(l[3], l[4]), mul(b.D, sub(l[2], l[5]))));
+  const Cx X = mul(t.pf, add(sub(l[5], l[0]), mul(t.D, sub(l[4], l[1]))));
+  const Cx Y = mul(t.pf, add(sub(l[1], l[2]), mul(t.D, sub(l[0], l[3]))));
+  const Cx Z = mul(t.pf, add(sub(l[3], l[4]), mul(t.D, sub(l[2], l[5]))));
   if (det.orthogonal) {
     const double r2 = 0.70710678118654752440, r6 = 0.40824829046386301637;
     out[0] = scale(sub(Z, X), r2);
     out[1] = scale(add(sub(X, scale(Y, 2.0)), Z), r6);
-    out[2] = mul(b.pfT, L.tl);
+    out[2] = mul(t.pfT, L.tl);
   } else {
     out[0] = X;
     out[1] = Y;
@@ -363,7 +368,7 @@ __device__ inline Cx row_etc(const BinPre& b, double tc) {
 template <bool MIXED>
 __device__ inline void response_row(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
                                     const double* __restrict__ nodes, size_t pbase, int NNS, const BinPre& b,
-                                    const RowC& rc, Cx etc, Cx out[3]) {
+                                    const RowC& rc, Cx etc, const TdiC& tdc, Cx out[3]) {
   LinksT L;
 #pragma unroll
   for (int i = 0; i < 6; ++i) L.l[i] = {0.0, 0.0};
@@ -383,7 +388,7 @@ __device__ inline void response_row(const DetectorDev& det, const double* __rest
       L.tl = axpy(v.dl, v.C, L.tl);
     }
   }
-  tdi_out(det, b, L, out);
+  tdi_out(det, tdc, L, out);
 }
 
 // ------------------------------- adjoint ---------------------------------------------------
@@ -498,7 +503,7 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
 // zero response for that row: its lnL contribution is the precomputed -sum w |d|^2 of the
 // tile and its gradient contribution is zero, so the row skips it.
 template <bool MIXED, bool GRAD>
-__global__ __launch_bounds__(TILE) void k_lnl(const DetectorDev* __restrict__ dets, int n_det, int n_tiles,
+__global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__ dets, int n_det, int n_tiles,
                                               const double* __restrict__ bin_f, const int* __restrict__ bin_j,
                                               const double* __restrict__ bin_s, const double* __restrict__ bin_w,
                                               const double* __restrict__ dre, const double* __restrict__ dim, int NB,
@@ -533,13 +538,6 @@ __global__ __launch_bounds__(TILE) void k_lnl(const DetectorDev* __restrict__ de
       const int bi = tile * TILE + threadIdx.x;
       BinPre b;
       bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
-      double wc[3];
-      Cx dc[3];
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch) {
-        wc[ch] = bin_w[ch * NB + bi];
-        dc[ch] = {dre[ch * NB + bi], dim[ch * NB + bi]};
-      }
       for (int p = p0; p < p1; ++p) {
         const double* pc = pconst + (size_t)p * PC_STRIDE;
         if (f0 > pc[PC_FCUT]) {
@@ -554,14 +552,18 @@ __global__ __launch_bounds__(TILE) void k_lnl(const DetectorDev* __restrict__ de
           const size_t pbase = (size_t)p * modes.n * NNS;
           const RowC rc = load_row(pc);
           const Cx etc = row_etc<MIXED>(b, rc.tc);
+          TdiC tdc;
+          tdi_consts(det, b, tdc);
           Cx out[3];
-          response_row<MIXED>(det, pc, modes, nodes, pbase, NNS, b, rc, etc, out);
+          response_row<MIXED>(det, pc, modes, nodes, pbase, NNS, b, rc, etc, tdc, out);
+          // data and weights are re-read per row (L1/L2 hits) instead of held in registers
           Cx wch[3];
 #pragma unroll
           for (int ch = 0; ch < 3; ++ch) {
-            const double rr = dc[ch].re - out[ch].re, ri = dc[ch].im - out[ch].im;
-            v -= wc[ch] * fma(rr, rr, ri * ri);
-            wch[ch] = {2.0 * wc[ch] * rr, 2.0 * wc[ch] * ri};
+            const double wc = bin_w[ch * NB + bi];
+            const double rr = dre[ch * NB + bi] - out[ch].re, ri = dim[ch * NB + bi] - out[ch].im;
+            v -= wc * fma(rr, rr, ri * ri);
+            wch[ch] = {2.0 * wc * rr, 2.0 * wc * ri};
           }
           if (GRAD) {
             Cx wX, wY, wZ, wT = {0.0, 0.0};
@@ -570,13 +572,13 @@ __global__ __launch_bounds__(TILE) void k_lnl(const DetectorDev* __restrict__ de
               wX = {-r2 * wch[0].re + r6 * wch[1].re, -r2 * wch[0].im + r6 * wch[1].im};
               wY = {-2.0 * r6 * wch[1].re, -2.0 * r6 * wch[1].im};
               wZ = {r2 * wch[0].re + r6 * wch[1].re, r2 * wch[0].im + r6 * wch[1].im};
-              wT = mul(conjx(b.pfT), wch[2]);
+              wT = mul(conjx(tdc.pfT), wch[2]);
             } else {
               wX = wch[0];
               wY = wch[1];
               wZ = wch[2];
             }
-            const Cx cpf = conjx(b.pf), cD = conjx(b.D);
+            const Cx cpf = conjx(tdc.pf), cD = conjx(tdc.D);
             const Cx WX = mul(cpf, wX), WY = mul(cpf, wY), WZ = mul(cpf, wZ);
             const Cx DWX = mul(cD, WX), DWY = mul(cD, WY), DWZ = mul(cD, WZ);
             Cx wl[6];
@@ -654,8 +656,10 @@ __global__ __launch_bounds__(128) void k_response_t(const DetectorDev* __restric
   const double* pc = pconst + (size_t)p * PC_STRIDE;
   const RowC rc = load_row(pc);
   const Cx etc = row_etc<MIXED>(b, rc.tc);
+  TdiC tdc;
+  tdi_consts(det, b, tdc);
   Cx o[3];
-  response_row<MIXED>(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, rc, etc, o);
+  response_row<MIXED>(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, rc, etc, tdc, o);
   for (int ch = 0; ch < 3; ++ch) {
     const size_t idx = (((size_t)p * 3 + ch) * det.n_bins + k) * 2;
     out[idx] = o[ch].re;

@@ -508,8 +508,6 @@ extern "C" int pes_context_add_detector(pes_context* ctx, int orbit_kind, double
     for (int i = 0; i < 3; ++i) {
       d.hck[i] = d.rp * d.ck[i];
       d.hsk[i] = d.rp * d.sk[i];
-      d.hzc[i] = -SQRT3 * d.rp * d.ck[i];
-      d.hzs[i] = -SQRT3 * d.rp * d.sk[i];
     }
   } else {  // orbit.py:77-139, TianQin defaults
     d.rp = d.L / std::sqrt(3.0);

@@ -28,11 +28,11 @@ struct DetectorDev {
   double slam, clam, sbet, cbet, omega_rot;  // geo reference source / rotation
   double invL;                          // 1 / L
   // constellation offsets xd_i in the fused kernel, products precomputed per detector:
-  // heliocentric  xd_i = (sc hsk_i - (1+sa^2) hck_i, sc hck_i - (1+ca^2) hsk_i, ca hzc_i + sa hzs_i)
-  //               with sa, ca of the revolution angle and sc = sa ca;
+  // heliocentric  xd_i = (sc hsk_i - (1+sa^2) hck_i, sc hck_i - (1+ca^2) hsk_i,
+  //                      -sqrt3 (ca hck_i + sa hsk_i)) with sa, ca of the revolution angle, sc = sa ca;
   // geocentric    xd_i = (gxs sk_i + gxc ck_i, gys sk_i + gyc ck_i, gzs sk_i) with the rotation
   //               angle of node i
-  double hck[3], hsk[3], hzc[3], hzs[3];
+  double hck[3], hsk[3];
   double gxs, gxc, gys, gyc, gzs;
 };
 
