@@ -39,9 +39,13 @@ extern "C" {
 #define PES_P_POLARIZATION 9
 #define PES_P_COALESCENCE_TIME 10   /* seconds                       */
 
-/* precision modes of pes_lnl_batch */
+/* precision modes of pes_lnl_batch / pes_response.  All arithmetic and accumulation is FP64;
+ * PES_PREC_MIXED takes the sine/cosine of the total phase Psi + 2 pi f (t_c + k.x0) (reduced in
+ * FP64) from the FP32 hardware units; PES_PREC_MIXED_FAST also the transfer-function sines,
+ * cosines and reciprocals (faster, accurate near the likelihood peak only). */
 #define PES_PREC_FP64 0
 #define PES_PREC_MIXED 1
+#define PES_PREC_MIXED_FAST 2
 
 typedef struct pes_context pes_context;
 
@@ -123,6 +127,11 @@ int pes_whittle_lnl(const double* d_data, const double* d_resp, const double* d_
  * No counterpart in the reference (it has no per-stage timing). */
 int pes_context_set_profiling(pes_context* ctx, int enable);
 int pes_profile_read(pes_context* ctx, int* n_launches, double* total_ms);
+
+/* Work of the last profiled pes_lnl_batch: the number of (bin, mode, row) triples with the bin
+ * at or below the mode's cut (where the fused kernel evaluates a mode) and of (bin, row) pairs
+ * at or below the row's largest cut; synchronises the device.  No counterpart in the reference. */
+int pes_profile_work(pes_context* ctx, double* bin_mode_rows, double* bin_rows);
 
 /* Number of bins of detector det (after finalize). */
 int pes_detector_bins(const pes_context* ctx, int det);

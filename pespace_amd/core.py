@@ -17,7 +17,7 @@ PARAM_NAMES = ("chirp_mass", "mass_ratio", "chi_1", "chi_2", "luminosity_distanc
                "inclination", "ecliptic_longitude", "ecliptic_latitude", "polarization", "coalescence_time")
 N_PARAMS = len(PARAM_NAMES)
 MODE_IDS = {"22": 0, "21": 1, "33": 2, "32": 3, "44": 4, "43": 5}
-PRECISION = {"fp64": 0, "mixed": 1}
+PRECISION = {"fp64": 0, "mixed": 1, "mixed_fast": 2}
 
 
 def _mode_key(m):
@@ -138,6 +138,14 @@ class LikelihoodCore:
         ms = ctypes.c_double()
         check(lib().pes_profile_read(self._ctx, ctypes.byref(n), ctypes.byref(ms)))
         return n.value, ms.value
+
+    def profile_work(self):
+        """(bin_mode_rows, bin_rows) of the last profiled evaluation: the triples on which a mode
+        is evaluated and the (bin, row) pairs at or below the row's largest mode cut."""
+        bm = ctypes.c_double()
+        br = ctypes.c_double()
+        check(lib().pes_profile_work(self._ctx, ctypes.byref(bm), ctypes.byref(br)))
+        return bm.value, br.value
 
     @property
     def n_bins(self):

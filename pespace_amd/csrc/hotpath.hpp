@@ -1,18 +1,22 @@
 // Fused per-(bin, row) forward response and adjoint.
 //
-// Arithmetic is FP64 in both precision modes; they differ only in where sines, cosines
-// and reciprocals come from:
-//   MIXED = false (FP64 path): sincos_d / sincos_small_d / rcp_d, and the t_c shift and the
-//     barycentric delay are applied as separate factors exp(-2 pi i f t_c),
-//     exp(-2 pi i f k.x0) exactly as the reference does (antenna.py:435, 530-533).
-//   MIXED = true: the transfer-function sines/cosines and reciprocals, and the phasor of the
-//     total phase Psi + 2 pi f (t_c + k.x0) (summed and reduced mod 2 pi in FP64), come from
-//     the FP32 hardware units (v_sin_f32, v_cos_f32, v_rcp_f32).  The orbit angle keeps the
-//     FP64 sincos_d: its error is multiplied by 2 pi f AU/c (~150 rad) in the delay phase.
-// Without packed FP32, an FP32 FMA issues at the FP64 rate on gfx950, so FP32 pays only in
-// the transcendental units; FP64 algebra also keeps per-row constants (polarization
-// factors, 1/d_L, node values) from carrying an FP32 rounding that would bias every bin
-// of a row the same way.
+// Arithmetic is FP64 in every precision mode; the modes differ only in where sines, cosines
+// and reciprocals come from (PREC template parameter, PES_PREC_* values):
+//   0 FP64: sincos_d / sincos_small_d / rcp_d, and the t_c shift and the barycentric delay
+//     applied as separate factors exp(-2 pi i f t_c), exp(-2 pi i f k.x0) exactly as the
+//     reference does (antenna.py:435, 530-533).
+//   1 MIXED: the phasor of the total phase Psi + 2 pi f (t_c + k.x0) (summed and reduced mod
+//     2 pi in FP64) comes from the FP32 hardware sine/cosine (v_sin_f32, v_cos_f32).  Its
+//     error changes from bin to bin (the phase moves by ~2 pi t df per bin), so it averages
+//     out of lnL; the transfer functions stay FP64.
+//   2 MIXED_FAST: the transfer-function sines/cosines and reciprocals come from the FP32 units
+//     too.  Their arguments vary slowly with f, so their ~1e-7 errors are coherent over many
+//     bins and bias |h|^2 by ~1e-7 relative: fine near the likelihood peak, not far from it.
+// The orbit angles keep the FP64 sincos_d in all modes: their error is multiplied by
+// 2 pi f AU/c (~150 rad) in the delay phase.  Without packed FP32, an FP32 FMA issues at the
+// FP64 rate on gfx950, so FP32 pays only in the transcendental units; FP64 algebra also keeps
+// per-row constants (polarization factors, 1/d_L, node values) from carrying an FP32 rounding
+// that would bias every bin of a row the same way.
 //
 // Algebraic rearrangements relative to antenna.py:496-543 (same values):
 //   * the constellation offsets sum to zero, so k.(x_i + x_j) = 2 k.x0 - k.xd_a for link
@@ -43,24 +47,19 @@ __device__ inline void axpy3(double s, const V3& a, V3& y) {
   y.z = fma(s, a.z, y.z);
 }
 
-template <bool MIXED>
+template <int PREC>
 struct Trig {
-  // any argument
-  __device__ static void big(double x, double* s, double* c) {
-    if constexpr (MIXED)
-      sincos_rev_f(x, s, c);
-    else
-      sincos_d(x, s, c);
-  }
-  // |x| <= 4 (the host checks pi f_max L <= 4 per detector)
+  static constexpr bool kFp32Phase = PREC >= 1;     // one FP32 sincos of the total phase
+  static constexpr bool kFp32Transfer = PREC >= 2;  // FP32 transfer-function trig
+  // transfer-function arguments (|x| <= 4: the host checks pi f_max L <= 4 per detector)
   __device__ static void small(double x, double* s, double* c) {
-    if constexpr (MIXED)
+    if constexpr (kFp32Transfer)
       sincos_rev_small_f(x, s, c);
     else
-      sincos_small_d(x, s, c);
+      sincos_tab(x, s, c);
   }
   __device__ static double rcp(double x) {
-    if constexpr (MIXED)
+    if constexpr (kFp32Transfer)
       return rcp_f(x);
     else
       return rcp_d(x);
@@ -152,7 +151,7 @@ struct ModeG {
 
 constexpr double kOmegaYear = 2.0 * PI / YEAR_SI;
 
-template <bool MIXED>
+template <int PREC>
 __device__ inline void mode_geom(const DetectorDev& det, const double* __restrict__ mf,
                                  const double* __restrict__ n0, const BinPre& b, const RowC& rc, Cx etc, ModeG& g) {
   const double A0 = n0[0], dA0 = n0[1], P0 = n0[2], dP0 = n0[3];
@@ -161,7 +160,7 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
   const double dP = P1 - P0;
   const double Psi = P0 + fma(b.hb1, dP, fma(b.hb2, dP0, b.hb3 * dP1));
   g.t = fma(b.he1, dP, fma(b.he2, dP0, b.he3 * dP1)) + rc.tc;
-  sincos_d(fma(kOmegaYear, g.t, det.rev0), &g.sa, &g.ca);
+  sincos_tab(fma(kOmegaYear, g.t, det.rev0), &g.sa, &g.ca);
   const double sa = g.sa, ca = g.ca;
   g.kx0 = AU_SEC * fma(rc.k.x, ca, rc.k.y * sa);
   if (det.kind == 0) {
@@ -176,7 +175,7 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
     g.cr = 0.0;
   } else {
     double s1, c1;
-    Trig<MIXED>::big(fma(det.omega_rot, g.t, det.rot0), &s1, &c1);
+    sincos_tab(fma(det.omega_rot, g.t, det.rot0), &s1, &c1);
     g.sr = s1;
     g.cr = c1;
     const double cr = -0.5, sr = 0.5 * SQRT3;
@@ -200,14 +199,16 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
     g.qx[i] = dot3(rc.q, g.xd[i]);
     g.kx[i] = dot3(rc.k, g.xd[i]);
   }
-  if constexpr (MIXED) {
+  if constexpr (Trig<PREC>::kFp32Phase) {
+    // unit modulus matters: a mean modulus error of the FP32 pair would scale every bin's
+    // |h|^2 the same way; the remaining error is a phase error that changes from bin to bin
     double sp, cp;
-    Trig<true>::big(Psi + 2.0 * b.pif * (rc.tc + g.kx0), &sp, &cp);
+    sincos_rev_fc(Psi + 2.0 * b.pif * (rc.tc + g.kx0), &sp, &cp);
     g.u = {cp, -sp};
   } else {
     double sp, cp, s0, c0;
-    sincos_d(Psi, &sp, &cp);
-    sincos_d(-2.0 * b.pif * g.kx0, &s0, &c0);
+    sincos_tab(Psi, &sp, &cp);
+    sincos_tab(-2.0 * b.pif * g.kx0, &s0, &c0);
     g.u = mul(mul(Cx{cp, -sp}, etc), Cx{c0, s0});
   }
   g.hm = scale(g.u, g.A);
@@ -247,23 +248,23 @@ __device__ inline void xd_dot(const DetectorDev& det, const ModeG& g, V3 xt[3]) 
 }
 
 // sinc(xL +- a) with a = xL k.n, their derivatives, and delta = sinc(xL-a) - sinc(xL+a).
-// In the mixed path delta is taken for xL < 0.3 from
+// With FP32 transfer trig, delta is taken for xL < 0.3 from
 //   delta = 2 x a sum_{m,n>=0} (-1)^{m+n} X^m A^n / ((2m+1)! (2n+1)! (2m+2n+3)),  X = x^2, A = a^2
 // (sinc y = int_0^1 cos(y t) dt, so delta = 2 int_0^1 sin(x t) sin(a t) dt), which has no
 // cancellation; the direct difference would lose the FP32 sines' accuracy there.
-template <bool MIXED>
+template <int PREC>
 __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, double& sp, double& sm, double& dsp,
                                  double& dsm, double& delta) {
   const double a = xL * kn;
   double sa, ca;
-  Trig<MIXED>::small(a, &sa, &ca);
+  Trig<PREC>::small(a, &sa, &ca);
   const double xp = xL + a, xm = xL - a;
   const double sinp = fma(sxL, ca, cxL * sa), cosp = fma(cxL, ca, -sxL * sa);
   const double sinm = fma(sxL, ca, -cxL * sa), cosm = fma(cxL, ca, sxL * sa);
   const double thr = 1e-2;
   const bool small_p = fabs(xp) < thr, small_m = fabs(xm) < thr;
   if (!small_p && !small_m) {
-    const double inv = Trig<MIXED>::rcp(xp * xm);
+    const double inv = Trig<PREC>::rcp(xp * xm);
     const double ip = xm * inv, im = xp * inv;
     sp = sinp * ip;
     dsp = (cosp - sp) * ip;
@@ -275,7 +276,7 @@ __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, d
       sp = 1.0 - x2 * (1.0 / 6.0 - x2 * (1.0 / 120.0 - x2 * (1.0 / 5040.0)));
       dsp = -xp * (1.0 / 3.0 - x2 * (1.0 / 30.0 - x2 * (1.0 / 840.0)));
     } else {
-      const double ip = Trig<MIXED>::rcp(xp);
+      const double ip = Trig<PREC>::rcp(xp);
       sp = sinp * ip;
       dsp = (cosp - sp) * ip;
     }
@@ -284,12 +285,12 @@ __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, d
       sm = 1.0 - x2 * (1.0 / 6.0 - x2 * (1.0 / 120.0 - x2 * (1.0 / 5040.0)));
       dsm = -xm * (1.0 / 3.0 - x2 * (1.0 / 30.0 - x2 * (1.0 / 840.0)));
     } else {
-      const double im = Trig<MIXED>::rcp(xm);
+      const double im = Trig<PREC>::rcp(xm);
       sm = sinm * im;
       dsm = (cosm - sm) * im;
     }
   }
-  if (MIXED && xL < 0.3) {
+  if (Trig<PREC>::kFp32Transfer && xL < 0.3) {
     const double X = xL * xL, A = a * a;
     // q_m(A) = sum_n c_mn A^n, total degree <= 3
     const double q0 = 1.0 / 3.0 + A * (-1.0 / 30.0 + A * (1.0 / 840.0 + A * (-1.0 / 45360.0)));
@@ -312,7 +313,7 @@ struct PairV {
   double sp, sm, dsp, dsm, dl;
 };
 
-template <bool MIXED>
+template <int PREC>
 __device__ inline void pair_eval(const DetectorDev& det, const BinPre& b, const ModeG& g, int a, PairV& v) {
   const int ia = a == 0 ? 2 : a - 1;
   const int ib = a == 0 ? 1 : (a == 1 ? 2 : 0);
@@ -323,11 +324,11 @@ __device__ inline void pair_eval(const DetectorDev& det, const BinPre& b, const 
   v.Pc = 2.0 * v.np * v.nq;
   const Cx nhn = {fma(v.Pp, g.hp.re, v.Pc * g.hc.re), fma(v.Pp, g.hp.im, v.Pc * g.hc.im)};
   double s_, c_;
-  Trig<MIXED>::small(b.pif * g.kx[a], &s_, &c_);
+  Trig<PREC>::small(b.pif * g.kx[a], &s_, &c_);
   v.E = mul(Cx{b.cxL, -b.sxL}, Cx{c_, s_});  // exp(-i pi f L) exp(+i pi f k.xd_a)
   const Cx T = mul(nhn, v.E);
   v.C = {b.xL * T.im, -b.xL * T.re};  // (-i xL) nhn E
-  sinc_pair<MIXED>(b.xL, b.sxL, b.cxL, v.kn, v.sp, v.sm, v.dsp, v.dsm, v.dl);
+  sinc_pair<PREC>(b.xL, b.sxL, b.cxL, v.kn, v.sp, v.sm, v.dsp, v.dsm, v.dl);
 }
 
 struct LinksT {
@@ -354,18 +355,18 @@ __device__ inline void tdi_out(const DetectorDev& det, const TdiC& t, const Link
   }
 }
 
-template <bool MIXED>
+template <int PREC>
 __device__ inline Cx row_etc(const BinPre& b, double tc) {
   Cx etc = {1.0, 0.0};
-  if constexpr (!MIXED) {
+  if constexpr (!Trig<PREC>::kFp32Phase) {
     double s, c;
-    sincos_d(-2.0 * b.pif * tc, &s, &c);
+    sincos_tab(-2.0 * b.pif * tc, &s, &c);
     etc = {c, s};
   }
   return etc;
 }
 
-template <bool MIXED>
+template <int PREC>
 __device__ inline void response_row(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
                                     const double* __restrict__ nodes, size_t pbase, int NNS, const BinPre& b,
                                     const RowC& rc, Cx etc, const TdiC& tdc, Cx out[3]) {
@@ -377,11 +378,11 @@ __device__ inline void response_row(const DetectorDev& det, const double* __rest
     const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
     if (b.f > mf[MF_CUT]) continue;
     ModeG g;
-    mode_geom<MIXED>(det, mf, nodes + (pbase + (size_t)mi * NNS + b.j) * 4, b, rc, etc, g);
+    mode_geom<PREC>(det, mf, nodes + (pbase + (size_t)mi * NNS + b.j) * 4, b, rc, etc, g);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       PairV v;
-      pair_eval<MIXED>(det, b, g, a, v);
+      pair_eval<PREC>(det, b, g, a, v);
       const int lp = a == 0 ? 2 : (a == 1 ? 4 : 0), lm = lp + 1;
       L.l[lp] = axpy(v.sp, v.C, L.l[lp]);
       L.l[lm] = axpy(v.sm, v.C, L.l[lm]);
@@ -404,18 +405,18 @@ struct AdjAcc {
 };
 
 // wl: adjoints of the six links; wT: adjoint of the T pair sum; mid: mode id
-template <bool MIXED>
+template <int PREC>
 __device__ inline void mode_adjoint(const DetectorDev& det, const double* __restrict__ mf, int mid,
                                     const double* __restrict__ n0, const double* __restrict__ g0, const BinPre& b,
                                     const RowC& rc, Cx etc, const Cx wl[6], Cx wT, AdjAcc& acc) {
   ModeG g;
-  mode_geom<MIXED>(det, mf, n0, b, rc, etc, g);
+  mode_geom<PREC>(det, mf, n0, b, rc, etc, g);
   double pxb[3] = {0.0, 0.0, 0.0}, qxb[3] = {0.0, 0.0, 0.0}, kxb[3] = {0.0, 0.0, 0.0};
   Cx Hp = {0.0, 0.0}, Hc = {0.0, 0.0};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     PairV v;
-    pair_eval<MIXED>(det, b, g, a, v);
+    pair_eval<PREC>(det, b, g, a, v);
     const int ia = a == 0 ? 2 : a - 1;
     const int ib = a == 0 ? 1 : (a == 1 ? 2 : 0);
     const int lp = a == 0 ? 2 : (a == 1 ? 4 : 0), lm = lp + 1;
@@ -526,7 +527,7 @@ __device__ inline void wave_sum12(const double x[12], int lane, double z[3]) {
 // lane <-> bin.  A tile whose first frequency lies above every mode's cut of a row has a
 // zero response for that row: its lnL contribution is the precomputed -sum w |d|^2 of the
 // tile and its gradient contribution is zero, so the row skips it.
-template <bool MIXED, bool GRAD>
+template <int PREC, bool GRAD>
 __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__ dets, int n_det, int n_tiles,
                                               const double* __restrict__ bin_f, const int* __restrict__ bin_j,
                                               const double* __restrict__ bin_s, const double* __restrict__ bin_w,
@@ -537,6 +538,7 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
                                               ModeList modes, int P, double* __restrict__ partial) {
   constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
   __shared__ double acc[TILE / 64][CHUNK][NOUT];
+  trig_table_load();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n_chunks = (P + CHUNK - 1) / CHUNK;
   const int n_groups = (n_tiles + TILES_PER_GROUP - 1) / TILES_PER_GROUP;
@@ -575,11 +577,11 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
         if (pc[PC_VALID] != 0.0) {
           const size_t pbase = (size_t)p * modes.n * NNS;
           const RowC rc = load_row(pc);
-          const Cx etc = row_etc<MIXED>(b, rc.tc);
+          const Cx etc = row_etc<PREC>(b, rc.tc);
           TdiC tdc;
           tdi_consts(det, b, tdc);
           Cx out[3];
-          response_row<MIXED>(det, pc, modes, nodes, pbase, NNS, b, rc, etc, tdc, out);
+          response_row<PREC>(det, pc, modes, nodes, pbase, NNS, b, rc, etc, tdc, out);
           // data and weights are re-read per row (L1/L2 hits) instead of held in registers
           Cx wch[3];
 #pragma unroll
@@ -623,7 +625,7 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
               const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
               if (b.f > mf[MF_CUT]) continue;
               const size_t nrow = pbase + (size_t)mi * NNS + b.j;
-              mode_adjoint<MIXED>(det, mf, modes.id[mi], nodes + nrow * 4, nodes_grad + nrow * 16, b, rc, etc, wl,
+              mode_adjoint<PREC>(det, mf, modes.id[mi], nodes + nrow * 4, nodes_grad + nrow * 16, b, rc, etc, wl,
                                   wT, aa);
             }
 #pragma unroll
@@ -671,11 +673,12 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
   }
 }
 
-template <bool MIXED>
+template <int PREC>
 __global__ __launch_bounds__(128) void k_response_t(const DetectorDev* __restrict__ dets, int det_i, const double* __restrict__ bin_f,
                              const int* __restrict__ bin_j, const double* __restrict__ bin_s,
                              const double* __restrict__ pconst, const double* __restrict__ nodes, int NNS, double du,
                              ModeList modes, int P, double* __restrict__ out) {
+  trig_table_load();
   const DetectorDev det = dets[det_i];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int p = blockIdx.y;
@@ -685,11 +688,11 @@ __global__ __launch_bounds__(128) void k_response_t(const DetectorDev* __restric
   bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
   const double* pc = pconst + (size_t)p * PC_STRIDE;
   const RowC rc = load_row(pc);
-  const Cx etc = row_etc<MIXED>(b, rc.tc);
+  const Cx etc = row_etc<PREC>(b, rc.tc);
   TdiC tdc;
   tdi_consts(det, b, tdc);
   Cx o[3];
-  response_row<MIXED>(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, rc, etc, tdc, o);
+  response_row<PREC>(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, rc, etc, tdc, o);
   for (int ch = 0; ch < 3; ++ch) {
     const size_t idx = (((size_t)p * 3 + ch) * det.n_bins + k) * 2;
     out[idx] = o[ch].re;

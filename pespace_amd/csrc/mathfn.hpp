@@ -7,14 +7,28 @@
 // no argument size needs a fallback (the error of r is ~ulp(r) + |k| 1.5e-33).  Then
 // Taylor polynomials on [-pi/4, pi/4] (coefficients 1/n!, truncation < 1e-19) and the
 // quadrant k mod 4.
-// sincos_small_d: |x| <= 4 without reduction: the same polynomials at x/2, then the
-// double-angle formulas (error < 1e-13 at |x| = 4, < 1e-19 for |x| <= 2).
+// sincos_tab: the hot-loop variant.  Reduction by h = 2 pi / 256 in the same FMA form (the
+// first FMA is again exact), sin and cos of the remainder |r| <= pi/256 from short Taylor
+// polynomials (truncation < 5e-18), and the angle-addition formulas with sin, cos of k h read
+// from a 256-entry table of correctly rounded values held in LDS (4 KiB): about 18 VALU
+// instructions and one LDS read instead of ~37, error within ~2 ulp.
 // rcp_d: hardware reciprocal estimate refined by two Newton steps (correctly rounded to
 // about one ulp).
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "trig_table.inc"
+
 namespace pes {
+
+// sin, cos of 2 pi i / kTrigN in LDS; every kernel that calls sincos_tab fills it first with
+// trig_table_load()
+__shared__ double2 g_trig[kTrigN];
+
+__device__ inline void trig_table_load() {
+  for (int i = threadIdx.x; i < kTrigN; i += blockDim.x) g_trig[i] = make_double2(kTrigTab[i][0], kTrigTab[i][1]);
+  __syncthreads();
+}
 
 constexpr double kTwoOverPi = 0.6366197723675814;
 constexpr double kPio2Hi = 1.5707963267948966;     // pi/2 rounded to double
@@ -60,12 +74,21 @@ __device__ inline void sincos_d(double x, double* s, double* c) {
   *c = ((q + 1) & 2) ? -cc : cc;
 }
 
-// FP64 sin/cos for |x| <= 4
-__device__ inline void sincos_small_d(double x, double* s, double* c) {
-  double sh, ch;
-  sincos_poly_d(0.5 * x, &sh, &ch);
-  *s = 2.0 * sh * ch;
-  *c = (ch - sh) * (ch + sh);
+constexpr double kTabScale = 40.74366543152521;      // 256 / (2 pi)
+constexpr double kTabStepHi = 0.02454369260617026;   // 2 pi / 256 rounded to double
+constexpr double kTabStepLo = 9.567553118338697e-19; // 2 pi / 256 - kTabStepHi, rounded
+
+__device__ inline void sincos_tab(double x, double* s, double* c) {
+  const double k = rint(x * kTabScale);
+  double r = fma(-k, kTabStepHi, x);
+  r = fma(-k, kTabStepLo, r);
+  const int idx = (int)fma(-256.0, floor(k * (1.0 / 256.0)), k);  // k mod 256, exact for |k| < 2^53
+  const double2 t = g_trig[idx];
+  const double r2 = r * r;
+  const double sr = fma(r * r2, fma(r2, 1.0 / 120.0, -1.0 / 6.0), r);
+  const double cr = fma(r2, fma(r2, fma(r2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+  *s = fma(t.x, cr, t.y * sr);
+  *c = fma(t.y, cr, -t.x * sr);
 }
 
 __device__ inline double rcp_d(double x) {
@@ -77,16 +100,22 @@ __device__ inline double rcp_d(double x) {
 }
 
 // FP64 argument reduced to revolutions in FP64, then the hardware FP32 sine/cosine
-// (v_sin_f32 / v_cos_f32 take revolutions); absolute error ~1e-7
-__device__ inline void sincos_rev_f(double x, double* s, double* c) {
+// (v_sin_f32 / v_cos_f32 take revolutions), with the rounding of the argument to FP32 undone
+// to first order and the pair brought back to unit modulus by one Newton step: what remains
+// is the FP32 units' own phase error (a few 1e-8 rad)
+__device__ inline void sincos_rev_fc(double x, double* s, double* c) {
   double r = x * kInv2Pi;
   r -= rint(r);
   const float rf = (float)r;
-  *s = (double)__builtin_amdgcn_sinf(rf);
-  *c = (double)__builtin_amdgcn_cosf(rf);
+  const double dr = (r - (double)rf) * (2.0 * 3.14159265358979323846);
+  const double sf = (double)__builtin_amdgcn_sinf(rf), cf = (double)__builtin_amdgcn_cosf(rf);
+  const double s1 = fma(cf, dr, sf), c1 = fma(-sf, dr, cf);
+  const double nrm = fma(-0.5, fma(s1, s1, c1 * c1), 1.5);
+  *s = s1 * nrm;
+  *c = c1 * nrm;
 }
 
-// the same for |x| <= 4 (|x / 2 pi| < 0.64 revolutions: no reduction needed)
+// FP32 hardware sine/cosine for |x| <= 4 (|x / 2 pi| < 0.64 revolutions: no reduction needed)
 __device__ inline void sincos_rev_small_f(double x, double* s, double* c) {
   const float rf = (float)(x * kInv2Pi);
   *s = (double)__builtin_amdgcn_sinf(rf);

@@ -301,6 +301,38 @@ __global__ __launch_bounds__(128) void k_reduce(const double* __restrict__ parti
   }
 }
 
+// ---- work accounting for the profiler: per row, the number of (bin, mode) pairs below the
+// mode cuts and of bins below the largest cut (the bins on which the fused kernel has work)
+__device__ inline int count_le(const double* __restrict__ f, int n, double x) {
+  int lo = 0, hi = n;  // first index with f > x (f ascending)
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (f[mid] <= x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+__global__ __launch_bounds__(64) void k_live(const DetectorDev* __restrict__ dets, int n_det,
+                                             const double* __restrict__ bin_f, const double* __restrict__ pconst,
+                                             ModeList modes, int P, double* __restrict__ live) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const double* pc = pconst + (size_t)p * PC_STRIDE;
+  double bm = 0.0, br = 0.0;
+  if (pc[PC_VALID] != 0.0) {
+    for (int d = 0; d < n_det; ++d) {
+      const double* f = bin_f + dets[d].bin_offset;
+      const int n = dets[d].n_bins;
+      for (int mi = 0; mi < modes.n; ++mi) bm += (double)count_le(f, n, pc[PC_MODE + mi * PC_MODE_STRIDE + MF_CUT]);
+      br += (double)count_le(f, n, pc[PC_FCUT]);
+    }
+  }
+  live[2 * (size_t)p] = bm;
+  live[2 * (size_t)p + 1] = br;
+}
+
 // ---- response / waveform output kernels (parity checks) -------------------------------------
 __global__ __launch_bounds__(128) void k_waveform(const DetectorDev* __restrict__ dets, int det_i, const double* __restrict__ bin_f,
                            const int* __restrict__ bin_j, const double* __restrict__ bin_s,
@@ -430,6 +462,8 @@ struct pes_context {
   // optional timing of the fused kernel (HIP events on the launch stream)
   bool profiling = false;
   std::vector<hipEvent_t> ev_start, ev_stop;
+  double* d_live = nullptr;  // [cap_live][2] live (bin, mode) and bin counts of the last profiled launch
+  int cap_live = 0, n_live = 0;
 };
 
 extern "C" const char* pes_last_error(void) { return g_err.c_str(); }
@@ -650,6 +684,7 @@ extern "C" void pes_context_destroy(pes_context* ctx) {
   free_dev(ctx->d_nodes);
   free_dev(ctx->d_nodes_grad);
   free_dev(ctx->d_partial);
+  free_dev(ctx->d_live);
   for (auto e : ctx->ev_start) (void)hipEventDestroy(e);
   for (auto e : ctx->ev_stop) (void)hipEventDestroy(e);
   delete ctx;
@@ -711,8 +746,8 @@ static int prepare(pes_context* ctx, const double* d_params, int P, bool grad, h
 
 extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, double* d_lnl, double* d_grad,
                              int precision, void* stream) {
-  if (precision != PES_PREC_FP64 && precision != PES_PREC_MIXED)
-    return fail(PES_ERR_INVALID, "precision must be PES_PREC_FP64 or PES_PREC_MIXED");
+  if (precision != PES_PREC_FP64 && precision != PES_PREC_MIXED && precision != PES_PREC_MIXED_FAST)
+    return fail(PES_ERR_INVALID, "precision must be PES_PREC_FP64, PES_PREC_MIXED or PES_PREC_MIXED_FAST");
   if (P > 0 && !d_lnl) return fail(PES_ERR_INVALID, "lnl output is NULL");
   hipStream_t st = (hipStream_t)stream;
   bool grad = d_grad != nullptr;
@@ -737,16 +772,13 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
   ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre,       \
       ctx->d_dim, ctx->NB, ctx->d_tile_f0, ctx->d_tile_dd, ctx->d_pconst, ctx->d_nodes, ctx->d_nodes_grad, NNS, \
       ctx->du, ctx->modes, P, ctx->d_partial
-  if (precision == PES_PREC_FP64) {
-    if (grad)
-      hipLaunchKernelGGL((k_lnl<false, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
-    else
-      hipLaunchKernelGGL((k_lnl<false, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
-  } else {
-    if (grad)
-      hipLaunchKernelGGL((k_lnl<true, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
-    else
-      hipLaunchKernelGGL((k_lnl<true, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+  switch (precision * 2 + (grad ? 1 : 0)) {
+    case 0: hipLaunchKernelGGL((k_lnl<0, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
+    case 1: hipLaunchKernelGGL((k_lnl<0, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
+    case 2: hipLaunchKernelGGL((k_lnl<1, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
+    case 3: hipLaunchKernelGGL((k_lnl<1, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
+    case 4: hipLaunchKernelGGL((k_lnl<2, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
+    default: hipLaunchKernelGGL((k_lnl<2, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
   }
 #undef PES_LNL_ARGS
   HIP_TRY(hipGetLastError());
@@ -754,6 +786,16 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
     HIP_TRY(hipEventRecord(e1, st));
     ctx->ev_start.push_back(e0);
     ctx->ev_stop.push_back(e1);
+    if (P > ctx->cap_live) {
+      free_dev(ctx->d_live);
+      ctx->d_live = nullptr;
+      HIP_TRY(hipMalloc(&ctx->d_live, (size_t)P * 2 * sizeof(double)));
+      ctx->cap_live = P;
+    }
+    hipLaunchKernelGGL(k_live, dim3((P + 63) / 64), dim3(64), 0, st, ctx->d_dets, n_det, ctx->d_bin_f,
+                       ctx->d_pconst, ctx->modes, P, ctx->d_live);
+    HIP_TRY(hipGetLastError());
+    ctx->n_live = P;
   }
   hipLaunchKernelGGL(k_reduce, dim3((P + 127) / 128), dim3(128), 0, st, ctx->d_partial, n_groups, P, nout,
                      ctx->d_pconst, d_lnl, d_grad);
@@ -763,22 +805,24 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
 
 extern "C" int pes_response(pes_context* ctx, int det, const double* d_params, int P, double* d_out, int precision,
                             void* stream) {
-  if (precision != PES_PREC_FP64 && precision != PES_PREC_MIXED)
-    return fail(PES_ERR_INVALID, "precision must be PES_PREC_FP64 or PES_PREC_MIXED");
+  if (precision != PES_PREC_FP64 && precision != PES_PREC_MIXED && precision != PES_PREC_MIXED_FAST)
+    return fail(PES_ERR_INVALID, "precision must be PES_PREC_FP64, PES_PREC_MIXED or PES_PREC_MIXED_FAST");
   if (!ctx || det < 0 || det >= (int)ctx->hdets.size()) return fail(PES_ERR_INVALID, "bad detector index");
   if (P > 0 && !d_out) return fail(PES_ERR_INVALID, "out is NULL");
   hipStream_t st = (hipStream_t)stream;
   int rc = prepare(ctx, d_params, P, false, st);
   if (rc != PES_OK || P == 0) return rc;
   int n = ctx->hdets[det].d.n_bins;
+#define PES_RESP_ARGS                                                                                          \
+  dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s,     \
+      ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1, ctx->du, ctx->modes, P, d_out
   if (precision == PES_PREC_FP64)
-    hipLaunchKernelGGL(k_response_t<false>, dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det,
-                       ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1,
-                       ctx->du, ctx->modes, P, d_out);
+    hipLaunchKernelGGL(k_response_t<0>, PES_RESP_ARGS);
+  else if (precision == PES_PREC_MIXED)
+    hipLaunchKernelGGL(k_response_t<1>, PES_RESP_ARGS);
   else
-    hipLaunchKernelGGL(k_response_t<true>, dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det,
-                       ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1,
-                       ctx->du, ctx->modes, P, d_out);
+    hipLaunchKernelGGL(k_response_t<2>, PES_RESP_ARGS);
+#undef PES_RESP_ARGS
   HIP_TRY(hipGetLastError());
   return PES_OK;
 }
@@ -853,5 +897,21 @@ extern "C" int pes_profile_read(pes_context* ctx, int* n_launches, double* total
   *total_ms = tot;
   ctx->ev_start.clear();
   ctx->ev_stop.clear();
+  return PES_OK;
+}
+
+extern "C" int pes_profile_work(pes_context* ctx, double* bin_mode_rows, double* bin_rows) {
+  if (!ctx || !bin_mode_rows || !bin_rows) return fail(PES_ERR_INVALID, "NULL argument");
+  *bin_mode_rows = 0.0;
+  *bin_rows = 0.0;
+  if (ctx->n_live == 0) return PES_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipDeviceSynchronize());
+  std::vector<double> h((size_t)ctx->n_live * 2);
+  HIP_TRY(hipMemcpy(h.data(), ctx->d_live, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int p = 0; p < ctx->n_live; ++p) {
+    *bin_mode_rows += h[2 * (size_t)p];
+    *bin_rows += h[2 * (size_t)p + 1];
+  }
   return PES_OK;
 }

@@ -1,9 +1,13 @@
-"""Mixed precision (FP32 bulk, FP64 phases and reductions) vs the FP64 path.
+"""Mixed precision vs the FP64 path.
 
-Near the likelihood peak (where samplers spend their time) the FP32 rounding of
-the response enters lnL through <d - h, dh> ~ SNR * eps and must stay below 1e-3.
-Far from the peak the error grows like SNR^2 * eps; those rows are reported, not
-asserted.
+All arithmetic stays FP64; "mixed" takes the total-phase sine/cosine from the FP32 units and
+"mixed_fast" also the transfer-function sines, cosines and reciprocals.
+
+Near the likelihood peak the error enters lnL through <d - h, dh> ~ SNR eps and both modes
+hold |dlnL| < 1e-3.  Far from the peak the FP32 phase error of "mixed", which changes from
+bin to bin, adds up like a random walk over the bins, 2 sum_k w_k Im(d_k* h_k) dphi_k: a few
+1e-3 at SNR ~ 2000 (asserted below 5e-3); the transfer-function errors of "mixed_fast" are
+coherent across bins and grow like SNR^2 eps (reported, not asserted).
 """
 
 import numpy as np
@@ -22,28 +26,31 @@ def _core(modes, d, labels):
     return core
 
 
+@pytest.mark.parametrize("precision", ["mixed", "mixed_fast"])
 @pytest.mark.parametrize("modes,gen", [(((2, 2),), "1.5"), (oph.MODES, "1.5"), (oph.MODES, "2.0")])
-def test_mixed_near_peak(modes, gen):
+def test_mixed_accuracy(modes, gen, precision):
     d, labels = make_detector(modes=modes, gen=gen)
     core = _core(modes, d, labels)
     near = np.vstack([inj_row(), particle_cloud(63, seed=5, rel=1e-8)])
     far = particle_cloud(16, seed=6, rel=1e-4)
     rows = np.vstack([near, far])
     l64, g64 = core.lnl(rows, grad=True, precision="fp64")
-    lmx, gmx = core.lnl(rows, grad=True, precision="mixed")
+    lmx, gmx = core.lnl(rows, grad=True, precision=precision)
     l64, lmx, g64, gmx = (x.cpu().numpy() for x in (l64, lmx, g64, gmx))
     dl = np.abs(lmx - l64)
     n = near.shape[0]
     scale = np.abs(g64[:n]).max(axis=0)
     rel = np.abs(gmx[:n] - g64[:n]) / scale
-    print(f"near: dlnl max {dl[:n].max():.2e} p99 {np.percentile(dl[:n], 99):.2e}; "
+    print(f"{precision}: near: dlnl max {dl[:n].max():.2e} p99 {np.percentile(dl[:n], 99):.2e}; "
           f"far: dlnl max {dl[n:].max():.2e}; lnl-range near {l64[:n].min():.1f}..{l64[:n].max():.1f}; "
-          f"grad rel (to max |g|) max {rel.max():.2e}")
+          f"far {l64[n:].min():.3g}..{l64[n:].max():.3g}; grad rel (to max |g|) max {rel.max():.2e}")
     assert dl[:n].max() < 1e-3
+    if precision == "mixed":
+        assert dl[n:].max() < 5e-3
     # near the peak the gradient is O(SNR) while its FP32 error is O(eps SNR^2): ~1e-4 relative
     assert rel.max() < 5e-3
     r64 = core.response(0, rows[:2], precision="fp64").cpu().numpy()
-    rmx = core.response(0, rows[:2], precision="mixed").cpu().numpy()
+    rmx = core.response(0, rows[:2], precision=precision).cpu().numpy()
     err = np.linalg.norm(rmx - r64) / np.linalg.norm(r64)
     print("response rel err", err)
     assert err < 2e-6
