@@ -535,15 +535,24 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
                                               const double* __restrict__ tile_f0, const double* __restrict__ tile_dd,
                                               const double* __restrict__ pconst, const double* __restrict__ nodes,
                                               const double* __restrict__ nodes_grad, int NNS, double du,
-                                              ModeList modes, int P, double* __restrict__ partial) {
+                                              ModeList modes, int P, int* __restrict__ work_counter,
+                                              double* __restrict__ partial) {
   constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
   __shared__ double acc[TILE / 64][CHUNK][NOUT];
+  __shared__ int next_work;
   trig_table_load();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n_chunks = (P + CHUNK - 1) / CHUNK;
   const int n_groups = (n_tiles + TILES_PER_GROUP - 1) / TILES_PER_GROUP;
   const int n_work = n_chunks * n_groups;
-  for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
+  // work items are taken from a counter (zeroed by k_setup earlier on the same stream), so
+  // blocks that draw cheap items (tiles above the mode cuts) take more of them; every block
+  // leaves once the counter passes n_work
+  for (;;) {
+    if (threadIdx.x == 0) next_work = atomicAdd(work_counter, 1);
+    __syncthreads();
+    const int w = next_work;
+    if (w >= n_work) break;
     const int g = w / n_chunks, c = w % n_chunks;
     const int p0 = c * CHUNK, p1 = min(P, p0 + CHUNK);
     for (int i = threadIdx.x; i < (TILE / 64) * CHUNK * NOUT; i += TILE) (&acc[0][0][0])[i] = 0.0;

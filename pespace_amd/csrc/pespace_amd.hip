@@ -81,8 +81,9 @@ __device__ inline Cplx mode_factor(int id) {
 
 // ---- k_setup ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_setup(const double* __restrict__ params, int P, ModeList modes, double f_ref,
-                        double* __restrict__ setup, double* __restrict__ pconst) {
+                        double* __restrict__ setup, double* __restrict__ pconst, int* __restrict__ work_counter) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) work_counter[0] = 0;  // work queue of the fused kernel that follows on this stream
   if (p >= P) return;
   const double* th = params + (size_t)p * PES_N_PARAMS;
   double Mc = th[0], q = th[1], c1 = th[2], c2 = th[3], dL = th[4], phi = th[5], iota = th[6];
@@ -459,6 +460,7 @@ struct pes_context {
   bool cap_grad = false;
   double *d_setup = nullptr, *d_pconst = nullptr, *d_nodes = nullptr, *d_nodes_grad = nullptr,
          *d_partial = nullptr;
+  int* d_work_counter = nullptr;
   int n_cu = 256;
   // optional timing of the fused kernel (HIP events on the launch stream)
   bool profiling = false;
@@ -664,6 +666,7 @@ extern "C" int pes_context_finalize(pes_context* ctx) {
     HIP_TRY(hipMemcpy(ctx->d_tile_dd, tdd.data(), nt * sizeof(double), hipMemcpyHostToDevice));
   }
   if (ctx->direct) ctx->d_node_f = ctx->d_bin_f;
+  HIP_TRY(hipMalloc(&ctx->d_work_counter, sizeof(int)));
   ctx->finalized = true;
   return PES_OK;
 }
@@ -679,6 +682,7 @@ extern "C" void pes_context_destroy(pes_context* ctx) {
   free_dev(ctx->d_dre);
   free_dev(ctx->d_dim);
   free_dev(ctx->d_tile_f0);
+  free_dev(ctx->d_work_counter);
   free_dev(ctx->d_tile_dd);
   free_dev(ctx->d_setup);
   free_dev(ctx->d_pconst);
@@ -731,7 +735,7 @@ static int prepare(pes_context* ctx, const double* d_params, int P, bool grad, h
   if (rc != PES_OK) return rc;
   if (P == 0) return PES_OK;
   hipLaunchKernelGGL(k_setup, dim3((P + 63) / 64), dim3(64), 0, st, d_params, P, ctx->modes, ctx->f_ref,
-                     ctx->d_setup, ctx->d_pconst);
+                     ctx->d_setup, ctx->d_pconst, ctx->d_work_counter);
   HIP_TRY(hipGetLastError());
   const int NNS = ctx->n_nodes + 1;
   dim3 g((NNS + 127) / 128, P * ctx->modes.n);
@@ -758,7 +762,7 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
   const int n_groups = n_groups_of(ctx);
   const int n_chunks = (P + CHUNK - 1) / CHUNK;
   const int n_work = n_groups * n_chunks;
-  int grid = ctx->n_cu * 8;
+  int grid = ctx->n_cu * 4;
   if (grid > n_work) grid = n_work;
   int nout = 1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -772,7 +776,7 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
 #define PES_LNL_ARGS                                                                                         \
   ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre,       \
       ctx->d_dim, ctx->NB, ctx->d_tile_f0, ctx->d_tile_dd, ctx->d_pconst, ctx->d_nodes, ctx->d_nodes_grad, NNS, \
-      ctx->du, ctx->modes, P, ctx->d_partial
+      ctx->du, ctx->modes, P, ctx->d_work_counter, ctx->d_partial
   switch (precision * 2 + (grad ? 1 : 0)) {
     case 0: hipLaunchKernelGGL((k_lnl<0, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
     case 1: hipLaunchKernelGGL((k_lnl<0, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
