@@ -191,6 +191,7 @@ def main():
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--precision", default="all", choices=("all", "fp64", "mixed", "mixed_fast"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule of the fused kernel")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -218,6 +219,18 @@ def main():
     setup_s = time.perf_counter() - t_setup
     print(f"[bench] workload={args.workload} bins={nbins} modes={len(wf.modes)} P={args.particles} grad={grad} "
           f"setup {setup_s:.1f}s", file=sys.stderr, flush=True)
+
+    tune = None
+    if not args.no_autotune:
+        # schedule chosen by the autotuner on a 512-row sample of the batch (untimed)
+        t0 = time.perf_counter()
+        best, times = core.autotune(params[:min(512, args.particles)], grad=grad,
+                                    precision="mixed" if args.precision == "all" else args.precision,
+                                    candidates=[(t, b) for t in (8, 16, 32, 64) for b in (2, 4)], reps=1)
+        tune = {"tiles_per_group": best[0], "blocks_per_cu": best[1],
+                "candidates_ms": {f"{t}x{b}": round(ms, 2) for (t, b), ms in times.items()}}
+        print(f"[bench] autotune {time.perf_counter() - t0:.1f}s: best {best}; "
+              + ", ".join(f"{k}: {v}" for k, v in tune["candidates_ms"].items()), file=sys.stderr, flush=True)
 
     precs = ("fp64", "mixed", "mixed_fast") if args.precision == "all" else (args.precision,)
     res = {}
@@ -259,7 +272,8 @@ def main():
         "dtype": "f64" if head == "fp64" else "f64+f32trig",
         "data": "synthetic (seeded noise + injected signal, random particle cloud)",
         "config": {"workload": args.workload, "particles": P, "bins_per_detector": nbins, "modes": list(wf.modes),
-                   "nodes": wf.n_nodes, "gradient": grad, "precision": head},
+                   "nodes": wf.n_nodes, "gradient": grad, "precision": head,
+                   "schedule": tune if tune else dict(zip(("tiles_per_group", "blocks_per_cu"), core.schedule()))},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": traffic,
                      "work": {"bin_mode_rows": h["bm"], "bin_rows": h["br"], "flops": flops},

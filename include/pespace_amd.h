@@ -82,7 +82,8 @@ int pes_context_finalize(pes_context* ctx);
 
 void pes_context_destroy(pes_context* ctx);
 
-/* Batched lnL (and gradient) for P parameter rows.
+/* Batched lnL (and gradient) for P parameter rows.  A context serves one stream at a time:
+ * its scratch and work queue are shared by the calls made on it.
  * Replaces compute_whittle_likelihood_ti (likelihood_ti.py:17-69) summed over the
  * detectors (interface_bilby.py:97-116), vmapped over particles
  * (sampler/base.py:27-35).  d_params: device [P][11]; d_lnl: device [P];
@@ -120,6 +121,14 @@ int pes_response_from_waveform(pes_context* ctx, int det, int n_modes, const dou
 #define PES_WHITTLE_WORK 1024
 int pes_whittle_lnl(const double* d_data, const double* d_resp, const double* d_psd, int n_chan, int n, double df,
                     double* d_work, double* d_out, void* stream);
+
+/* Schedule of the fused kernel: work items of tiles_per_group 256-bin tiles x 16 rows, taken
+ * from a queue by blocks_per_cu x (number of CUs) persistent blocks of 256 threads.  Defaults
+ * 32 and 4; pespace_amd.core.LikelihoodCore.autotune picks them by timing.  Changing the
+ * schedule may reallocate scratch at the next call: set it before capturing a HIP graph.
+ * No counterpart in the reference. */
+int pes_context_set_schedule(pes_context* ctx, int tiles_per_group, int blocks_per_cu);
+int pes_context_get_schedule(const pes_context* ctx, int* tiles_per_group, int* blocks_per_cu);
 
 /* Timing of the fused lnL kernel: when enabled, pes_lnl_batch records a HIP event
  * pair around the fused kernel on its stream.  pes_profile_read waits for the

@@ -41,6 +41,8 @@ SIGNATURES = [
     ("pes_whittle_lnl", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("pes_context_set_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("pes_context_get_schedule", ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p]),
     ("pes_context_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("pes_profile_read", ctypes.c_int, [ctypes.c_void_p, c_int_p, c_double_p]),
     ("pes_profile_work", ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p]),

@@ -128,6 +128,47 @@ class LikelihoodCore:
                                   PRECISION[precision], stream))
         return (lnl, g) if grad else lnl
 
+    # -- schedule of the fused kernel ------------------------------------------------------
+    def set_schedule(self, tiles_per_group, blocks_per_cu):
+        """Work items of ``tiles_per_group`` 256-bin tiles (x 16 rows), ``blocks_per_cu``
+        persistent blocks per CU."""
+        self._build()
+        check(lib().pes_context_set_schedule(self._ctx, int(tiles_per_group), int(blocks_per_cu)))
+
+    def schedule(self):
+        self._build()
+        t, b = ctypes.c_int(), ctypes.c_int()
+        check(lib().pes_context_get_schedule(self._ctx, ctypes.byref(t), ctypes.byref(b)))
+        return t.value, b.value
+
+    def autotune(self, params, grad=True, precision="fp64", candidates=None, reps=2, warmup=1):
+        """Time the fused kernel on ``params`` for each candidate (tiles_per_group, blocks_per_cu)
+        with HIP events, keep the fastest, and return it with every candidate's mean time (ms).
+
+        The default candidates cover work items of 8-64 tiles and 2-8 persistent blocks per CU
+        (with two waves per SIMD, 2 blocks per CU fill the machine once; more add queue slack).
+        """
+        self._build()
+        p = self._params(params)
+        P = p.shape[0]
+        cands = candidates or [(t, b) for t in (8, 16, 32, 64) for b in (2, 4, 8)]
+        lnl = torch.empty(P, dtype=torch.float64, device=p.device)
+        g = torch.empty(P, N_PARAMS, dtype=torch.float64, device=p.device) if grad else None
+        times = {}
+        for t, b in cands:
+            self.set_schedule(t, b)
+            for _ in range(warmup):
+                self.lnl(p, grad=grad, precision=precision, out=lnl, grad_out=g)
+            self.set_profiling(True)
+            for _ in range(reps):
+                self.lnl(p, grad=grad, precision=precision, out=lnl, grad_out=g)
+            n, ms = self.read_profile()
+            self.set_profiling(False)
+            times[(t, b)] = ms / max(n, 1)
+        best = min(times, key=times.get)
+        self.set_schedule(*best)
+        return best, times
+
     def set_profiling(self, enable=True):
         self._build()
         check(lib().pes_context_set_profiling(self._ctx, int(bool(enable))))

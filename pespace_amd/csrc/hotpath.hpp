@@ -523,7 +523,7 @@ __device__ inline void wave_sum12(const double x[12], int lane, double z[3]) {
 }
 
 // ------------------------------- kernels ---------------------------------------------------
-// Persistent fused kernel.  Work item = (group of TILES_PER_GROUP tiles, chunk of CHUNK rows);
+// Persistent fused kernel.  Work item = (group of tiles_per_group tiles, chunk of CHUNK rows);
 // lane <-> bin.  A tile whose first frequency lies above every mode's cut of a row has a
 // zero response for that row: its lnL contribution is the precomputed -sum w |d|^2 of the
 // tile and its gradient contribution is zero, so the row skips it.
@@ -533,7 +533,8 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
                                               const double* __restrict__ bin_s, const double* __restrict__ bin_w,
                                               const double* __restrict__ dre, const double* __restrict__ dim, int NB,
                                               const double* __restrict__ tile_f0, const double* __restrict__ tile_dd,
-                                              const double* __restrict__ pconst, const double* __restrict__ nodes,
+                                              int tiles_per_group, const double* __restrict__ pconst,
+                                              const double* __restrict__ nodes,
                                               const double* __restrict__ nodes_grad, int NNS, double du,
                                               ModeList modes, int P, int* __restrict__ work_counter,
                                               double* __restrict__ partial) {
@@ -543,7 +544,7 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
   trig_table_load();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n_chunks = (P + CHUNK - 1) / CHUNK;
-  const int n_groups = (n_tiles + TILES_PER_GROUP - 1) / TILES_PER_GROUP;
+  const int n_groups = (n_tiles + tiles_per_group - 1) / tiles_per_group;
   const int n_work = n_chunks * n_groups;
   // work items are taken from a counter (zeroed by k_setup earlier on the same stream), so
   // blocks that draw cheap items (tiles above the mode cuts) take more of them; every block
@@ -559,8 +560,8 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
     double chunk_cut = 0.0;
     for (int p = p0; p < p1; ++p) chunk_cut = fmax(chunk_cut, pconst[(size_t)p * PC_STRIDE + PC_FCUT]);
     __syncthreads();
-    const int t_end = min(n_tiles, (g + 1) * TILES_PER_GROUP);
-    for (int tile = g * TILES_PER_GROUP; tile < t_end; ++tile) {
+    const int t_end = min(n_tiles, (g + 1) * tiles_per_group);
+    for (int tile = g * tiles_per_group; tile < t_end; ++tile) {
       const double f0 = tile_f0[tile];
       if (f0 > chunk_cut) {  // no row of the chunk has a mode up here
         if (threadIdx.x == 0) {

@@ -41,7 +41,8 @@ int fail(int code, const std::string& msg) {
 
 constexpr int TILE = 256;      // bins per tile = threads per block
 constexpr int CHUNK = 16;      // parameter rows per work item
-constexpr int TILES_PER_GROUP = 32;
+constexpr int TILES_PER_GROUP = 32;  // default schedule: tiles per work item ...
+constexpr int BLOCKS_PER_CU = 4;     // ... and persistent blocks launched per CU
 
 // ---- per-row constant block ------------------------------------------------------------
 enum PConst {
@@ -458,6 +459,10 @@ struct pes_context {
   // scratch, grown on demand
   int cap_P = 0;
   bool cap_grad = false;
+  size_t cap_partial = 0;  // doubles
+  // schedule of the fused kernel (pes_context_set_schedule)
+  int tiles_per_group = TILES_PER_GROUP;
+  int blocks_per_cu = BLOCKS_PER_CU;
   double *d_setup = nullptr, *d_pconst = nullptr, *d_nodes = nullptr, *d_nodes_grad = nullptr,
          *d_partial = nullptr;
   int* d_work_counter = nullptr;
@@ -700,10 +705,22 @@ extern "C" int pes_detector_bins(const pes_context* ctx, int det) {
   return ctx->hdets[det].d.n_bins;
 }
 
-static int n_groups_of(const pes_context* ctx) { return (ctx->n_tiles + TILES_PER_GROUP - 1) / TILES_PER_GROUP; }
+static int n_groups_of(const pes_context* ctx) {
+  return (ctx->n_tiles + ctx->tiles_per_group - 1) / ctx->tiles_per_group;
+}
 
-// (re)allocate scratch for P rows; allocation happens only when P or the gradient flag grows
+// (re)allocate scratch for P rows; allocation happens only when P, the gradient flag or the
+// number of tile groups grows
 static int ensure_scratch(pes_context* ctx, int P, bool grad) {
+  {
+    const size_t need = (size_t)n_groups_of(ctx) * (1 + PES_N_PARAMS) * (size_t)(P > ctx->cap_P ? P : ctx->cap_P);
+    if (need > ctx->cap_partial) {
+      free_dev(ctx->d_partial);
+      ctx->d_partial = nullptr;
+      HIP_TRY(hipMalloc(&ctx->d_partial, need * sizeof(double)));
+      ctx->cap_partial = need;
+    }
+  }
   if (P <= ctx->cap_P && (!grad || ctx->cap_grad)) return PES_OK;
   int cap = P > ctx->cap_P ? P : ctx->cap_P;
   bool g = grad || ctx->cap_grad;
@@ -711,15 +728,12 @@ static int ensure_scratch(pes_context* ctx, int P, bool grad) {
   free_dev(ctx->d_pconst);
   free_dev(ctx->d_nodes);
   free_dev(ctx->d_nodes_grad);
-  free_dev(ctx->d_partial);
-  ctx->d_setup = ctx->d_pconst = ctx->d_nodes = ctx->d_nodes_grad = ctx->d_partial = nullptr;
+  ctx->d_setup = ctx->d_pconst = ctx->d_nodes = ctx->d_nodes_grad = nullptr;
   size_t NNS = (size_t)ctx->n_nodes + 1;
   HIP_TRY(hipMalloc(&ctx->d_setup, (size_t)cap * N_SETUP * 5 * sizeof(double)));
   HIP_TRY(hipMalloc(&ctx->d_pconst, (size_t)cap * PC_STRIDE * sizeof(double)));
   HIP_TRY(hipMalloc(&ctx->d_nodes, (size_t)cap * ctx->modes.n * NNS * 4 * sizeof(double)));
   if (g) HIP_TRY(hipMalloc(&ctx->d_nodes_grad, (size_t)cap * ctx->modes.n * NNS * 16 * sizeof(double)));
-  size_t nout = g ? 1 + PES_N_PARAMS : 1;
-  HIP_TRY(hipMalloc(&ctx->d_partial, (size_t)n_groups_of(ctx) * nout * cap * sizeof(double)));
   ctx->cap_P = cap;
   ctx->cap_grad = g;
   return PES_OK;
@@ -762,7 +776,7 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
   const int n_groups = n_groups_of(ctx);
   const int n_chunks = (P + CHUNK - 1) / CHUNK;
   const int n_work = n_groups * n_chunks;
-  int grid = ctx->n_cu * 4;
+  int grid = ctx->n_cu * ctx->blocks_per_cu;
   if (grid > n_work) grid = n_work;
   int nout = 1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -775,8 +789,8 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
   if (grad) nout = 1 + PES_N_PARAMS;
 #define PES_LNL_ARGS                                                                                         \
   ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre,       \
-      ctx->d_dim, ctx->NB, ctx->d_tile_f0, ctx->d_tile_dd, ctx->d_pconst, ctx->d_nodes, ctx->d_nodes_grad, NNS, \
-      ctx->du, ctx->modes, P, ctx->d_work_counter, ctx->d_partial
+      ctx->d_dim, ctx->NB, ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, ctx->d_pconst, ctx->d_nodes,   \
+      ctx->d_nodes_grad, NNS, ctx->du, ctx->modes, P, ctx->d_work_counter, ctx->d_partial
   switch (precision * 2 + (grad ? 1 : 0)) {
     case 0: hipLaunchKernelGGL((k_lnl<0, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
     case 1: hipLaunchKernelGGL((k_lnl<0, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
@@ -877,6 +891,22 @@ extern "C" int pes_whittle_lnl(const double* d_data, const double* d_resp, const
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_whittle_final, dim3(1), dim3(64), 0, (hipStream_t)stream, d_work, nb, df, d_out);
   HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" int pes_context_set_schedule(pes_context* ctx, int tiles_per_group, int blocks_per_cu) {
+  if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
+  if (tiles_per_group < 1 || tiles_per_group > 4096 || blocks_per_cu < 1 || blocks_per_cu > 64)
+    return fail(PES_ERR_INVALID, "schedule: tiles_per_group in [1, 4096], blocks_per_cu in [1, 64]");
+  ctx->tiles_per_group = tiles_per_group;
+  ctx->blocks_per_cu = blocks_per_cu;
+  return PES_OK;
+}
+
+extern "C" int pes_context_get_schedule(const pes_context* ctx, int* tiles_per_group, int* blocks_per_cu) {
+  if (!ctx || !tiles_per_group || !blocks_per_cu) return fail(PES_ERR_INVALID, "NULL argument");
+  *tiles_per_group = ctx->tiles_per_group;
+  *blocks_per_cu = ctx->blocks_per_cu;
   return PES_OK;
 }
 

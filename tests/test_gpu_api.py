@@ -138,3 +138,27 @@ def test_hip_graph_capture_of_lnl_grad_step(precision):
         torch.cuda.synchronize()
         assert torch.equal(lnl, l_e)
         assert torch.equal(g, g_e)
+
+
+def test_schedules_agree_and_autotune_picks_one():
+    """Every schedule of the fused kernel gives the same lnL and gradient (up to the order of
+    the partial sums), and the autotuner keeps the fastest candidate it timed."""
+    import torch
+
+    from pespace_amd.inference.likelihood import WhittleLikelihoodHip
+    det, wf = _host_detector()
+    lk = WhittleLikelihoodHip(wf, det, ("A", "E", "T"))
+    core = lk.core
+    rows = np.vstack([inj_row(), particle_cloud(40, seed=11)])
+    ref_l, ref_g = (x.cpu().numpy() for x in core.lnl(rows, grad=True))
+    scale = np.abs(ref_g).max(axis=0)
+    for t, b in ((1, 1), (3, 2), (64, 8), (4096, 1)):
+        core.set_schedule(t, b)
+        assert core.schedule() == (t, b)
+        l, g = (x.cpu().numpy() for x in core.lnl(rows, grad=True))
+        np.testing.assert_allclose(l, ref_l, rtol=1e-12)
+        assert np.max(np.abs(g - ref_g) / scale) < 1e-11
+    best, times = core.autotune(rows, grad=True, candidates=[(8, 2), (32, 4)], reps=1, warmup=0)
+    assert best in times and times[best] == min(times.values())
+    assert core.schedule() == best
+    torch.cuda.synchronize()

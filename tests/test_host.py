@@ -56,6 +56,9 @@ def test_c_abi_argument_validation():
     assert L.pes_lnl_batch(None, None, 1, None, None, 7, None) == 1
     assert L.pes_whittle_lnl(None, None, None, 1, 1, 1.0, None, None, None) == 1
     assert L.pes_detector_bins(None, 0) == -1
+    assert L.pes_context_set_schedule(None, 32, 4) == 1
+    t, b = ctypes.c_int(), ctypes.c_int()
+    assert L.pes_context_get_schedule(None, ctypes.byref(t), ctypes.byref(b)) == 1
 
 
 def test_product_grid_psd_noise_match_oracle():
