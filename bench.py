@@ -41,9 +41,15 @@ DETECTORS = {
     "Taiji": ("Taiji_analytic", "Taiji_TDC", 1e-4, 0.06),
     "TianQin": ("Tianqin_analytic", "Tianqin_GWSpace", 1e-3, 0.039),
 }
+HM = ("22", "21", "33", "32", "44", "43")
 WORKLOADS = {
-    "network_hm_1yr": dict(detectors=("LISA", "Taiji", "TianQin"), modes=("22", "21", "33", "32", "44", "43")),
-    "lisa_22_1yr": dict(detectors=("LISA",), modes=("22",)),
+    # 1-yr rfft grids, 1024 interpolation nodes, particle cloud of width 1e-3 around the injection
+    "network_hm_1yr": dict(detectors=("LISA", "Taiji", "TianQin"), modes=HM, grid="year", cloud="local"),
+    # the same network with particles from the wide prior (t_c anywhere in the year, M 1e5-1e7)
+    "network_hm_prior": dict(detectors=("LISA", "Taiji", "TianQin"), modes=HM, grid="year", cloud="prior"),
+    "lisa_22_1yr": dict(detectors=("LISA",), modes=("22",), grid="year", cloud="local"),
+    # 2^14 log-spaced bins over 1e-4-0.05 Hz, PhenomD evaluated at every bin (no interpolation)
+    "lisa_22_log16k": dict(detectors=("LISA",), modes=("22",), grid="log16k", cloud="local"),
 }
 
 # Algorithmic FP64 work of the fused kernel, counted from its source with add/sub/mul = 1 and
@@ -73,21 +79,41 @@ def build_network(workload, seed=7):
     for i, name in enumerate(wl["detectors"]):
         orbit, psd, fmin, fmax = DETECTORS[name]
         td = TDIChannelData(backend="hip")
-        td.set_fd_data_from_zero(("A", "E", "T"), YEAR_SI, 5.0, 0.0, fmin, fmax)
+        if wl["grid"] == "year":
+            td.set_fd_data_from_zero(("A", "E", "T"), YEAR_SI, 5.0, 0.0, fmin, fmax)
+        else:
+            f = np.geomspace(fmin, fmax, 2 ** 14)
+            td.set_fd_data_on_grid(("A", "E", "T"), f, np.gradient(f))
         td.set_fd_noise_power_density_from_model(psd, tdi_generation="1.5")
         det = InterferometerAntenna(name, td, available_orbit_models()[orbit], FDResponseModelMarset2018(),
                                     FDMichelsonConstantEqualArm("1.5", True), backend="hip")
         dets.append(det)
     fmin = min(DETECTORS[n][2] for n in wl["detectors"])
     fmax = max(DETECTORS[n][3] for n in wl["detectors"])
-    wf = IMRPhenomHM(dets[0].tdi_data.frequency_samples, 0.0, modes=wl["modes"], n_nodes=1024, f_node_min=fmin,
-                     f_node_max=fmax)
+    wf = IMRPhenomHM(dets[0].tdi_data.frequency_samples, 0.0, modes=wl["modes"],
+                     n_nodes=1024 if wl["grid"] == "year" else 0, f_node_min=fmin, f_node_max=fmax)
     wf.update_waveform(INJ)
     for i, det in enumerate(dets):
         det.tdi_data.add_into_fd_data(det.tdi_data.get_fd_noise_realization(seed + i))
         det.inject_signal(wf.waveform_container, INJ["ecliptic_longitude"], INJ["ecliptic_latitude"],
                           INJ["polarization"], INJ["coalescence_time"])
     return dets, wf
+
+
+def prior_cloud(P, seed=2027):
+    """The wide prior of the benchmark configurations: M_tot log-uniform 1e5-1e7 Msun, q = m1/m2
+    uniform 1-8, chi uniform [-0.9, 0.9], d_L uniform 5-50 Gpc, sky uniform on the sphere,
+    cos(inclination) uniform, polarization and phase uniform, t_c uniform in [0, 1 yr]."""
+    rng = np.random.default_rng(seed)
+    M = 10.0 ** rng.uniform(5.0, 7.0, P)
+    q = rng.uniform(1.0, 8.0, P)
+    eta = q / (1.0 + q) ** 2
+    rows = np.column_stack([M * eta ** 0.6, 1.0 / q, rng.uniform(-0.9, 0.9, P), rng.uniform(-0.9, 0.9, P),
+                            rng.uniform(5000.0, 50000.0, P), rng.uniform(0.0, 2 * PI, P),
+                            np.arccos(rng.uniform(-1.0, 1.0, P)), rng.uniform(0.0, 2 * PI, P),
+                            np.arcsin(rng.uniform(-1.0, 1.0, P)), rng.uniform(0.0, PI, P),
+                            rng.uniform(0.0, YEAR_SI, P)])
+    return rows
 
 
 def particle_cloud(P, seed=2026, rel=1e-3):
@@ -152,10 +178,13 @@ def cpu_baseline(dets, wf, rows, budget_s=15.0):
         psd = {c: np.asarray(getattr(td.fd_noise_power_density, c))[sl] for c in ("A", "E", "T")}
         orb = d.orbit_model
         kind = "helio" if orb.kind == 0 else "geo"
-        odets.append(olk.Detector(f, data, psd, td.data_info.delta_frequency,
+        df = td.data_info.delta_frequency
+        if df is None:  # explicit grid: the sampled bins carry the weight of the bins they stand for
+            df = np.asarray(td.frequency_weights())[sl] * stride
+        odets.append(olk.Detector(f, data, psd, df,
                                   (kind, orb.arm_length, orb.rotation_initial, orb.revolution_initial), "1.5"))
     sample_bins = sum(o.f.size for o in odets)
-    interp = (wf.f_node_min, wf.f_node_max, wf.n_nodes)
+    interp = (wf.f_node_min, wf.f_node_max, wf.n_nodes) if wf.n_nodes > 0 else None
     n_eval, t0 = 0, time.perf_counter()
     while True:
         olk.lnl(olk.as_dict(rows[n_eval % rows.shape[0]]), odets, modes, wf.reference_frequency, interp)
@@ -213,7 +242,8 @@ def main():
                           f_node_min=wf.f_node_min, f_node_max=wf.f_node_max)
     for d in dets:
         add_detector_to_core(core, d, data_channels=("A", "E", "T"))
-    rows = particle_cloud(args.particles)
+    rows = particle_cloud(args.particles) if WORKLOADS[args.workload]["cloud"] == "local" \
+        else prior_cloud(args.particles)
     params = torch.as_tensor(rows, device="cuda")
     nbins = core.n_bins
     setup_s = time.perf_counter() - t_setup

@@ -31,6 +31,29 @@ def particle_cloud(n, seed=2026, rel=1e-3, center=None):
     return rows
 
 
+def prior_rows(n, seed=3, tc_range=(0.0, 655360.0)):
+    """Rows from the wide prior of the benchmark configurations: M_tot log-uniform 1e5-1e7 Msun,
+    q = m1/m2 uniform 1-8, chi uniform [-0.9, 0.9], d_L uniform 5-50 Gpc, sky uniform on the
+    sphere, cos(inclination) uniform, polarization and phase uniform, t_c uniform in tc_range."""
+    rng = np.random.default_rng(seed)
+    M = 10.0 ** rng.uniform(5.0, 7.0, n)
+    q = rng.uniform(1.0, 8.0, n)
+    eta = q / (1.0 + q) ** 2
+    rows = np.empty((n, 11))
+    rows[:, 0] = M * eta ** 0.6
+    rows[:, 1] = 1.0 / q
+    rows[:, 2] = rng.uniform(-0.9, 0.9, n)
+    rows[:, 3] = rng.uniform(-0.9, 0.9, n)
+    rows[:, 4] = rng.uniform(5000.0, 50000.0, n)
+    rows[:, 5] = rng.uniform(0.0, 2 * np.pi, n)
+    rows[:, 6] = np.arccos(rng.uniform(-1.0, 1.0, n))
+    rows[:, 7] = rng.uniform(0.0, 2 * np.pi, n)
+    rows[:, 8] = np.arcsin(rng.uniform(-1.0, 1.0, n))
+    rows[:, 9] = rng.uniform(0.0, np.pi, n)
+    rows[:, 10] = rng.uniform(*tc_range, n)
+    return rows
+
+
 def make_detector(orbit="LISA_analytic", psd_model="LISA_SciRDv1", gen="1.5", T=655360.0, dt=5.0,
                   fmin=1e-4, fmax=0.05, seed=7, modes=((2, 2),), interp=(1e-4, 0.05, 1024), orthogonal=True,
                   channels=("A", "E", "T"), inject=True):

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from oracle import phenom as oph
-from tests._setup import add_to_core, inj_row, make_detector, particle_cloud
+from tests._setup import add_to_core, inj_row, make_detector, particle_cloud, prior_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -54,3 +54,22 @@ def test_mixed_accuracy(modes, gen, precision):
     err = np.linalg.norm(rmx - r64) / np.linalg.norm(r64)
     print("response rel err", err)
     assert err < 2e-6
+
+
+@pytest.mark.parametrize("P", [1, 257, 1000])
+def test_uneven_batches_wide_prior(P):
+    """Batches of uneven size (not multiples of the 16-row chunk) drawn from the wide prior:
+    the mixed path against the FP64 path, and the batch against its rows evaluated alone."""
+    modes = oph.MODES
+    d, labels = make_detector(modes=modes)
+    core = _core(modes, d, labels)
+    rows = prior_rows(P, seed=P)
+    l64, g64 = (x.cpu().numpy() for x in core.lnl(rows, grad=True, precision="fp64"))
+    lmx = core.lnl(rows, precision="mixed").cpu().numpy()
+    rel = np.abs(lmx - l64) / np.abs(l64)
+    print(f"P={P}: mixed |dlnl| max {np.abs(lmx - l64).max():.2e}, rel max {rel.max():.2e}")
+    assert np.all(np.isfinite(l64)) and np.all(np.isfinite(g64))
+    assert rel.max() < 1e-6
+    for i in (0, P // 2, P - 1):
+        l1 = core.lnl(rows[i:i + 1]).cpu().numpy()
+        np.testing.assert_allclose(l1, l64[i:i + 1], rtol=1e-13)

@@ -12,7 +12,7 @@ import pytest
 from oracle import detector as odet
 from oracle import likelihood as olk
 from oracle import phenom as oph
-from tests._setup import INJ, add_to_core, inj_row, make_detector, particle_cloud
+from tests._setup import INJ, add_to_core, inj_row, make_detector, particle_cloud, prior_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -122,3 +122,16 @@ def test_direct_log_grid():
     r = core.response(0, rows[:1]).cpu().numpy()[0]
     for ci, c in enumerate(labels):
         assert _relnorm(r[ci], resp[c]) < 1e-10
+
+
+def test_wide_prior_rows():
+    """Rows drawn from the wide prior (masses 1e5-1e7 Msun, t_c anywhere in the data span,
+    any sky position and orientation), six modes: signals that end inside, above or below
+    the band, and phases up to ~1e7 rad."""
+    modes = oph.MODES
+    d, labels = make_detector(modes=modes)
+    core = _core(modes, [(d, labels)])
+    rows = prior_rows(12, seed=21)
+    lnl = core.lnl(rows).cpu().numpy()
+    ref = olk.lnl_batch(rows, [d], modes, 0.0, INTERP)
+    np.testing.assert_allclose(lnl, ref, rtol=1e-10)
