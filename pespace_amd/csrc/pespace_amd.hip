@@ -187,11 +187,7 @@ __global__ __launch_bounds__(128) void k_nodes(const double* __restrict__ setup,
   int id = modes.id[mi];
   if (!GRAD) {
     typedef Dual<double, 1> S;
-    Phenom<S> ph;
-    for (int i = 0; i < N_SETUP; ++i) {
-      ph.c[i].v = su[i * 5];
-      ph.c[i].d[0] = 0.0;
-    }
+    Phenom<S, SetupView<S>> ph{SetupView<S>{su}};
     S Mf;
     Mf.v = f * su[SU_MSEC * 5];
     Mf.d[0] = Mf.v;  // dMf/du
@@ -204,15 +200,7 @@ __global__ __launch_bounds__(128) void k_nodes(const double* __restrict__ setup,
   } else {
     typedef Dual<double, 1> D1;
     typedef Dual<D1, 4> S;
-    Phenom<S> ph;
-    for (int i = 0; i < N_SETUP; ++i) {
-      ph.c[i].v.v = su[i * 5];
-      ph.c[i].v.d[0] = 0.0;
-      for (int k = 0; k < 4; ++k) {
-        ph.c[i].d[k].v = su[i * 5 + 1 + k];
-        ph.c[i].d[k].d[0] = 0.0;
-      }
-    }
+    Phenom<S, SetupView<S>> ph{SetupView<S>{su}};
     S Mf;
     Mf.v.v = f * su[SU_MSEC * 5];
     Mf.v.d[0] = Mf.v.v;

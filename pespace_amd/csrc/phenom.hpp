@@ -144,9 +144,40 @@ enum SetupId {
   N_SETUP
 };
 
+// Coefficient store of Phenom: an array (written by phenom_setup) or, in the node kernels,
+// SetupView: a read-only view of one row of the setup table [N_SETUP][5] (value and d/dtheta
+// for Mc, q, chi1, chi2), building each coefficient as the requested dual on access, so the
+// kernels hold no per-lane copy of the ~60 coefficients.
 template <class S>
+struct SetupView;
+template <>
+struct SetupView<Dual<double, 1>> {
+  const double* su;
+  PES_HD Dual<double, 1> operator[](int i) const {
+    Dual<double, 1> r;
+    r.v = su[i * 5];
+    r.d[0] = 0.0;
+    return r;
+  }
+};
+template <>
+struct SetupView<Dual<Dual<double, 1>, 4>> {
+  const double* su;
+  PES_HD Dual<Dual<double, 1>, 4> operator[](int i) const {
+    Dual<Dual<double, 1>, 4> r;
+    r.v.v = su[i * 5];
+    r.v.d[0] = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      r.d[k].v = su[i * 5 + 1 + k];
+      r.d[k].d[0] = 0.0;
+    }
+    return r;
+  }
+};
+
+template <class S, class Store = S[N_SETUP]>
 struct Phenom {
-  S c[N_SETUP];
+  Store c;
 
   // -- amplitude (reduced ansatz) --
   PES_HD S amp_ins(const S& f) const {
