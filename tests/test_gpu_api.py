@@ -162,3 +162,28 @@ def test_schedules_agree_and_autotune_picks_one():
     assert best in times and times[best] == min(times.values())
     assert core.schedule() == best
     torch.cuda.synchronize()
+
+
+def test_invalid_rows_and_empty_batch():
+    """Rows outside the model's domain give lnL = -inf and a zero gradient without disturbing
+    the valid rows of the same batch; an empty batch is a no-op."""
+    import torch
+
+    from pespace_amd.inference.likelihood import WhittleLikelihoodHip
+    det, wf = _host_detector()
+    lk = WhittleLikelihoodHip(wf, det, ("A", "E", "T"))
+    good = np.vstack([inj_row(), particle_cloud(2, seed=4)])
+    bad = np.repeat(inj_row()[None, :], 5, axis=0)
+    bad[0, 1] = 1.5        # mass ratio above 1
+    bad[1, 2] = 1.0        # |chi_1| = 1
+    bad[2, 4] = -1.0       # negative distance
+    bad[3, 10] = np.nan    # non-finite
+    bad[4, 0] = 0.0        # zero chirp mass
+    rows = np.vstack([good[:1], bad, good[1:]])
+    lnl, g = (x.cpu().numpy() for x in lk.log_likelihood_and_grad(rows))
+    ref_l, ref_g = (x.cpu().numpy() for x in lk.log_likelihood_and_grad(good))
+    assert np.all(np.isneginf(lnl[1:6])) and np.all(g[1:6] == 0.0)
+    np.testing.assert_array_equal(lnl[[0, 6, 7]], ref_l)
+    np.testing.assert_array_equal(g[[0, 6, 7]], ref_g)
+    empty = lk.log_likelihood_batch(torch.empty(0, 11, dtype=torch.float64, device="cuda"))
+    assert empty.shape == (0,)
