@@ -397,7 +397,7 @@ __device__ inline void response_row(const DetectorDev& det, const double* __rest
 // and the propagation direction (contracted with their lambda/beta/psi derivatives once per
 // row); the rest are parameter adjoints.
 struct AdjAcc {
-  V3 pb, qb, kb;
+  double glam, gbet, gpsi;  // lambda, beta, psi (through p, q, k)
   double gint[4];  // Mc, q, chi1, chi2
   double gh_re;    // sum Re(G h): d lnL / d d_L = -gh_re / d_L
   double mphi;     // sum m phibar: d lnL / d phi_ref = -mphi
@@ -408,7 +408,8 @@ struct AdjAcc {
 template <int PREC>
 __device__ inline void mode_adjoint(const DetectorDev& det, const double* __restrict__ mf, int mid,
                                     const double* __restrict__ n0, const double* __restrict__ g0, const BinPre& b,
-                                    const RowC& rc, Cx etc, const Cx wl[6], Cx wT, AdjAcc& acc) {
+                                    const RowC& rc, Cx etc, const Cx wl[6], Cx wT, const double* __restrict__ pc,
+                                    AdjAcc& acc) {
   ModeG g;
   mode_geom<PREC>(det, mf, n0, b, rc, etc, g);
   double pxb[3] = {0.0, 0.0, 0.0}, qxb[3] = {0.0, 0.0, 0.0}, kxb[3] = {0.0, 0.0, 0.0};
@@ -448,11 +449,12 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
   V3 xt[3];
   xd_dot(det, g, xt);
   double tgeo = 0.0;
+  V3 pb = {0.0, 0.0, 0.0}, qb = {0.0, 0.0, 0.0}, kb = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    axpy3(pxb[i], g.xd[i], acc.pb);
-    axpy3(qxb[i], g.xd[i], acc.qb);
-    axpy3(kxb[i], g.xd[i], acc.kb);
+    axpy3(pxb[i], g.xd[i], pb);
+    axpy3(qxb[i], g.xd[i], qb);
+    axpy3(kxb[i], g.xd[i], kb);
     tgeo = fma(pxb[i], dot3(rc.p, xt[i]), tgeo);
     tgeo = fma(qxb[i], dot3(rc.q, xt[i]), tgeo);
     tgeo = fma(kxb[i], dot3(rc.k, xt[i]), tgeo);
@@ -463,8 +465,20 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
   const Cx Gh = mul(Gm, g.hm);
   const double phib = Gh.im;  // adjoint of the total phase Psi + 2 pi f (t_c + k.x0)
   const double kx0b = 2.0 * b.pif * phib;
-  acc.kb.x = fma(kx0b * AU_SEC, g.ca, acc.kb.x);
-  acc.kb.y = fma(kx0b * AU_SEC, g.sa, acc.kb.y);
+  kb.x = fma(kx0b * AU_SEC, g.ca, kb.x);
+  kb.y = fma(kx0b * AU_SEC, g.sa, kb.y);
+  // p, q, k adjoints contracted with their lambda, beta, psi derivatives (row constants)
+  {
+    const V3 dlp = {pc[PC_DLP], pc[PC_DLP + 1], pc[PC_DLP + 2]};
+    const V3 dlq = {pc[PC_DLQ], pc[PC_DLQ + 1], pc[PC_DLQ + 2]};
+    const V3 dlk = {pc[PC_DLK], pc[PC_DLK + 1], pc[PC_DLK + 2]};
+    acc.glam += dot3(pb, dlp) + dot3(qb, dlq) + dot3(kb, dlk);
+    const V3 dbp = {pc[PC_DBP], pc[PC_DBP + 1], pc[PC_DBP + 2]};
+    const V3 dbq = {pc[PC_DBQ], pc[PC_DBQ + 1], pc[PC_DBQ + 2]};
+    const V3 dbk = {pc[PC_DBK], pc[PC_DBK + 1], pc[PC_DBK + 2]};
+    acc.gbet += dot3(pb, dbp) + dot3(qb, dbq) + dot3(kb, dbk);
+    acc.gpsi += dot3(pb, rc.q) - dot3(qb, rc.p);
+  }
   const double kx0_t = AU_SEC * kOmegaYear * fma(-rc.k.x, g.sa, rc.k.y * g.ca);
   const double tb = fma(kx0b, kx0_t, tgeo);  // adjoint of t = t_f + t_c
   acc.gh_re += Gh.re;
@@ -625,9 +639,7 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
             wl[4] = sub(DWX, WZ);
             wl[5] = sub(WX, DWZ);
             AdjAcc aa;
-            aa.pb = {0.0, 0.0, 0.0};
-            aa.qb = {0.0, 0.0, 0.0};
-            aa.kb = {0.0, 0.0, 0.0};
+            aa.glam = aa.gbet = aa.gpsi = 0.0;
 #pragma unroll
             for (int i = 0; i < 4; ++i) aa.gint[i] = 0.0;
             aa.gh_re = aa.mphi = aa.iota = aa.tc = 0.0;
@@ -636,22 +648,16 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
               if (b.f > mf[MF_CUT]) continue;
               const size_t nrow = pbase + (size_t)mi * NNS + b.j;
               mode_adjoint<PREC>(det, mf, modes.id[mi], nodes + nrow * 4, nodes_grad + nrow * 16, b, rc, etc, wl,
-                                  wT, aa);
+                                  wT, pc, aa);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) G[i] = aa.gint[i];
             G[4] = -aa.gh_re * pc[PC_INVDL];
             G[5] = -aa.mphi;
             G[6] = aa.iota;
-            const V3 dlp = {pc[PC_DLP], pc[PC_DLP + 1], pc[PC_DLP + 2]};
-            const V3 dlq = {pc[PC_DLQ], pc[PC_DLQ + 1], pc[PC_DLQ + 2]};
-            const V3 dbp = {pc[PC_DBP], pc[PC_DBP + 1], pc[PC_DBP + 2]};
-            const V3 dbq = {pc[PC_DBQ], pc[PC_DBQ + 1], pc[PC_DBQ + 2]};
-            const V3 dlk = {pc[PC_DLK], pc[PC_DLK + 1], pc[PC_DLK + 2]};
-            const V3 dbk = {pc[PC_DBK], pc[PC_DBK + 1], pc[PC_DBK + 2]};
-            G[7] = dot3(aa.pb, dlp) + dot3(aa.qb, dlq) + dot3(aa.kb, dlk);
-            G[8] = dot3(aa.pb, dbp) + dot3(aa.qb, dbq) + dot3(aa.kb, dbk);
-            G[9] = dot3(aa.pb, rc.q) - dot3(aa.qb, rc.p);
+            G[7] = aa.glam;
+            G[8] = aa.gbet;
+            G[9] = aa.gpsi;
             G[10] = aa.tc;
           }
         }
