@@ -2,9 +2,11 @@
 //
 // Arithmetic is FP64 in every precision mode; the modes differ only in where sines, cosines
 // and reciprocals come from (PREC template parameter, PES_PREC_* values):
-//   0 FP64: sincos_d / sincos_small_d / rcp_d, and the t_c shift and the barycentric delay
-//     applied as separate factors exp(-2 pi i f t_c), exp(-2 pi i f k.x0) exactly as the
-//     reference does (antenna.py:435, 530-533).
+//   0 FP64: sincos_tab / rcp_d; the phasor of the total phase Psi + 2 pi f (t_c + k.x0) (the
+//     waveform phase, the t_c shift and the barycentric delay, antenna.py:435, 530-533, which
+//     the reference applies as three factors) from one sincos_tab of the exact two-double sum
+//     of 2 pi f t_c and Psi + 2 pi f k.x0: each term keeps the rounding it has as a separate
+//     factor.
 //   1 MIXED: the phasor of the total phase Psi + 2 pi f (t_c + k.x0) (summed and reduced mod
 //     2 pi in FP64) comes from the FP32 hardware sine/cosine (v_sin_f32, v_cos_f32).  Its
 //     error changes from bin to bin (the phase moves by ~2 pi t df per bin), so it averages
@@ -21,8 +23,14 @@
 // Algebraic rearrangements relative to antenna.py:496-543 (same values):
 //   * the constellation offsets sum to zero, so k.(x_i + x_j) = 2 k.x0 - k.xd_a for link
 //     pair a with nodes (i, j) and third node a: exp(-i pi f (L + k.(x_i + x_j))) =
-//     exp(-i pi f L) exp(-2 pi i f k.x0) exp(+i pi f k.xd_a), one small sincos per pair;
-//   * sinc(xL(1 +- k.n)): sin(xL +- a) = sin xL cos a +- cos xL sin a, a = xL k.n;
+//     exp(-i pi f L) exp(-2 pi i f k.x0) z_a with z_a = exp(+i pi f k.xd_a);
+//   * z_0 z_1 z_2 = 1 (the offsets sum to zero), so z_2 = conj(z_0 z_1): two small sincos per
+//     mode give all three pair phasors;
+//   * sinc(xL(1 +- k.n_a)): sin(xL +- a) = sin xL cos a +- cos xL sin a with
+//     a = xL k.n_a = pi f k.(xd_i - xd_j), so exp(i a) = z_i conj(z_j) is a product of the
+//     pair phasors (no further sincos);
+//   * the factor (-i xL) exp(-i xL) common to every link term of every mode is applied once
+//     per bin, inside the TDI prefactor;
 //   * n_a.v = (xd_i.v - xd_j.v)/L for v in {p, q, k}: nine scalar products per mode;
 //   * n.e+.n = (n.p)^2 - (n.q)^2, n.ex.n = 2 (n.p)(n.q) with e+ = pp - qq, ex = pq + qp.
 // This header is included inside the anonymous namespace of pespace_amd.hip.
@@ -100,8 +108,8 @@ __device__ inline void bin_prepare(const DetectorDev& det, double f, double s, i
 // TDI constants of one bin, formed in FP64 from their product forms (1 - D^n cancels at
 // low f); recomputed per row from sin, cos of xL rather than held in registers
 struct TdiC {
-  Cx D, pf;  // delay exp(-2 i xL) and generation prefactor
-  Cx pfT;    // T-channel factor pf (1 - D) / sqrt(3)
+  Cx D, pf;  // delay exp(-2 i xL) and generation prefactor times (-i xL) exp(-i xL)
+  Cx pfT;    // T-channel factor pf (1 - D) / sqrt(3), with the same common factor
 };
 __device__ inline void tdi_consts(const DetectorDev& det, const BinPre& b, TdiC& t) {
   const double sx = b.sxL, cx = b.cxL;
@@ -114,6 +122,13 @@ __device__ inline void tdi_consts(const DetectorDev& det, const BinPre& b, TdiC&
     const double s4 = 2.0 * s2 * c2, c4 = (c2 - s2) * (c2 + s2);
     const double b_re = 2.0 * s4 * s4, b_im = 2.0 * s4 * c4;
     const double r = pfr * b_re - pfi * b_im, i = pfr * b_im + pfi * b_re;
+    pfr = r;
+    pfi = i;
+  }
+  // common factor of every link term: (-i xL) exp(-i xL) = xL (-sin xL, -cos xL)
+  {
+    const double fr = -b.xL * sx, fi = -b.xL * cx;
+    const double r = pfr * fr - pfi * fi, i = pfr * fi + pfi * fr;
     pfr = r;
     pfi = i;
   }
@@ -142,6 +157,7 @@ __device__ inline RowC load_row(const double* __restrict__ pc) {
 struct ModeG {
   double A, t;      // amplitude, t_f + t_c
   Cx u, hm, hp, hc; // unit phasor, A u, F+ A u, Fx A u
+  Cx z0, z1, z01;   // pair phasors exp(i pi f k.xd_a) of nodes 0, 1, and their product
   double sa, ca;    // orbit angle (heliocentric revolution)
   double sr, cr;    // geocentric rotation angle of node 1
   double kx0;
@@ -153,7 +169,7 @@ constexpr double kOmegaYear = 2.0 * PI / YEAR_SI;
 
 template <int PREC>
 __device__ inline void mode_geom(const DetectorDev& det, const double* __restrict__ mf,
-                                 const double* __restrict__ n0, const BinPre& b, const RowC& rc, Cx etc, ModeG& g) {
+                                 const double* __restrict__ n0, const BinPre& b, const RowC& rc, ModeG& g) {
   const double A0 = n0[0], dA0 = n0[1], P0 = n0[2], dP0 = n0[3];
   const double A1 = n0[4], dA1 = n0[5], P1 = n0[6], dP1 = n0[7];
   g.A = A0 + fma(b.hb1, A1 - A0, fma(b.hb2, dA0, b.hb3 * dA1));
@@ -199,18 +215,28 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
     g.qx[i] = dot3(rc.q, g.xd[i]);
     g.kx[i] = dot3(rc.k, g.xd[i]);
   }
+  {
+    double s0, c0, s1, c1;
+    Trig<PREC>::small(b.pif * g.kx[0], &s0, &c0);
+    Trig<PREC>::small(b.pif * g.kx[1], &s1, &c1);
+    g.z0 = {c0, s0};
+    g.z1 = {c1, s1};
+    g.z01 = mul(g.z0, g.z1);
+  }
+  double sp, cp;
   if constexpr (Trig<PREC>::kFp32Phase) {
     // unit modulus matters: a mean modulus error of the FP32 pair would scale every bin's
     // |h|^2 the same way; the remaining error is a phase error that changes from bin to bin
-    double sp, cp;
-    sincos_rev_fc(Psi + 2.0 * b.pif * (rc.tc + g.kx0), &sp, &cp);
-    g.u = {cp, -sp};
+    sincos_rev_fc(fma(2.0 * b.pif, rc.tc + g.kx0, Psi), &sp, &cp);
   } else {
-    double sp, cp, s0, c0;
-    sincos_tab(Psi, &sp, &cp);
-    sincos_tab(-2.0 * b.pif * g.kx0, &s0, &c0);
-    g.u = mul(mul(Cx{cp, -sp}, etc), Cx{c0, s0});
+    // 2 pi f t_c (rounded once, as the reference's own factor) plus Psi + 2 pi f k.x0, summed
+    // without rounding: the phase's rounding error then does not move with Psi or k.x0, and
+    // lnL stays as smooth in the parameters as with three separate factors
+    double hi, lo;
+    two_sum(2.0 * b.pif * rc.tc, fma(2.0 * b.pif, g.kx0, Psi), hi, lo);
+    sincos_tab2(hi, lo, &sp, &cp);
   }
+  g.u = {cp, -sp};
   g.hm = scale(g.u, g.A);
   g.hp = mul(Cx{mf[MF_FPR], mf[MF_FPI]}, g.hm);
   g.hc = mul(Cx{mf[MF_FCR], mf[MF_FCI]}, g.hm);
@@ -253,11 +279,9 @@ __device__ inline void xd_dot(const DetectorDev& det, const ModeG& g, V3 xt[3]) 
 // (sinc y = int_0^1 cos(y t) dt, so delta = 2 int_0^1 sin(x t) sin(a t) dt), which has no
 // cancellation; the direct difference would lose the FP32 sines' accuracy there.
 template <int PREC>
-__device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, double& sp, double& sm, double& dsp,
-                                 double& dsm, double& delta) {
+__device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, double ca, double sa, double& sp,
+                                 double& sm, double& dsp, double& dsm, double& delta) {
   const double a = xL * kn;
-  double sa, ca;
-  Trig<PREC>::small(a, &sa, &ca);
   const double xp = xL + a, xm = xL - a;
   const double sinp = fma(sxL, ca, cxL * sa), cosp = fma(cxL, ca, -sxL * sa);
   const double sinm = fma(sxL, ca, -cxL * sa), cosm = fma(cxL, ca, sxL * sa);
@@ -309,7 +333,7 @@ __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, d
 
 struct PairV {
   double np, nq, kn, Pp, Pc;
-  Cx E, C;
+  Cx z, C;  // pair phasor z_a; link term C = (n.h.n) z_a (without the common (-i xL) exp(-i xL))
   double sp, sm, dsp, dsm, dl;
 };
 
@@ -323,12 +347,20 @@ __device__ inline void pair_eval(const DetectorDev& det, const BinPre& b, const 
   v.Pp = (v.np - v.nq) * (v.np + v.nq);
   v.Pc = 2.0 * v.np * v.nq;
   const Cx nhn = {fma(v.Pp, g.hp.re, v.Pc * g.hc.re), fma(v.Pp, g.hp.im, v.Pc * g.hc.im)};
-  double s_, c_;
-  Trig<PREC>::small(b.pif * g.kx[a], &s_, &c_);
-  v.E = mul(Cx{b.cxL, -b.sxL}, Cx{c_, s_});  // exp(-i pi f L) exp(+i pi f k.xd_a)
-  const Cx T = mul(nhn, v.E);
-  v.C = {b.xL * T.im, -b.xL * T.re};  // (-i xL) nhn E
-  sinc_pair<PREC>(b.xL, b.sxL, b.cxL, v.kn, v.sp, v.sm, v.dsp, v.dsm, v.dl);
+  // z_a and exp(i xL k.n_a) = z_ia conj(z_ib), with z_2 = conj(z_0 z_1)
+  Cx w;
+  if (a == 0) {
+    v.z = g.z0;
+    w = conjx(mul(g.z01, g.z1));
+  } else if (a == 1) {
+    v.z = g.z1;
+    w = mul(g.z0, g.z01);
+  } else {
+    v.z = conjx(g.z01);
+    w = mul(g.z1, conjx(g.z0));
+  }
+  v.C = mul(nhn, v.z);
+  sinc_pair<PREC>(b.xL, b.sxL, b.cxL, v.kn, w.re, w.im, v.sp, v.sm, v.dsp, v.dsm, v.dl);
 }
 
 struct LinksT {
@@ -356,20 +388,9 @@ __device__ inline void tdi_out(const DetectorDev& det, const TdiC& t, const Link
 }
 
 template <int PREC>
-__device__ inline Cx row_etc(const BinPre& b, double tc) {
-  Cx etc = {1.0, 0.0};
-  if constexpr (!Trig<PREC>::kFp32Phase) {
-    double s, c;
-    sincos_tab(-2.0 * b.pif * tc, &s, &c);
-    etc = {c, s};
-  }
-  return etc;
-}
-
-template <int PREC>
 __device__ inline void response_row(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
                                     const double* __restrict__ nodes, size_t pbase, int NNS, const BinPre& b,
-                                    const RowC& rc, Cx etc, const TdiC& tdc, Cx out[3]) {
+                                    const RowC& rc, const TdiC& tdc, Cx out[3]) {
   LinksT L;
 #pragma unroll
   for (int i = 0; i < 6; ++i) L.l[i] = {0.0, 0.0};
@@ -378,7 +399,7 @@ __device__ inline void response_row(const DetectorDev& det, const double* __rest
     const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
     if (b.f > mf[MF_CUT]) continue;
     ModeG g;
-    mode_geom<PREC>(det, mf, nodes + (pbase + (size_t)mi * NNS + b.j) * 4, b, rc, etc, g);
+    mode_geom<PREC>(det, mf, nodes + (pbase + (size_t)mi * NNS + b.j) * 4, b, rc, g);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       PairV v;
@@ -408,10 +429,10 @@ struct AdjAcc {
 template <int PREC>
 __device__ inline void mode_adjoint(const DetectorDev& det, const double* __restrict__ mf, int mid,
                                     const double* __restrict__ n0, const double* __restrict__ g0, const BinPre& b,
-                                    const RowC& rc, Cx etc, const Cx wl[6], Cx wT, const double* __restrict__ pc,
+                                    const RowC& rc, const Cx wl[6], Cx wT, const double* __restrict__ pc,
                                     AdjAcc& acc) {
   ModeG g;
-  mode_geom<PREC>(det, mf, n0, b, rc, etc, g);
+  mode_geom<PREC>(det, mf, n0, b, rc, g);
   double pxb[3] = {0.0, 0.0, 0.0}, qxb[3] = {0.0, 0.0, 0.0}, kxb[3] = {0.0, 0.0, 0.0};
   Cx Hp = {0.0, 0.0}, Hc = {0.0, 0.0};
 #pragma unroll
@@ -429,8 +450,7 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
     const Cx W = {fma(v.sp, wp.re, fma(v.sm, wm.re, v.dl * wT.re)), fma(v.sp, wp.im, fma(v.sm, wm.im, v.dl * wT.im))};
     // adjoint of k.xd_a through exp(+i pi f k.xd_a): -pi f Im(conj(W) C)
     kxb[a] -= b.pif * (W.re * v.C.im - W.im * v.C.re);
-    const Cx t1 = mul(conjx(W), v.E);
-    const Cx V = {b.xL * t1.im, -b.xL * t1.re};  // conj(W) (-i xL) E: adjoint of n.h.n
+    const Cx V = mul(conjx(W), v.z);  // adjoint of n.h.n
     const double Ppb = V.re * g.hp.re - V.im * g.hp.im;  // Re(V h+)
     const double Pcb = V.re * g.hc.re - V.im * g.hc.im;  // Re(V hx)
     const double npb = 2.0 * fma(v.np, Ppb, v.nq * Pcb);
@@ -601,11 +621,10 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
         if (pc[PC_VALID] != 0.0) {
           const size_t pbase = (size_t)p * modes.n * NNS;
           const RowC rc = load_row(pc);
-          const Cx etc = row_etc<PREC>(b, rc.tc);
           TdiC tdc;
           tdi_consts(det, b, tdc);
           Cx out[3];
-          response_row<PREC>(det, pc, modes, nodes, pbase, NNS, b, rc, etc, tdc, out);
+          response_row<PREC>(det, pc, modes, nodes, pbase, NNS, b, rc, tdc, out);
           // data and weights are re-read per row (L1/L2 hits) instead of held in registers
           Cx wch[3];
 #pragma unroll
@@ -647,7 +666,7 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
               const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
               if (b.f > mf[MF_CUT]) continue;
               const size_t nrow = pbase + (size_t)mi * NNS + b.j;
-              mode_adjoint<PREC>(det, mf, modes.id[mi], nodes + nrow * 4, nodes_grad + nrow * 16, b, rc, etc, wl,
+              mode_adjoint<PREC>(det, mf, modes.id[mi], nodes + nrow * 4, nodes_grad + nrow * 16, b, rc, wl,
                                   wT, pc, aa);
             }
 #pragma unroll
@@ -704,11 +723,10 @@ __global__ __launch_bounds__(128) void k_response_t(const DetectorDev* __restric
   bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
   const double* pc = pconst + (size_t)p * PC_STRIDE;
   const RowC rc = load_row(pc);
-  const Cx etc = row_etc<PREC>(b, rc.tc);
   TdiC tdc;
   tdi_consts(det, b, tdc);
   Cx o[3];
-  response_row<PREC>(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, rc, etc, tdc, o);
+  response_row<PREC>(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, rc, tdc, o);
   for (int ch = 0; ch < 3; ++ch) {
     const size_t idx = (((size_t)p * 3 + ch) * det.n_bins + k) * 2;
     out[idx] = o[ch].re;

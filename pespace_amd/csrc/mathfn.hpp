@@ -78,10 +78,8 @@ constexpr double kTabScale = 40.74366543152521;      // 256 / (2 pi)
 constexpr double kTabStepHi = 0.02454369260617026;   // 2 pi / 256 rounded to double
 constexpr double kTabStepLo = 9.567553118338697e-19; // 2 pi / 256 - kTabStepHi, rounded
 
-__device__ inline void sincos_tab(double x, double* s, double* c) {
-  const double k = rint(x * kTabScale);
-  double r = fma(-k, kTabStepHi, x);
-  r = fma(-k, kTabStepLo, r);
+// table part of sincos_tab: r = x - k 2 pi / 256 already formed
+__device__ inline void sincos_tab_kr(double k, double r, double* s, double* c) {
   const int idx = (int)fma(-256.0, floor(k * (1.0 / 256.0)), k);  // k mod 256, exact for |k| < 2^53
   const double2 t = g_trig[idx];
   const double r2 = r * r;
@@ -89,6 +87,29 @@ __device__ inline void sincos_tab(double x, double* s, double* c) {
   const double cr = fma(r2, fma(r2, fma(r2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
   *s = fma(t.x, cr, t.y * sr);
   *c = fma(t.y, cr, -t.x * sr);
+}
+
+__device__ inline void sincos_tab(double x, double* s, double* c) {
+  const double k = rint(x * kTabScale);
+  double r = fma(-k, kTabStepHi, x);
+  r = fma(-k, kTabStepLo, r);
+  sincos_tab_kr(k, r, s, c);
+}
+
+// sin, cos of hi + lo (|lo| <= ulp(hi)/2, e.g. the exact sum of two doubles): the exact first
+// reduction step keeps lo's bits
+__device__ inline void sincos_tab2(double hi, double lo, double* s, double* c) {
+  const double k = rint(hi * kTabScale);
+  double r = fma(-k, kTabStepHi, hi);
+  r = fma(-k, kTabStepLo, r) + lo;
+  sincos_tab_kr(k, r, s, c);
+}
+
+// a + b = s + e exactly (Knuth's TwoSum, no ordering of |a|, |b| needed)
+__device__ inline void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
 }
 
 __device__ inline double rcp_d(double x) {
