@@ -41,11 +41,9 @@ extern "C" {
 
 /* precision modes of pes_lnl_batch / pes_response.  All arithmetic and accumulation is FP64;
  * PES_PREC_MIXED takes the sine/cosine of the total phase Psi + 2 pi f (t_c + k.x0) (reduced in
- * FP64) from the FP32 hardware units; PES_PREC_MIXED_FAST also the transfer-function sines,
- * cosines and reciprocals (faster, accurate near the likelihood peak only). */
+ * FP64) from the FP32 hardware units.  Any other value is rejected with PES_ERR_INVALID. */
 #define PES_PREC_FP64 0
 #define PES_PREC_MIXED 1
-#define PES_PREC_MIXED_FAST 2
 
 typedef struct pes_context pes_context;
 

@@ -17,7 +17,7 @@ PARAM_NAMES = ("chirp_mass", "mass_ratio", "chi_1", "chi_2", "luminosity_distanc
                "inclination", "ecliptic_longitude", "ecliptic_latitude", "polarization", "coalescence_time")
 N_PARAMS = len(PARAM_NAMES)
 MODE_IDS = {"22": 0, "21": 1, "33": 2, "32": 3, "44": 4, "43": 5}
-PRECISION = {"fp64": 0, "mixed": 1, "mixed_fast": 2}
+PRECISION = {"fp64": 0, "mixed": 1}
 
 
 def _mode_key(m):

@@ -1,8 +1,8 @@
 // Fused per-(bin, row) forward response and adjoint.
 //
-// Arithmetic is FP64 in every precision mode; the modes differ only in where sines, cosines
-// and reciprocals come from (PREC template parameter, PES_PREC_* values):
-//   0 FP64: sincos_tab / rcp_d; the phasor of the total phase Psi + 2 pi f (t_c + k.x0) (the
+// Arithmetic is FP64 in every precision mode; the two modes differ only in where the sine and
+// cosine of the total phase come from (PREC template parameter, PES_PREC_* values):
+//   0 FP64: the phasor of the total phase Psi + 2 pi f (t_c + k.x0) (the
 //     waveform phase, the t_c shift and the barycentric delay, antenna.py:435, 530-533, which
 //     the reference applies as three factors) from one sincos_tab of the exact two-double sum
 //     of 2 pi f t_c and Psi + 2 pi f k.x0: each term keeps the rounding it has as a separate
@@ -10,15 +10,16 @@
 //   1 MIXED: the phasor of the total phase Psi + 2 pi f (t_c + k.x0) (summed and reduced mod
 //     2 pi in FP64) comes from the FP32 hardware sine/cosine (v_sin_f32, v_cos_f32).  Its
 //     error changes from bin to bin (the phase moves by ~2 pi t df per bin), so it averages
-//     out of lnL; the transfer functions stay FP64.
-//   2 MIXED_FAST: the transfer-function sines/cosines and reciprocals come from the FP32 units
-//     too.  Their arguments vary slowly with f, so their ~1e-7 errors are coherent over many
-//     bins and bias |h|^2 by ~1e-7 relative: fine near the likelihood peak, not far from it.
-// The orbit angles keep the FP64 sincos_d in all modes: their error is multiplied by
-// 2 pi f AU/c (~150 rad) in the delay phase.  Without packed FP32, an FP32 FMA issues at the
-// FP64 rate on gfx950, so FP32 pays only in the transcendental units; FP64 algebra also keeps
-// per-row constants (polarization factors, 1/d_L, node values) from carrying an FP32 rounding
-// that would bias every bin of a row the same way.
+//     out of lnL.
+// Every other sine and cosine (orbit angles, pair phasors) comes from sincos_tab and every
+// reciprocal from rcp_d in both modes.  FP32 there does not pay: the orbit angle's error is
+// multiplied by 2 pi f AU/c (~150 rad) in the delay phase, and the transfer-function
+// arguments vary slowly with f, so FP32 errors there are coherent over many bins and bias
+// |h|^2 (|dlnL| ~ 1 far from the peak), while FP32 sines cost about as many issue cycles as
+// the table sincos.  Without packed FP32, an FP32 FMA issues at the FP64 rate on gfx950, so
+// FP32 pays only in the transcendental units; FP64 algebra also keeps per-row constants
+// (polarization factors, 1/d_L, node values) from carrying an FP32 rounding that would bias
+// every bin of a row the same way.
 //
 // Algebraic rearrangements relative to antenna.py:496-543 (same values):
 //   * the constellation offsets sum to zero, so k.(x_i + x_j) = 2 k.x0 - k.xd_a for link
@@ -31,7 +32,8 @@
 //     pair phasors (no further sincos);
 //   * the factor (-i xL) exp(-i xL) common to every link term of every mode is applied once
 //     per bin, inside the TDI prefactor;
-//   * n_a.v = (xd_i.v - xd_j.v)/L for v in {p, q, k}: nine scalar products per mode;
+//   * n_a.v = (xd_i.v - xd_j.v)/L for v in {p, q, k} from six scalar products per mode (the
+//     node offsets are two fixed vectors weighted by equally spaced phases, see ModeG);
 //   * n.e+.n = (n.p)^2 - (n.q)^2, n.ex.n = 2 (n.p)(n.q) with e+ = pp - qq, ex = pq + qp.
 // This header is included inside the anonymous namespace of pespace_amd.hip.
 
@@ -49,29 +51,10 @@ struct V3 {
   double x, y, z;
 };
 __device__ inline double dot3(const V3& a, const V3& b) { return fma(a.x, b.x, fma(a.y, b.y, a.z * b.z)); }
-__device__ inline void axpy3(double s, const V3& a, V3& y) {
-  y.x = fma(s, a.x, y.x);
-  y.y = fma(s, a.y, y.y);
-  y.z = fma(s, a.z, y.z);
-}
 
 template <int PREC>
 struct Trig {
-  static constexpr bool kFp32Phase = PREC >= 1;     // one FP32 sincos of the total phase
-  static constexpr bool kFp32Transfer = PREC >= 2;  // FP32 transfer-function trig
-  // transfer-function arguments (|x| <= 4: the host checks pi f_max L <= 4 per detector)
-  __device__ static void small(double x, double* s, double* c) {
-    if constexpr (kFp32Transfer)
-      sincos_rev_small_f(x, s, c);
-    else
-      sincos_tab(x, s, c);
-  }
-  __device__ static double rcp(double x) {
-    if constexpr (kFp32Transfer)
-      return rcp_f(x);
-    else
-      return rcp_d(x);
-  }
+  static constexpr bool kFp32Phase = PREC >= 1;  // one FP32 sincos of the total phase
 };
 
 // per-bin quantities shared by every row and mode of a tile lane (kept small: they stay in
@@ -153,21 +136,30 @@ __device__ inline RowC load_row(const double* __restrict__ pc) {
   return r;
 }
 
-// One mode at one bin: waveform, orbit and the nine projections of the constellation
+// One mode at one bin: waveform, orbit, and the projections of the three link pairs.
+//
+// The constellation offsets are xd_i = al_i U + be_i V.  Heliocentric orbits (KIND 0):
+// al_i = r' sin kappa_i, be_i = r' cos kappa_i are constants and U, V follow the revolution
+// angle; geocentric orbits (KIND 1): al_i, be_i are sin, cos of node i's rotation angle and
+// U, V are constants.  The kappa_i are equally spaced by 2 pi / 3, so for pair a with nodes
+// (i, j): al_i - al_j = -sqrt3 be_a and be_i - be_j = sqrt3 al_a, and every projection of a
+// link vector n_a = (xd_i - xd_j) / L is
+//   n_a.v = (sqrt3 / L) (al_a v.V - be_a v.U),      k.xd_a = al_a k.U + be_a k.V:
+// six scalar products per mode instead of nine, and two terms per projection.
 struct ModeG {
   double A, t;      // amplitude, t_f + t_c
   Cx u, hm, hp, hc; // unit phasor, A u, F+ A u, Fx A u
   Cx z0, z1, z01;   // pair phasors exp(i pi f k.xd_a) of nodes 0, 1, and their product
-  double sa, ca;    // orbit angle (heliocentric revolution)
-  double sr, cr;    // geocentric rotation angle of node 1
+  double sa, ca;    // revolution angle
   double kx0;
-  V3 xd[3];
-  double px[3], qx[3], kx[3];
+  double np[3], nq[3], kn[3];     // n_a.p, n_a.q, n_a.k
+  double al[3], be[3];            // KIND 1: sin, cos of the node rotation angles
+  double pU, pV, qU, qV, kU, kV;  // KIND 1: (sqrt3/L) p.U, (sqrt3/L) p.V, same for q; k.U, k.V
 };
 
 constexpr double kOmegaYear = 2.0 * PI / YEAR_SI;
 
-template <int PREC>
+template <int PREC, int KIND>
 __device__ inline void mode_geom(const DetectorDev& det, const double* __restrict__ mf,
                                  const double* __restrict__ n0, const BinPre& b, const RowC& rc, ModeG& g) {
   const double A0 = n0[0], dA0 = n0[1], P0 = n0[2], dP0 = n0[3];
@@ -179,46 +171,52 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
   sincos_tab(fma(kOmegaYear, g.t, det.rev0), &g.sa, &g.ca);
   const double sa = g.sa, ca = g.ca;
   g.kx0 = AU_SEC * fma(rc.k.x, ca, rc.k.y * sa);
-  if (det.kind == 0) {
-    const double sc = sa * ca, sa2 = fma(sa, sa, 1.0), ca2 = fma(ca, ca, 1.0);
+  double kx_0, kx_1;
+  if constexpr (KIND == 0) {
+    const double sc = sa * ca;
+    const V3 U = {sc, -fma(ca, ca, 1.0), -SQRT3 * sa};
+    const V3 V = {-fma(sa, sa, 1.0), sc, -SQRT3 * ca};
+    const double pU = dot3(rc.p, U), pV = dot3(rc.p, V);
+    const double qU = dot3(rc.q, U), qV = dot3(rc.q, V);
+    const double kU = dot3(rc.k, U), kV = dot3(rc.k, V);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      g.xd[i].x = fma(sc, det.hsk[i], -sa2 * det.hck[i]);
-      g.xd[i].y = fma(sc, det.hck[i], -ca2 * det.hsk[i]);
-      g.xd[i].z = -SQRT3 * fma(ca, det.hck[i], sa * det.hsk[i]);
+    for (int a = 0; a < 3; ++a) {
+      g.np[a] = fma(det.hsa[a], pV, -det.hca[a] * pU);
+      g.nq[a] = fma(det.hsa[a], qV, -det.hca[a] * qU);
+      g.kn[a] = fma(det.hsa[a], kV, -det.hca[a] * kU);
     }
-    g.sr = 0.0;
-    g.cr = 0.0;
+    kx_0 = fma(det.hsk[0], kU, det.hck[0] * kV);
+    kx_1 = fma(det.hsk[1], kU, det.hck[1] * kV);
   } else {
     double s1, c1;
     sincos_tab(fma(det.omega_rot, g.t, det.rot0), &s1, &c1);
-    g.sr = s1;
-    g.cr = c1;
-    const double cr = -0.5, sr = 0.5 * SQRT3;
-    double sk[3], ck[3];
-    sk[0] = s1;
-    ck[0] = c1;
-    sk[1] = fma(sk[0], cr, ck[0] * sr);
-    ck[1] = fma(ck[0], cr, -sk[0] * sr);
-    sk[2] = fma(sk[1], cr, ck[1] * sr);
-    ck[2] = fma(ck[1], cr, -sk[1] * sr);
+    const double cr = -0.5, sr = 0.5 * SQRT3;  // rotation by 2 pi / 3
+    g.al[0] = s1;
+    g.be[0] = c1;
+    g.al[1] = fma(s1, cr, c1 * sr);
+    g.be[1] = fma(c1, cr, -s1 * sr);
+    g.al[2] = fma(g.al[1], cr, g.be[1] * sr);
+    g.be[2] = fma(g.be[1], cr, -g.al[1] * sr);
+    g.pU = fma(rc.p.x, det.sgxs, fma(rc.p.y, det.sgys, rc.p.z * det.sgzs));
+    g.pV = fma(rc.p.x, det.sgxc, rc.p.y * det.sgyc);
+    g.qU = fma(rc.q.x, det.sgxs, fma(rc.q.y, det.sgys, rc.q.z * det.sgzs));
+    g.qV = fma(rc.q.x, det.sgxc, rc.q.y * det.sgyc);
+    g.kU = fma(rc.k.x, det.gxs, fma(rc.k.y, det.gys, rc.k.z * det.gzs));
+    g.kV = fma(rc.k.x, det.gxc, rc.k.y * det.gyc);
+    const double kUs = det.c3L * g.kU, kVs = det.c3L * g.kV;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      g.xd[i].x = fma(det.gxs, sk[i], det.gxc * ck[i]);
-      g.xd[i].y = fma(det.gys, sk[i], det.gyc * ck[i]);
-      g.xd[i].z = det.gzs * sk[i];
+    for (int a = 0; a < 3; ++a) {
+      g.np[a] = fma(g.al[a], g.pV, -g.be[a] * g.pU);
+      g.nq[a] = fma(g.al[a], g.qV, -g.be[a] * g.qU);
+      g.kn[a] = fma(g.al[a], kVs, -g.be[a] * kUs);
     }
-  }
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    g.px[i] = dot3(rc.p, g.xd[i]);
-    g.qx[i] = dot3(rc.q, g.xd[i]);
-    g.kx[i] = dot3(rc.k, g.xd[i]);
+    kx_0 = fma(g.al[0], g.kU, g.be[0] * g.kV);
+    kx_1 = fma(g.al[1], g.kU, g.be[1] * g.kV);
   }
   {
     double s0, c0, s1, c1;
-    Trig<PREC>::small(b.pif * g.kx[0], &s0, &c0);
-    Trig<PREC>::small(b.pif * g.kx[1], &s1, &c1);
+    sincos_tab(b.pif * kx_0, &s0, &c0);
+    sincos_tab(b.pif * kx_1, &s1, &c1);
     g.z0 = {c0, s0};
     g.z1 = {c1, s1};
     g.z01 = mul(g.z0, g.z1);
@@ -242,43 +240,8 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
   g.hc = mul(Cx{mf[MF_FCR], mf[MF_FCI]}, g.hm);
 }
 
-// time derivatives of the constellation offsets
-__device__ inline void xd_dot(const DetectorDev& det, const ModeG& g, V3 xt[3]) {
-  if (det.kind == 0) {
-    const double sa = g.sa, ca = g.ca;
-    const double w = kOmegaYear;
-    const double sc2 = 2.0 * sa * ca, c2a = (ca - sa) * (ca + sa);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      xt[i].x = w * fma(c2a, det.hsk[i], -sc2 * det.hck[i]);
-      xt[i].y = w * fma(c2a, det.hck[i], sc2 * det.hsk[i]);
-      xt[i].z = (-SQRT3 * w) * fma(ca, det.hsk[i], -sa * det.hck[i]);
-    }
-  } else {
-    const double cr = -0.5, sr = 0.5 * SQRT3;
-    double sk[3], ck[3];
-    sk[0] = g.sr;
-    ck[0] = g.cr;
-    sk[1] = fma(sk[0], cr, ck[0] * sr);
-    ck[1] = fma(ck[0], cr, -sk[0] * sr);
-    sk[2] = fma(sk[1], cr, ck[1] * sr);
-    ck[2] = fma(ck[1], cr, -sk[1] * sr);
-    const double w = det.omega_rot;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      xt[i].x = w * fma(det.gxs, ck[i], -det.gxc * sk[i]);
-      xt[i].y = w * fma(det.gys, ck[i], -det.gyc * sk[i]);
-      xt[i].z = w * det.gzs * ck[i];
-    }
-  }
-}
-
-// sinc(xL +- a) with a = xL k.n, their derivatives, and delta = sinc(xL-a) - sinc(xL+a).
-// With FP32 transfer trig, delta is taken for xL < 0.3 from
-//   delta = 2 x a sum_{m,n>=0} (-1)^{m+n} X^m A^n / ((2m+1)! (2n+1)! (2m+2n+3)),  X = x^2, A = a^2
-// (sinc y = int_0^1 cos(y t) dt, so delta = 2 int_0^1 sin(x t) sin(a t) dt), which has no
-// cancellation; the direct difference would lose the FP32 sines' accuracy there.
-template <int PREC>
+// sinc(xL +- a) with a = xL k.n, their derivatives, and delta = sinc(xL-a) - sinc(xL+a)
+// (cos a, sin a given)
 __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, double ca, double sa, double& sp,
                                  double& sm, double& dsp, double& dsm, double& delta) {
   const double a = xL * kn;
@@ -288,7 +251,7 @@ __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, d
   const double thr = 1e-2;
   const bool small_p = fabs(xp) < thr, small_m = fabs(xm) < thr;
   if (!small_p && !small_m) {
-    const double inv = Trig<PREC>::rcp(xp * xm);
+    const double inv = rcp_d(xp * xm);
     const double ip = xm * inv, im = xp * inv;
     sp = sinp * ip;
     dsp = (cosp - sp) * ip;
@@ -300,7 +263,7 @@ __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, d
       sp = 1.0 - x2 * (1.0 / 6.0 - x2 * (1.0 / 120.0 - x2 * (1.0 / 5040.0)));
       dsp = -xp * (1.0 / 3.0 - x2 * (1.0 / 30.0 - x2 * (1.0 / 840.0)));
     } else {
-      const double ip = Trig<PREC>::rcp(xp);
+      const double ip = rcp_d(xp);
       sp = sinp * ip;
       dsp = (cosp - sp) * ip;
     }
@@ -309,22 +272,12 @@ __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, d
       sm = 1.0 - x2 * (1.0 / 6.0 - x2 * (1.0 / 120.0 - x2 * (1.0 / 5040.0)));
       dsm = -xm * (1.0 / 3.0 - x2 * (1.0 / 30.0 - x2 * (1.0 / 840.0)));
     } else {
-      const double im = Trig<PREC>::rcp(xm);
+      const double im = rcp_d(xm);
       sm = sinm * im;
       dsm = (cosm - sm) * im;
     }
   }
-  if (Trig<PREC>::kFp32Transfer && xL < 0.3) {
-    const double X = xL * xL, A = a * a;
-    // q_m(A) = sum_n c_mn A^n, total degree <= 3
-    const double q0 = 1.0 / 3.0 + A * (-1.0 / 30.0 + A * (1.0 / 840.0 + A * (-1.0 / 45360.0)));
-    const double q1 = -1.0 / 30.0 + A * (1.0 / 252.0 + A * (-1.0 / 6480.0));
-    const double q2 = 1.0 / 840.0 + A * (-1.0 / 6480.0);
-    const double q3 = -1.0 / 45360.0;
-    delta = 2.0 * xL * a * (q0 + X * (q1 + X * (q2 + X * q3)));
-  } else {
-    delta = sm - sp;
-  }
+  delta = sm - sp;
 }
 
 // link pair a: link vector n_a = (xd[ia] - xd[ib]) / L with (ia, ib) = (2,1), (0,2), (1,0),
@@ -339,11 +292,9 @@ struct PairV {
 
 template <int PREC>
 __device__ inline void pair_eval(const DetectorDev& det, const BinPre& b, const ModeG& g, int a, PairV& v) {
-  const int ia = a == 0 ? 2 : a - 1;
-  const int ib = a == 0 ? 1 : (a == 1 ? 2 : 0);
-  v.np = (g.px[ia] - g.px[ib]) * det.invL;
-  v.nq = (g.qx[ia] - g.qx[ib]) * det.invL;
-  v.kn = (g.kx[ia] - g.kx[ib]) * det.invL;
+  v.np = g.np[a];
+  v.nq = g.nq[a];
+  v.kn = g.kn[a];
   v.Pp = (v.np - v.nq) * (v.np + v.nq);
   v.Pc = 2.0 * v.np * v.nq;
   const Cx nhn = {fma(v.Pp, g.hp.re, v.Pc * g.hc.re), fma(v.Pp, g.hp.im, v.Pc * g.hc.im)};
@@ -360,7 +311,7 @@ __device__ inline void pair_eval(const DetectorDev& det, const BinPre& b, const 
     w = mul(g.z1, conjx(g.z0));
   }
   v.C = mul(nhn, v.z);
-  sinc_pair<PREC>(b.xL, b.sxL, b.cxL, v.kn, w.re, w.im, v.sp, v.sm, v.dsp, v.dsm, v.dl);
+  sinc_pair(b.xL, b.sxL, b.cxL, v.kn, w.re, w.im, v.sp, v.sm, v.dsp, v.dsm, v.dl);
 }
 
 struct LinksT {
@@ -387,7 +338,7 @@ __device__ inline void tdi_out(const DetectorDev& det, const TdiC& t, const Link
   }
 }
 
-template <int PREC>
+template <int PREC, int KIND>
 __device__ inline void response_row(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
                                     const double* __restrict__ nodes, size_t pbase, int NNS, const BinPre& b,
                                     const RowC& rc, const TdiC& tdc, Cx out[3]) {
@@ -399,7 +350,7 @@ __device__ inline void response_row(const DetectorDev& det, const double* __rest
     const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
     if (b.f > mf[MF_CUT]) continue;
     ModeG g;
-    mode_geom<PREC>(det, mf, nodes + (pbase + (size_t)mi * NNS + b.j) * 4, b, rc, g);
+    mode_geom<PREC, KIND>(det, mf, nodes + (pbase + (size_t)mi * NNS + b.j) * 4, b, rc, g);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       PairV v;
@@ -426,58 +377,82 @@ struct AdjAcc {
 };
 
 // wl: adjoints of the six links; wT: adjoint of the T pair sum; mid: mode id
-template <int PREC>
+template <int PREC, int KIND>
 __device__ inline void mode_adjoint(const DetectorDev& det, const double* __restrict__ mf, int mid,
                                     const double* __restrict__ n0, const double* __restrict__ g0, const BinPre& b,
                                     const RowC& rc, const Cx wl[6], Cx wT, const double* __restrict__ pc,
                                     AdjAcc& acc) {
   ModeG g;
-  mode_geom<PREC>(det, mf, n0, b, rc, g);
-  double pxb[3] = {0.0, 0.0, 0.0}, qxb[3] = {0.0, 0.0, 0.0}, kxb[3] = {0.0, 0.0, 0.0};
+  mode_geom<PREC, KIND>(det, mf, n0, b, rc, g);
+  // adjoints of n_a.p, n_a.q, n_a.k and of k.xd_a
+  double npb[3], nqb[3], knb[3], kxb[3];
   Cx Hp = {0.0, 0.0}, Hc = {0.0, 0.0};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     PairV v;
     pair_eval<PREC>(det, b, g, a, v);
-    const int ia = a == 0 ? 2 : a - 1;
-    const int ib = a == 0 ? 1 : (a == 1 ? 2 : 0);
     const int lp = a == 0 ? 2 : (a == 1 ? 4 : 0), lm = lp + 1;
     const Cx wp = wl[lp], wm = wl[lm];
     const double gp = fma(wp.re, v.C.re, wp.im * v.C.im);  // Re[conj(w) C]
     const double gm = fma(wm.re, v.C.re, wm.im * v.C.im);
     const double gT = fma(wT.re, v.C.re, wT.im * v.C.im);
-    const double Sbar = b.xL * (v.dsp * gp - v.dsm * gm - (v.dsm + v.dsp) * gT);  // adjoint of k.n_a
+    knb[a] = b.xL * (v.dsp * gp - v.dsm * gm - (v.dsm + v.dsp) * gT);
     const Cx W = {fma(v.sp, wp.re, fma(v.sm, wm.re, v.dl * wT.re)), fma(v.sp, wp.im, fma(v.sm, wm.im, v.dl * wT.im))};
-    // adjoint of k.xd_a through exp(+i pi f k.xd_a): -pi f Im(conj(W) C)
-    kxb[a] -= b.pif * (W.re * v.C.im - W.im * v.C.re);
+    // through exp(+i pi f k.xd_a): -pi f Im(conj(W) C)
+    kxb[a] = -b.pif * (W.re * v.C.im - W.im * v.C.re);
     const Cx V = mul(conjx(W), v.z);  // adjoint of n.h.n
     const double Ppb = V.re * g.hp.re - V.im * g.hp.im;  // Re(V h+)
     const double Pcb = V.re * g.hc.re - V.im * g.hc.im;  // Re(V hx)
-    const double npb = 2.0 * fma(v.np, Ppb, v.nq * Pcb);
-    const double nqb = 2.0 * fma(v.np, Pcb, -v.nq * Ppb);
+    npb[a] = 2.0 * fma(v.np, Ppb, v.nq * Pcb);
+    nqb[a] = 2.0 * fma(v.np, Pcb, -v.nq * Ppb);
     Hp = axpy(v.Pp, V, Hp);
     Hc = axpy(v.Pc, V, Hc);
-    const double npl = npb * det.invL, nql = nqb * det.invL, knl = Sbar * det.invL;
-    pxb[ia] += npl;
-    pxb[ib] -= npl;
-    qxb[ia] += nql;
-    qxb[ib] -= nql;
-    kxb[ia] += knl;
-    kxb[ib] -= knl;
   }
-  // constellation adjoints: p, q, k directly, and the time through the offsets
-  V3 xt[3];
-  xd_dot(det, g, xt);
-  double tgeo = 0.0;
-  V3 pb = {0.0, 0.0, 0.0}, qb = {0.0, 0.0, 0.0}, kb = {0.0, 0.0, 0.0};
+  // constellation adjoints: p, q, k directly, and the time through the orbit angle
+  V3 pb, qb, kb;
+  double tgeo;
+  double pUb = 0.0, pVb = 0.0, qUb = 0.0, qVb = 0.0, kUb = 0.0, kVb = 0.0;
+  if constexpr (KIND == 0) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    axpy3(pxb[i], g.xd[i], pb);
-    axpy3(qxb[i], g.xd[i], qb);
-    axpy3(kxb[i], g.xd[i], kb);
-    tgeo = fma(pxb[i], dot3(rc.p, xt[i]), tgeo);
-    tgeo = fma(qxb[i], dot3(rc.q, xt[i]), tgeo);
-    tgeo = fma(kxb[i], dot3(rc.k, xt[i]), tgeo);
+    for (int a = 0; a < 3; ++a) {
+      pUb = fma(-npb[a], det.hca[a], pUb);
+      pVb = fma(npb[a], det.hsa[a], pVb);
+      qUb = fma(-nqb[a], det.hca[a], qUb);
+      qVb = fma(nqb[a], det.hsa[a], qVb);
+      kUb = fma(-knb[a], det.hca[a], fma(kxb[a], det.hsk[a], kUb));
+      kVb = fma(knb[a], det.hsa[a], fma(kxb[a], det.hck[a], kVb));
+    }
+    const double sa = g.sa, ca = g.ca, sc = sa * ca;
+    const V3 U = {sc, -fma(ca, ca, 1.0), -SQRT3 * sa};
+    const V3 V = {-fma(sa, sa, 1.0), sc, -SQRT3 * ca};
+    pb = {fma(pUb, U.x, pVb * V.x), fma(pUb, U.y, pVb * V.y), fma(pUb, U.z, pVb * V.z)};
+    qb = {fma(qUb, U.x, qVb * V.x), fma(qUb, U.y, qVb * V.y), fma(qUb, U.z, qVb * V.z)};
+    kb = {fma(kUb, U.x, kVb * V.x), fma(kUb, U.y, kVb * V.y), fma(kUb, U.z, kVb * V.z)};
+    // dU/dalpha = (c2, s2, -sqrt3 ca), dV/dalpha = (-s2, c2, sqrt3 sa)
+    const V3 Ua = {fma(pUb, rc.p.x, fma(qUb, rc.q.x, kUb * rc.k.x)), fma(pUb, rc.p.y, fma(qUb, rc.q.y, kUb * rc.k.y)),
+                   fma(pUb, rc.p.z, fma(qUb, rc.q.z, kUb * rc.k.z))};
+    const V3 Va = {fma(pVb, rc.p.x, fma(qVb, rc.q.x, kVb * rc.k.x)), fma(pVb, rc.p.y, fma(qVb, rc.q.y, kVb * rc.k.y)),
+                   fma(pVb, rc.p.z, fma(qVb, rc.q.z, kVb * rc.k.z))};
+    const double c2 = (ca - sa) * (ca + sa), s2 = 2.0 * sc;
+    const double ta = fma(c2, Ua.x, fma(s2, Ua.y, fma(-s2, Va.x, c2 * Va.y)));
+    tgeo = kOmegaYear * fma(SQRT3, fma(sa, Va.z, -ca * Ua.z), ta);
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      pUb = fma(-npb[a], g.be[a], pUb);
+      pVb = fma(npb[a], g.al[a], pVb);
+      qUb = fma(-nqb[a], g.be[a], qUb);
+      qVb = fma(nqb[a], g.al[a], qVb);
+      const double kns = det.c3L * knb[a];
+      kUb = fma(-kns, g.be[a], fma(kxb[a], g.al[a], kUb));
+      kVb = fma(kns, g.al[a], fma(kxb[a], g.be[a], kVb));
+    }
+    pb = {fma(pUb, det.sgxs, pVb * det.sgxc), fma(pUb, det.sgys, pVb * det.sgyc), pUb * det.sgzs};
+    qb = {fma(qUb, det.sgxs, qVb * det.sgxc), fma(qUb, det.sgys, qVb * det.sgyc), qUb * det.sgzs};
+    kb = {fma(kUb, det.gxs, kVb * det.gxc), fma(kUb, det.gys, kVb * det.gyc), kUb * det.gzs};
+    // d al_a / d theta = be_a, d be_a / d theta = -al_a
+    const double th = fma(g.pU, pVb, fma(-g.pV, pUb, fma(g.qU, qVb, fma(-g.qV, qUb, fma(g.kU, kVb, -g.kV * kUb)))));
+    tgeo = det.omega_rot * th;
   }
   // waveform-side adjoints: Gm = Hp Fp + Hc Fc, contributions Re[Gm dh_m]
   const Cx Fp = {mf[MF_FPR], mf[MF_FPI]}, Fc = {mf[MF_FCR], mf[MF_FCI]};
@@ -556,6 +531,112 @@ __device__ inline void wave_sum12(const double x[12], int lane, double z[3]) {
   }
 }
 
+// The rows p0 .. p1-1 of one tile (lane <-> bin) on a detector of orbit kind KIND: lnL and,
+// with GRAD, the 11 gradient terms of each row summed over the tile into acc
+// [TILE / 64][CHUNK][NOUT].  Instantiated per orbit kind so that only that kind's detector
+// constants are held in scalar registers.
+template <int PREC, bool GRAD, int KIND>
+__device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, int p0, int p1,
+                                 const double* __restrict__ bin_f, const int* __restrict__ bin_j,
+                                 const double* __restrict__ bin_s, const double* __restrict__ bin_w,
+                                 const double* __restrict__ dre, const double* __restrict__ dim, int NB,
+                                 const double* __restrict__ tile_dd, const double* __restrict__ pconst,
+                                 const double* __restrict__ nodes, const double* __restrict__ nodes_grad, int NNS,
+                                 double du, const ModeList& modes, int lane, int wave, double* acc) {
+  constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
+  const int bi = tile * TILE + threadIdx.x;
+  BinPre b;
+  bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
+  for (int p = p0; p < p1; ++p) {
+    const double* pc = pconst + (size_t)p * PC_STRIDE;
+    if (f0 > pc[PC_FCUT]) {
+      if (threadIdx.x == 0) acc[(p - p0) * NOUT] -= tile_dd[tile];
+      continue;
+    }
+    double G[PES_N_PARAMS];
+#pragma unroll
+    for (int i = 0; i < PES_N_PARAMS; ++i) G[i] = 0.0;
+    double v = 0.0;
+    if (pc[PC_VALID] != 0.0) {
+      const size_t pbase = (size_t)p * modes.n * NNS;
+      const RowC rc = load_row(pc);
+      TdiC tdc;
+      tdi_consts(det, b, tdc);
+      Cx out[3];
+      response_row<PREC, KIND>(det, pc, modes, nodes, pbase, NNS, b, rc, tdc, out);
+      // data and weights are re-read per row (L1/L2 hits) instead of held in registers
+      Cx wch[3];
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        const double wc = bin_w[ch * NB + bi];
+        const double rr = dre[ch * NB + bi] - out[ch].re, ri = dim[ch * NB + bi] - out[ch].im;
+        v -= wc * fma(rr, rr, ri * ri);
+        wch[ch] = {2.0 * wc * rr, 2.0 * wc * ri};
+      }
+      if (GRAD) {
+        Cx wX, wY, wZ, wT = {0.0, 0.0};
+        if (det.orthogonal) {
+          const double r2 = 0.70710678118654752440, r6 = 0.40824829046386301637;
+          wX = {-r2 * wch[0].re + r6 * wch[1].re, -r2 * wch[0].im + r6 * wch[1].im};
+          wY = {-2.0 * r6 * wch[1].re, -2.0 * r6 * wch[1].im};
+          wZ = {r2 * wch[0].re + r6 * wch[1].re, r2 * wch[0].im + r6 * wch[1].im};
+          wT = mul(conjx(tdc.pfT), wch[2]);
+        } else {
+          wX = wch[0];
+          wY = wch[1];
+          wZ = wch[2];
+        }
+        const Cx cpf = conjx(tdc.pf), cD = conjx(tdc.D);
+        const Cx WX = mul(cpf, wX), WY = mul(cpf, wY), WZ = mul(cpf, wZ);
+        const Cx DWX = mul(cD, WX), DWY = mul(cD, WY), DWZ = mul(cD, WZ);
+        Cx wl[6];
+        wl[0] = sub(DWY, WX);
+        wl[1] = sub(WY, DWX);
+        wl[2] = sub(DWZ, WY);
+        wl[3] = sub(WZ, DWY);
+        wl[4] = sub(DWX, WZ);
+        wl[5] = sub(WX, DWZ);
+        AdjAcc aa;
+        aa.glam = aa.gbet = aa.gpsi = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) aa.gint[i] = 0.0;
+        aa.gh_re = aa.mphi = aa.iota = aa.tc = 0.0;
+        for (int mi = 0; mi < modes.n; ++mi) {
+          const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
+          if (b.f > mf[MF_CUT]) continue;
+          const size_t nrow = pbase + (size_t)mi * NNS + b.j;
+          mode_adjoint<PREC, KIND>(det, mf, modes.id[mi], nodes + nrow * 4, nodes_grad + nrow * 16, b, rc, wl,
+                                   wT, pc, aa);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) G[i] = aa.gint[i];
+        G[4] = -aa.gh_re * pc[PC_INVDL];
+        G[5] = -aa.mphi;
+        G[6] = aa.iota;
+        G[7] = aa.glam;
+        G[8] = aa.gbet;
+        G[9] = aa.gpsi;
+        G[10] = aa.tc;
+      }
+    }
+    if (GRAD) {
+      double x[12], z[3];
+      x[0] = v;
+#pragma unroll
+      for (int i = 0; i < PES_N_PARAMS; ++i) x[1 + i] = G[i];
+      wave_sum12(x, lane, z);
+      if ((lane & 15) == 0) {
+        const int q = lane >> 4;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc[(wave * CHUNK + p - p0) * NOUT + 3 * q + i] += z[i];
+      }
+    } else {
+      v = wave_sum(v);
+      if (lane == 0) acc[(wave * CHUNK + p - p0) * NOUT] += v;
+    }
+  }
+}
+
 // ------------------------------- kernels ---------------------------------------------------
 // Persistent fused kernel.  Work item = (group of tiles_per_group tiles, chunk of CHUNK rows);
 // lane <-> bin.  A tile whose first frequency lies above every mode's cut of a row has a
@@ -604,98 +685,13 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
         }
         continue;
       }
-      const DetectorDev det = dets[tile_detector(dets, n_det, tile)];
-      const int bi = tile * TILE + threadIdx.x;
-      BinPre b;
-      bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
-      for (int p = p0; p < p1; ++p) {
-        const double* pc = pconst + (size_t)p * PC_STRIDE;
-        if (f0 > pc[PC_FCUT]) {
-          if (threadIdx.x == 0) acc[0][p - p0][0] -= tile_dd[tile];
-          continue;
-        }
-        double G[PES_N_PARAMS];
-#pragma unroll
-        for (int i = 0; i < PES_N_PARAMS; ++i) G[i] = 0.0;
-        double v = 0.0;
-        if (pc[PC_VALID] != 0.0) {
-          const size_t pbase = (size_t)p * modes.n * NNS;
-          const RowC rc = load_row(pc);
-          TdiC tdc;
-          tdi_consts(det, b, tdc);
-          Cx out[3];
-          response_row<PREC>(det, pc, modes, nodes, pbase, NNS, b, rc, tdc, out);
-          // data and weights are re-read per row (L1/L2 hits) instead of held in registers
-          Cx wch[3];
-#pragma unroll
-          for (int ch = 0; ch < 3; ++ch) {
-            const double wc = bin_w[ch * NB + bi];
-            const double rr = dre[ch * NB + bi] - out[ch].re, ri = dim[ch * NB + bi] - out[ch].im;
-            v -= wc * fma(rr, rr, ri * ri);
-            wch[ch] = {2.0 * wc * rr, 2.0 * wc * ri};
-          }
-          if (GRAD) {
-            Cx wX, wY, wZ, wT = {0.0, 0.0};
-            if (det.orthogonal) {
-              const double r2 = 0.70710678118654752440, r6 = 0.40824829046386301637;
-              wX = {-r2 * wch[0].re + r6 * wch[1].re, -r2 * wch[0].im + r6 * wch[1].im};
-              wY = {-2.0 * r6 * wch[1].re, -2.0 * r6 * wch[1].im};
-              wZ = {r2 * wch[0].re + r6 * wch[1].re, r2 * wch[0].im + r6 * wch[1].im};
-              wT = mul(conjx(tdc.pfT), wch[2]);
-            } else {
-              wX = wch[0];
-              wY = wch[1];
-              wZ = wch[2];
-            }
-            const Cx cpf = conjx(tdc.pf), cD = conjx(tdc.D);
-            const Cx WX = mul(cpf, wX), WY = mul(cpf, wY), WZ = mul(cpf, wZ);
-            const Cx DWX = mul(cD, WX), DWY = mul(cD, WY), DWZ = mul(cD, WZ);
-            Cx wl[6];
-            wl[0] = sub(DWY, WX);
-            wl[1] = sub(WY, DWX);
-            wl[2] = sub(DWZ, WY);
-            wl[3] = sub(WZ, DWY);
-            wl[4] = sub(DWX, WZ);
-            wl[5] = sub(WX, DWZ);
-            AdjAcc aa;
-            aa.glam = aa.gbet = aa.gpsi = 0.0;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) aa.gint[i] = 0.0;
-            aa.gh_re = aa.mphi = aa.iota = aa.tc = 0.0;
-            for (int mi = 0; mi < modes.n; ++mi) {
-              const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
-              if (b.f > mf[MF_CUT]) continue;
-              const size_t nrow = pbase + (size_t)mi * NNS + b.j;
-              mode_adjoint<PREC>(det, mf, modes.id[mi], nodes + nrow * 4, nodes_grad + nrow * 16, b, rc, wl,
-                                  wT, pc, aa);
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i) G[i] = aa.gint[i];
-            G[4] = -aa.gh_re * pc[PC_INVDL];
-            G[5] = -aa.mphi;
-            G[6] = aa.iota;
-            G[7] = aa.glam;
-            G[8] = aa.gbet;
-            G[9] = aa.gpsi;
-            G[10] = aa.tc;
-          }
-        }
-        if (GRAD) {
-          double x[12], z[3];
-          x[0] = v;
-#pragma unroll
-          for (int i = 0; i < PES_N_PARAMS; ++i) x[1 + i] = G[i];
-          wave_sum12(x, lane, z);
-          if ((lane & 15) == 0) {
-            const int q = lane >> 4;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) acc[wave][p - p0][3 * q + i] += z[i];
-          }
-        } else {
-          v = wave_sum(v);
-          if (lane == 0) acc[wave][p - p0][0] += v;
-        }
-      }
+      const DetectorDev& det = dets[tile_detector(dets, n_det, tile)];
+      if (det.kind == 0)
+        tile_rows<PREC, GRAD, 0>(det, tile, f0, p0, p1, bin_f, bin_j, bin_s, bin_w, dre, dim, NB, tile_dd, pconst,
+                                 nodes, nodes_grad, NNS, du, modes, lane, wave, &acc[0][0][0]);
+      else
+        tile_rows<PREC, GRAD, 1>(det, tile, f0, p0, p1, bin_f, bin_j, bin_s, bin_w, dre, dim, NB, tile_dd, pconst,
+                                 nodes, nodes_grad, NNS, du, modes, lane, wave, &acc[0][0][0]);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < (p1 - p0) * NOUT; i += TILE) {
@@ -726,7 +722,10 @@ __global__ __launch_bounds__(128) void k_response_t(const DetectorDev* __restric
   TdiC tdc;
   tdi_consts(det, b, tdc);
   Cx o[3];
-  response_row<PREC>(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, rc, tdc, o);
+  if (det.kind == 0)
+    response_row<PREC, 0>(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, rc, tdc, o);
+  else
+    response_row<PREC, 1>(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, rc, tdc, o);
   for (int ch = 0; ch < 3; ++ch) {
     const size_t idx = (((size_t)p * 3 + ch) * det.n_bins + k) * 2;
     out[idx] = o[ch].re;

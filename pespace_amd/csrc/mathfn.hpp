@@ -8,7 +8,8 @@
 // Taylor polynomials on [-pi/4, pi/4] (coefficients 1/n!, truncation < 1e-19) and the
 // quadrant k mod 4.
 // sincos_tab: the hot-loop variant.  Reduction by h = 2 pi / 256 in the same FMA form (the
-// first FMA is again exact), sin and cos of the remainder |r| <= pi/256 from short Taylor
+// first FMA is again exact; k is rounded with the 1.5 * 2^52 trick, whose encoding also gives
+// k mod 256 without a conversion), sin and cos of the remainder |r| <= pi/256 from short Taylor
 // polynomials (truncation < 5e-18), and the angle-addition formulas with sin, cos of k h read
 // from a 256-entry table of correctly rounded values held in LDS (4 KiB): about 18 VALU
 // instructions and one LDS read instead of ~37, error within ~2 ulp.
@@ -78,9 +79,13 @@ constexpr double kTabScale = 40.74366543152521;      // 256 / (2 pi)
 constexpr double kTabStepHi = 0.02454369260617026;   // 2 pi / 256 rounded to double
 constexpr double kTabStepLo = 9.567553118338697e-19; // 2 pi / 256 - kTabStepHi, rounded
 
-// table part of sincos_tab: r = x - k 2 pi / 256 already formed
-__device__ inline void sincos_tab_kr(double k, double r, double* s, double* c) {
-  const int idx = (int)fma(-256.0, floor(k * (1.0 / 256.0)), k);  // k mod 256, exact for |k| < 2^53
+// 1.5 * 2^52: for |y| < 2^51, y + kRoundMagic rounds y to the nearest integer n and the low
+// 32 bits of the result's encoding hold n mod 2^32 (two's complement)
+constexpr double kRoundMagic = 6755399441055744.0;
+
+// table part of sincos_tab: km = k + kRoundMagic, r = x - k 2 pi / 256 already formed
+__device__ inline void sincos_tab_kr(double km, double r, double* s, double* c) {
+  const int idx = __double2loint(km) & (kTrigN - 1);  // k mod 256
   const double2 t = g_trig[idx];
   const double r2 = r * r;
   const double sr = fma(r * r2, fma(r2, 1.0 / 120.0, -1.0 / 6.0), r);
@@ -89,20 +94,23 @@ __device__ inline void sincos_tab_kr(double k, double r, double* s, double* c) {
   *c = fma(t.y, cr, -t.x * sr);
 }
 
+// any |x| < 5e13 (k = nearest integer to x 256 / (2 pi), |k| < 2^51)
 __device__ inline void sincos_tab(double x, double* s, double* c) {
-  const double k = rint(x * kTabScale);
+  const double km = fma(x, kTabScale, kRoundMagic);
+  const double k = km - kRoundMagic;
   double r = fma(-k, kTabStepHi, x);
   r = fma(-k, kTabStepLo, r);
-  sincos_tab_kr(k, r, s, c);
+  sincos_tab_kr(km, r, s, c);
 }
 
 // sin, cos of hi + lo (|lo| <= ulp(hi)/2, e.g. the exact sum of two doubles): the exact first
 // reduction step keeps lo's bits
 __device__ inline void sincos_tab2(double hi, double lo, double* s, double* c) {
-  const double k = rint(hi * kTabScale);
+  const double km = fma(hi, kTabScale, kRoundMagic);
+  const double k = km - kRoundMagic;
   double r = fma(-k, kTabStepHi, hi);
   r = fma(-k, kTabStepLo, r) + lo;
-  sincos_tab_kr(k, r, s, c);
+  sincos_tab_kr(km, r, s, c);
 }
 
 // a + b = s + e exactly (Knuth's TwoSum, no ordering of |a|, |b| needed)
@@ -135,14 +143,5 @@ __device__ inline void sincos_rev_fc(double x, double* s, double* c) {
   *s = s1 * nrm;
   *c = c1 * nrm;
 }
-
-// FP32 hardware sine/cosine for |x| <= 4 (|x / 2 pi| < 0.64 revolutions: no reduction needed)
-__device__ inline void sincos_rev_small_f(double x, double* s, double* c) {
-  const float rf = (float)(x * kInv2Pi);
-  *s = (double)__builtin_amdgcn_sinf(rf);
-  *c = (double)__builtin_amdgcn_cosf(rf);
-}
-
-__device__ inline double rcp_f(double x) { return (double)__builtin_amdgcn_rcpf((float)x); }
 
 }  // namespace pes

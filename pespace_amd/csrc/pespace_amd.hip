@@ -539,6 +539,11 @@ extern "C" int pes_context_add_detector(pes_context* ctx, int orbit_kind, double
       d.hck[i] = d.rp * d.ck[i];
       d.hsk[i] = d.rp * d.sk[i];
     }
+    d.c3L = std::sqrt(3.0) * d.invL;
+    for (int i = 0; i < 3; ++i) {
+      d.hca[i] = d.c3L * d.hck[i];
+      d.hsa[i] = d.c3L * d.hsk[i];
+    }
   } else {  // orbit.py:77-139, TianQin defaults
     d.rp = d.L / std::sqrt(3.0);
     double lr = 120.5 / 180.0 * PI, br = -4.7 / 180.0 * PI;
@@ -549,6 +554,12 @@ extern "C" int pes_context_add_detector(pes_context* ctx, int orbit_kind, double
     d.gys = d.rp * d.sbet * d.slam;
     d.gyc = -d.rp * d.clam;
     d.gzs = -d.rp * d.cbet;
+    d.c3L = std::sqrt(3.0) * d.invL;
+    d.sgxs = d.c3L * d.gxs;
+    d.sgxc = d.c3L * d.gxc;
+    d.sgys = d.c3L * d.gys;
+    d.sgyc = d.c3L * d.gyc;
+    d.sgzs = d.c3L * d.gzs;
   }
   h.f.assign(f, f + n_bins);
   h.w.assign(3 * (size_t)n_bins, 0.0);
@@ -753,8 +764,8 @@ static int prepare(pes_context* ctx, const double* d_params, int P, bool grad, h
 
 extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, double* d_lnl, double* d_grad,
                              int precision, void* stream) {
-  if (precision != PES_PREC_FP64 && precision != PES_PREC_MIXED && precision != PES_PREC_MIXED_FAST)
-    return fail(PES_ERR_INVALID, "precision must be PES_PREC_FP64, PES_PREC_MIXED or PES_PREC_MIXED_FAST");
+  if (precision != PES_PREC_FP64 && precision != PES_PREC_MIXED)
+    return fail(PES_ERR_INVALID, "precision must be PES_PREC_FP64 or PES_PREC_MIXED");
   if (P > 0 && !d_lnl) return fail(PES_ERR_INVALID, "lnl output is NULL");
   hipStream_t st = (hipStream_t)stream;
   bool grad = d_grad != nullptr;
@@ -783,9 +794,7 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
     case 0: hipLaunchKernelGGL((k_lnl<0, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
     case 1: hipLaunchKernelGGL((k_lnl<0, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
     case 2: hipLaunchKernelGGL((k_lnl<1, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
-    case 3: hipLaunchKernelGGL((k_lnl<1, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
-    case 4: hipLaunchKernelGGL((k_lnl<2, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
-    default: hipLaunchKernelGGL((k_lnl<2, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
+    default: hipLaunchKernelGGL((k_lnl<1, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
   }
 #undef PES_LNL_ARGS
   HIP_TRY(hipGetLastError());
@@ -812,8 +821,8 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
 
 extern "C" int pes_response(pes_context* ctx, int det, const double* d_params, int P, double* d_out, int precision,
                             void* stream) {
-  if (precision != PES_PREC_FP64 && precision != PES_PREC_MIXED && precision != PES_PREC_MIXED_FAST)
-    return fail(PES_ERR_INVALID, "precision must be PES_PREC_FP64, PES_PREC_MIXED or PES_PREC_MIXED_FAST");
+  if (precision != PES_PREC_FP64 && precision != PES_PREC_MIXED)
+    return fail(PES_ERR_INVALID, "precision must be PES_PREC_FP64 or PES_PREC_MIXED");
   if (!ctx || det < 0 || det >= (int)ctx->hdets.size()) return fail(PES_ERR_INVALID, "bad detector index");
   if (P > 0 && !d_out) return fail(PES_ERR_INVALID, "out is NULL");
   hipStream_t st = (hipStream_t)stream;
@@ -825,10 +834,8 @@ extern "C" int pes_response(pes_context* ctx, int det, const double* d_params, i
       ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1, ctx->du, ctx->modes, P, d_out
   if (precision == PES_PREC_FP64)
     hipLaunchKernelGGL(k_response_t<0>, PES_RESP_ARGS);
-  else if (precision == PES_PREC_MIXED)
-    hipLaunchKernelGGL(k_response_t<1>, PES_RESP_ARGS);
   else
-    hipLaunchKernelGGL(k_response_t<2>, PES_RESP_ARGS);
+    hipLaunchKernelGGL(k_response_t<1>, PES_RESP_ARGS);
 #undef PES_RESP_ARGS
   HIP_TRY(hipGetLastError());
   return PES_OK;

@@ -34,6 +34,11 @@ struct DetectorDev {
   //               angle of node i
   double hck[3], hsk[3];
   double gxs, gxc, gys, gyc, gzs;
+  // link projections n_a.v = (sqrt3/L) (al_a v.V - be_a v.U) with xd_i = al_i U + be_i V:
+  // c3L = sqrt3/L; heliocentric hsa = c3L hsk, hca = c3L hck; geocentric U, V scaled by c3L
+  double c3L;
+  double hsa[3], hca[3];
+  double sgxs, sgxc, sgys, sgyc, sgzs;
 };
 
 struct Vec3 {

@@ -187,3 +187,10 @@ def test_invalid_rows_and_empty_batch():
     np.testing.assert_array_equal(g[[0, 6, 7]], ref_g)
     empty = lk.log_likelihood_batch(torch.empty(0, 11, dtype=torch.float64, device="cuda"))
     assert empty.shape == (0,)
+    # an unknown precision mode is rejected at the C boundary before anything is launched
+    from pespace_amd._lib import lib
+    x = torch.as_tensor(good, device="cuda")
+    out = torch.empty(x.shape[0], dtype=torch.float64, device="cuda")
+    rc = lib().pes_lnl_batch(lk.core._ctx, x.data_ptr(), x.shape[0], out.data_ptr(), None, 2,
+                             torch.cuda.current_stream().cuda_stream)
+    assert rc != 0 and b"precision" in lib().pes_last_error()
