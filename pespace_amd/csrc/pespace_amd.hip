@@ -276,19 +276,30 @@ __device__ inline int tile_detector(const DetectorDev* dets, int n_det, int tile
 
 #include "hotpath.hpp"
 
-__global__ __launch_bounds__(128) void k_reduce(const double* __restrict__ partial, int n_groups, int P, int nout,
-                         const double* __restrict__ pconst, double* __restrict__ lnl, double* __restrict__ grad) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  bool valid = pconst[(size_t)p * PC_STRIDE + PC_VALID] != 0.0;
-  for (int o = 0; o < nout; ++o) {
-    double s = 0.0;
-    for (int g = 0; g < n_groups; ++g) s += partial[((size_t)g * nout + o) * P + p];
-    if (o == 0) {
-      lnl[p] = valid ? s : -INFINITY;
-    } else if (grad) {
-      grad[(size_t)p * PES_N_PARAMS + (o - 1)] = valid ? s : 0.0;
-    }
+// One block per (64 rows, output): each of the 4 waves sums every 4th bin group in order, then
+// the four wave sums are added in a fixed order, so results do not depend on scheduling.
+constexpr int REDUCE_WAVES = 4;  // the final sum below adds exactly four wave sums
+__global__ __launch_bounds__(64 * REDUCE_WAVES) void k_reduce(const double* __restrict__ partial, int n_groups, int P,
+                                                           int nout, const double* __restrict__ pconst,
+                                                           double* __restrict__ lnl, double* __restrict__ grad) {
+  __shared__ double part[REDUCE_WAVES][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int p = blockIdx.x * 64 + lane;
+  const int o = blockIdx.y;
+  double s = 0.0;
+  if (p < P) {
+#pragma unroll 8
+    for (int g = w; g < n_groups; g += REDUCE_WAVES) s += partial[((size_t)g * nout + o) * P + p];
+  }
+  part[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && p < P) {
+    const double t = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+    const bool valid = pconst[(size_t)p * PC_STRIDE + PC_VALID] != 0.0;
+    if (o == 0)
+      lnl[p] = valid ? t : -INFINITY;
+    else if (grad)
+      grad[(size_t)p * PES_N_PARAMS + (o - 1)] = valid ? t : 0.0;
   }
 }
 
@@ -813,8 +824,8 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
     HIP_TRY(hipGetLastError());
     ctx->n_live = P;
   }
-  hipLaunchKernelGGL(k_reduce, dim3((P + 127) / 128), dim3(128), 0, st, ctx->d_partial, n_groups, P, nout,
-                     ctx->d_pconst, d_lnl, d_grad);
+  hipLaunchKernelGGL(k_reduce, dim3((P + 63) / 64, nout), dim3(64 * REDUCE_WAVES), 0, st, ctx->d_partial, n_groups, P,
+                     nout, ctx->d_pconst, d_lnl, d_grad);
   HIP_TRY(hipGetLastError());
   return PES_OK;
 }
