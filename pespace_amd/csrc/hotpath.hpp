@@ -339,8 +339,9 @@ __device__ inline void tdi_out(const DetectorDev& det, const TdiC& t, const Link
 }
 
 template <int PREC, int KIND>
+// nb: node entries (j, j+1) of this bin for mode 0; mode mi's are at nb + mi * mstride
 __device__ inline void response_row(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
-                                    const double* __restrict__ nodes, size_t pbase, int NNS, const BinPre& b,
+                                    const double* __restrict__ nb, size_t mstride, const BinPre& b,
                                     const RowC& rc, const TdiC& tdc, Cx out[3]) {
   LinksT L;
 #pragma unroll
@@ -350,7 +351,7 @@ __device__ inline void response_row(const DetectorDev& det, const double* __rest
     const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
     if (b.f > mf[MF_CUT]) continue;
     ModeG g;
-    mode_geom<PREC, KIND>(det, mf, nodes + (pbase + (size_t)mi * NNS + b.j) * 4, b, rc, g);
+    mode_geom<PREC, KIND>(det, mf, nb + mi * mstride, b, rc, g);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       PairV v;
@@ -535,14 +536,24 @@ __device__ inline void wave_sum12(const double x[12], int lane, double z[3]) {
 // with GRAD, the 11 gradient terms of each row summed over the tile into acc
 // [TILE / 64][CHUNK][NOUT].  Instantiated per orbit kind so that only that kind's detector
 // constants are held in scalar registers.
-template <int PREC, bool GRAD, int KIND>
+//
+// STAGED: the tile's bins use the nj <= NJ_STAGE consecutive node rows j0 .. j0+nj-1 (bins are
+// sorted by frequency, so their node indices are too).  For each row the block copies those
+// node rows of every mode (value and, with GRAD, gradient tables) into one of two LDS buffers
+// and the mode loops read them there: one global-memory latency per row instead of one per
+// (mode, pass).  The buffers alternate from row to row (buf persists across tiles), so the
+// barrier that publishes a row's copy also guarantees that every wave has finished the row
+// that last used the other buffer.  Without STAGED each lane reads its node rows from global
+// memory (tiles whose node range is wider, e.g. the lowest frequencies of short data).
+template <int PREC, bool GRAD, int KIND, bool STAGED>
 __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, int p0, int p1,
                                  const double* __restrict__ bin_f, const int* __restrict__ bin_j,
                                  const double* __restrict__ bin_s, const double* __restrict__ bin_w,
                                  const double* __restrict__ dre, const double* __restrict__ dim, int NB,
                                  const double* __restrict__ tile_dd, const double* __restrict__ pconst,
                                  const double* __restrict__ nodes, const double* __restrict__ nodes_grad, int NNS,
-                                 double du, const ModeList& modes, int lane, int wave, double* acc) {
+                                 double du, const ModeList& modes, int lane, int wave, double* acc, int j0,
+                                 int nj, double* stage_n, double* stage_g, int& buf) {
   constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
   const int bi = tile * TILE + threadIdx.x;
   BinPre b;
@@ -559,11 +570,40 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
     double v = 0.0;
     if (pc[PC_VALID] != 0.0) {
       const size_t pbase = (size_t)p * modes.n * NNS;
+      const double* nb;  // node entries of this lane's bin, mode 0, and the per-mode strides
+      const double* gb;
+      size_t nstride, gstride;
+      if constexpr (STAGED) {
+        double* sn = stage_n + buf * (MAX_MODES * NJ_STAGE * 4);
+        double* sg = stage_g + buf * (MAX_MODES * NJ_STAGE * 16);
+        const size_t r0 = pbase + j0;
+        for (int i = threadIdx.x; i < modes.n * nj * 4; i += TILE) {
+          const int mi = i / (nj * 4), r = i - mi * (nj * 4);
+          sn[mi * (NJ_STAGE * 4) + r] = nodes[(r0 + (size_t)mi * NNS) * 4 + r];
+        }
+        if constexpr (GRAD) {
+          for (int i = threadIdx.x; i < modes.n * nj * 16; i += TILE) {
+            const int mi = i / (nj * 16), r = i - mi * (nj * 16);
+            sg[mi * (NJ_STAGE * 16) + r] = nodes_grad[(r0 + (size_t)mi * NNS) * 16 + r];
+          }
+        }
+        __syncthreads();
+        buf ^= 1;
+        nb = sn + (b.j - j0) * 4;
+        gb = sg + (b.j - j0) * 16;
+        nstride = NJ_STAGE * 4;
+        gstride = NJ_STAGE * 16;
+      } else {
+        nb = nodes + (pbase + b.j) * 4;
+        gb = nodes_grad + (pbase + b.j) * 16;
+        nstride = (size_t)NNS * 4;
+        gstride = (size_t)NNS * 16;
+      }
       const RowC rc = load_row(pc);
       TdiC tdc;
       tdi_consts(det, b, tdc);
       Cx out[3];
-      response_row<PREC, KIND>(det, pc, modes, nodes, pbase, NNS, b, rc, tdc, out);
+      response_row<PREC, KIND>(det, pc, modes, nb, nstride, b, rc, tdc, out);
       // data and weights are re-read per row (L1/L2 hits) instead of held in registers
       Cx wch[3];
 #pragma unroll
@@ -604,9 +644,8 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
         for (int mi = 0; mi < modes.n; ++mi) {
           const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
           if (b.f > mf[MF_CUT]) continue;
-          const size_t nrow = pbase + (size_t)mi * NNS + b.j;
-          mode_adjoint<PREC, KIND>(det, mf, modes.id[mi], nodes + nrow * 4, nodes_grad + nrow * 16, b, rc, wl,
-                                   wT, pc, aa);
+          mode_adjoint<PREC, KIND>(det, mf, modes.id[mi], nb + mi * nstride, gb + mi * gstride, b, rc, wl, wT, pc,
+                                   aa);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) G[i] = aa.gint[i];
@@ -651,10 +690,13 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
                                               int tiles_per_group, const double* __restrict__ pconst,
                                               const double* __restrict__ nodes,
                                               const double* __restrict__ nodes_grad, int NNS, double du,
-                                              ModeList modes, int P, int* __restrict__ work_counter,
-                                              double* __restrict__ partial) {
+                                              ModeList modes, int P, const int* __restrict__ tile_jr,
+                                              int* __restrict__ work_counter, double* __restrict__ partial) {
   constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
   __shared__ double acc[TILE / 64][CHUNK][NOUT];
+  __shared__ double stage_n[2 * MAX_MODES * NJ_STAGE * 4];
+  __shared__ typename std::conditional<GRAD, double[2 * MAX_MODES * NJ_STAGE * 16], double[1]>::type stage_g;
+  int buf = 0;
   __shared__ int next_work;
   trig_table_load();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -686,12 +728,22 @@ __global__ __launch_bounds__(TILE, 2) void k_lnl(const DetectorDev* __restrict__
         continue;
       }
       const DetectorDev& det = dets[tile_detector(dets, n_det, tile)];
-      if (det.kind == 0)
-        tile_rows<PREC, GRAD, 0>(det, tile, f0, p0, p1, bin_f, bin_j, bin_s, bin_w, dre, dim, NB, tile_dd, pconst,
-                                 nodes, nodes_grad, NNS, du, modes, lane, wave, &acc[0][0][0]);
-      else
-        tile_rows<PREC, GRAD, 1>(det, tile, f0, p0, p1, bin_f, bin_j, bin_s, bin_w, dre, dim, NB, tile_dd, pconst,
-                                 nodes, nodes_grad, NNS, du, modes, lane, wave, &acc[0][0][0]);
+      const int j0 = tile_jr[2 * tile], nj = tile_jr[2 * tile + 1];
+#define PES_TILE_ARGS                                                                                         \
+  det, tile, f0, p0, p1, bin_f, bin_j, bin_s, bin_w, dre, dim, NB, tile_dd, pconst, nodes, nodes_grad, NNS, du, \
+      modes, lane, wave, &acc[0][0][0], j0, nj, stage_n, &stage_g[0], buf
+      if (nj <= NJ_STAGE) {
+        if (det.kind == 0)
+          tile_rows<PREC, GRAD, 0, true>(PES_TILE_ARGS);
+        else
+          tile_rows<PREC, GRAD, 1, true>(PES_TILE_ARGS);
+      } else {
+        if (det.kind == 0)
+          tile_rows<PREC, GRAD, 0, false>(PES_TILE_ARGS);
+        else
+          tile_rows<PREC, GRAD, 1, false>(PES_TILE_ARGS);
+      }
+#undef PES_TILE_ARGS
     }
     __syncthreads();
     for (int i = threadIdx.x; i < (p1 - p0) * NOUT; i += TILE) {
@@ -723,9 +775,11 @@ __global__ __launch_bounds__(128) void k_response_t(const DetectorDev* __restric
   tdi_consts(det, b, tdc);
   Cx o[3];
   if (det.kind == 0)
-    response_row<PREC, 0>(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, rc, tdc, o);
+    response_row<PREC, 0>(det, pc, modes, nodes + ((size_t)p * modes.n * NNS + b.j) * 4, (size_t)NNS * 4, b, rc,
+                          tdc, o);
   else
-    response_row<PREC, 1>(det, pc, modes, nodes, (size_t)p * modes.n * NNS, NNS, b, rc, tdc, o);
+    response_row<PREC, 1>(det, pc, modes, nodes + ((size_t)p * modes.n * NNS + b.j) * 4, (size_t)NNS * 4, b, rc,
+                          tdc, o);
   for (int ch = 0; ch < 3; ++ch) {
     const size_t idx = (((size_t)p * 3 + ch) * det.n_bins + k) * 2;
     out[idx] = o[ch].re;

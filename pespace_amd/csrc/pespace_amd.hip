@@ -43,6 +43,8 @@ constexpr int TILE = 256;      // bins per tile = threads per block
 constexpr int CHUNK = 16;      // parameter rows per work item
 constexpr int TILES_PER_GROUP = 32;  // default schedule: tiles per work item ...
 constexpr int BLOCKS_PER_CU = 4;     // ... and persistent blocks launched per CU
+constexpr int MAX_MODES = 6;         // modes per context (ModeList)
+constexpr int NJ_STAGE = 32;         // node rows per tile that the fused kernel stages in LDS
 
 // ---- per-row constant block ------------------------------------------------------------
 enum PConst {
@@ -454,6 +456,7 @@ struct pes_context {
   double *d_bin_f = nullptr, *d_bin_s = nullptr, *d_bin_w = nullptr, *d_dre = nullptr, *d_dim = nullptr;
   int* d_bin_j = nullptr;
   double *d_tile_f0 = nullptr, *d_tile_dd = nullptr;  // first frequency and sum w |d|^2 per tile
+  int* d_tile_jr = nullptr;  // per tile: first node index and number of node rows its bins use
   double* d_node_f = nullptr;  // direct mode only
   // scratch, grown on demand
   int cap_P = 0;
@@ -679,6 +682,15 @@ extern "C" int pes_context_finalize(pes_context* ctx) {
     HIP_TRY(hipMalloc(&ctx->d_tile_dd, nt * sizeof(double)));
     HIP_TRY(hipMemcpy(ctx->d_tile_f0, tf0.data(), nt * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ctx->d_tile_dd, tdd.data(), nt * sizeof(double), hipMemcpyHostToDevice));
+    // node rows j .. j+1 of every bin of a tile (node indices are non-decreasing with f)
+    std::vector<int> tjr(2 * (size_t)nt);
+    for (int t = 0; t < nt; ++t) {
+      const int jf = bj[(size_t)t * TILE], jl = bj[(size_t)t * TILE + TILE - 1];
+      tjr[2 * (size_t)t] = jf;
+      tjr[2 * (size_t)t + 1] = jl - jf + 2;
+    }
+    HIP_TRY(hipMalloc(&ctx->d_tile_jr, 2 * (size_t)nt * sizeof(int)));
+    HIP_TRY(hipMemcpy(ctx->d_tile_jr, tjr.data(), 2 * (size_t)nt * sizeof(int), hipMemcpyHostToDevice));
   }
   if (ctx->direct) ctx->d_node_f = ctx->d_bin_f;
   HIP_TRY(hipMalloc(&ctx->d_work_counter, sizeof(int)));
@@ -699,6 +711,7 @@ extern "C" void pes_context_destroy(pes_context* ctx) {
   free_dev(ctx->d_tile_f0);
   free_dev(ctx->d_work_counter);
   free_dev(ctx->d_tile_dd);
+  free_dev(ctx->d_tile_jr);
   free_dev(ctx->d_setup);
   free_dev(ctx->d_pconst);
   free_dev(ctx->d_nodes);
@@ -800,7 +813,7 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
 #define PES_LNL_ARGS                                                                                         \
   ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre,       \
       ctx->d_dim, ctx->NB, ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, ctx->d_pconst, ctx->d_nodes,   \
-      ctx->d_nodes_grad, NNS, ctx->du, ctx->modes, P, ctx->d_work_counter, ctx->d_partial
+      ctx->d_nodes_grad, NNS, ctx->du, ctx->modes, P, ctx->d_tile_jr, ctx->d_work_counter, ctx->d_partial
   switch (precision * 2 + (grad ? 1 : 0)) {
     case 0: hipLaunchKernelGGL((k_lnl<0, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
     case 1: hipLaunchKernelGGL((k_lnl<0, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
