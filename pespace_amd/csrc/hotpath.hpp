@@ -532,19 +532,61 @@ __device__ inline void wave_sum12(const double x[12], int lane, double z[3]) {
   }
 }
 
+// A row that the fused kernel evaluates on a tile starting at f0: a mode of the row reaches
+// f0 and the row's parameters are valid (the two uniform skips of tile_rows).
+__device__ inline bool row_live(const double* __restrict__ pconst, int p, double f0) {
+  const double* pc = pconst + (size_t)p * PC_STRIDE;
+  return !(f0 > pc[PC_FCUT]) && pc[PC_VALID] != 0.0;
+}
+
+#define PES_LDS(ptr) ((__attribute__((address_space(3))) void*)(ptr))
+
+// Copy node rows r0 .. r0+nj-1 of every mode (value rows of 4 doubles into bn, with GRAD
+// gradient rows of 16 into bg, mode after mode) from global memory to LDS with the LDS-DMA
+// loads (global_load_lds_dwordx4: each lane moves 16 bytes to the wave-uniform LDS base plus
+// 16 x lane, without passing through registers).  Asynchronous: the data are in LDS once
+// the issuing wave's vmcnt reaches zero.  Node rows are 32 / 128 bytes and hipMalloc'd, so
+// every 16-byte piece is aligned.
+template <bool GRAD>
+__device__ inline void stage_nodes_dma(const double* __restrict__ nodes, const double* __restrict__ nodes_grad,
+                                       size_t r0, int NNS, int nmodes, int nj, double* bn, double* bg, int lane,
+                                       int wave) {
+  const int cn = nmodes * nj * 2;  // 16-byte pieces of the value rows
+  for (int base = wave * 64; base < cn; base += TILE) {
+    const int c = base + lane;
+    if (c < cn) {
+      const int mi = c / (nj * 2), w = c - mi * (nj * 2);
+      __builtin_amdgcn_global_load_lds(nodes + (r0 + (size_t)mi * NNS) * 4 + 2 * w, PES_LDS(bn + 2 * base), 16, 0, 0);
+    }
+  }
+  if constexpr (GRAD) {
+    const int cg = nmodes * nj * 8;
+    for (int base = wave * 64; base < cg; base += TILE) {
+      const int c = base + lane;
+      if (c < cg) {
+        const int mi = c / (nj * 8), w = c - mi * (nj * 8);
+        __builtin_amdgcn_global_load_lds(nodes_grad + (r0 + (size_t)mi * NNS) * 16 + 2 * w, PES_LDS(bg + 2 * base),
+                                         16, 0, 0);
+      }
+    }
+  }
+}
+
 // The rows p0 .. p1-1 of one tile (lane <-> bin) on a detector of orbit kind KIND: lnL and,
 // with GRAD, the 11 gradient terms of each row summed over the tile into acc
 // [TILE / 64][CHUNK][NOUT].  Instantiated per orbit kind so that only that kind's detector
 // constants are held in scalar registers.
 //
 // STAGED: the tile's bins use the nj <= NJ_STAGE consecutive node rows j0 .. j0+nj-1 (bins are
-// sorted by frequency, so their node indices are too).  For each row the block copies those
-// node rows of every mode (value and, with GRAD, gradient tables) into one of two LDS buffers
-// and the mode loops read them there: one global-memory latency per row instead of one per
-// (mode, pass).  The buffers alternate from row to row (buf persists across tiles), so the
-// barrier that publishes a row's copy also guarantees that every wave has finished the row
-// that last used the other buffer.  Without STAGED each lane reads its node rows from global
-// memory (tiles whose node range is wider, e.g. the lowest frequencies of short data).
+// sorted by frequency, so their node indices are too).  The block copies those node rows of
+// every mode (value and, with GRAD, gradient tables) into one of two LDS buffers and the mode
+// loops read them there.  The copy for the next evaluated row is issued (LDS-DMA,
+// asynchronous) right after the barrier that publishes the current row's copy, so it overlaps
+// the current row's arithmetic.  The buffers alternate from evaluated row to evaluated row
+// (buf persists across tiles): the barrier of row p also guarantees that every wave has
+// finished the previous row, the last reader of the buffer the copy for the next row goes to.
+// Without STAGED each lane reads its node rows from global memory (tiles whose node range is
+// wider, e.g. the lowest frequencies of short data).
 template <int PREC, bool GRAD, int KIND, bool STAGED>
 __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, int p0, int p1,
                                  const double* __restrict__ bin_f, const int* __restrict__ bin_j,
@@ -558,6 +600,14 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
   const int bi = tile * TILE + threadIdx.x;
   BinPre b;
   bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
+  constexpr int SN = MAX_MODES * NJ_STAGE * 4, SG = MAX_MODES * NJ_STAGE * 16;  // doubles per buffer
+  if constexpr (STAGED) {  // copy for the first evaluated row
+    int q = p0;
+    while (q < p1 && !row_live(pconst, q, f0)) ++q;
+    if (q < p1)
+      stage_nodes_dma<GRAD>(nodes, nodes_grad, (size_t)q * modes.n * NNS + j0, NNS, modes.n, nj, stage_n + buf * SN,
+                            stage_g + buf * SG, lane, wave);
+  }
   for (int p = p0; p < p1; ++p) {
     const double* pc = pconst + (size_t)p * PC_STRIDE;
     if (f0 > pc[PC_FCUT]) {
@@ -574,25 +624,18 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
       const double* gb;
       size_t nstride, gstride;
       if constexpr (STAGED) {
-        double* sn = stage_n + buf * (MAX_MODES * NJ_STAGE * 4);
-        double* sg = stage_g + buf * (MAX_MODES * NJ_STAGE * 16);
-        const size_t r0 = pbase + j0;
-        for (int i = threadIdx.x; i < modes.n * nj * 4; i += TILE) {
-          const int mi = i / (nj * 4), r = i - mi * (nj * 4);
-          sn[mi * (NJ_STAGE * 4) + r] = nodes[(r0 + (size_t)mi * NNS) * 4 + r];
-        }
-        if constexpr (GRAD) {
-          for (int i = threadIdx.x; i < modes.n * nj * 16; i += TILE) {
-            const int mi = i / (nj * 16), r = i - mi * (nj * 16);
-            sg[mi * (NJ_STAGE * 16) + r] = nodes_grad[(r0 + (size_t)mi * NNS) * 16 + r];
-          }
-        }
-        __syncthreads();
+        int qn = p + 1;  // next evaluated row
+        while (qn < p1 && !row_live(pconst, qn, f0)) ++qn;
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's copies for row p are in LDS
+        __syncthreads();                     // ... and every wave's; all waves are past row p - 1
+        if (qn < p1)
+          stage_nodes_dma<GRAD>(nodes, nodes_grad, (size_t)qn * modes.n * NNS + j0, NNS, modes.n, nj,
+                                stage_n + (buf ^ 1) * SN, stage_g + (buf ^ 1) * SG, lane, wave);
+        nb = stage_n + buf * SN + (b.j - j0) * 4;
+        gb = stage_g + buf * SG + (b.j - j0) * 16;
+        nstride = (size_t)nj * 4;
+        gstride = (size_t)nj * 16;
         buf ^= 1;
-        nb = sn + (b.j - j0) * 4;
-        gb = sg + (b.j - j0) * 16;
-        nstride = NJ_STAGE * 4;
-        gstride = NJ_STAGE * 16;
       } else {
         nb = nodes + (pbase + b.j) * 4;
         gb = nodes_grad + (pbase + b.j) * 16;
