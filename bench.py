@@ -55,16 +55,19 @@ WORKLOADS = {
 
 # Algorithmic FP64 work of the fused kernel, counted from its source with add/sub/mul = 1 and
 # FMA = 2; sines, cosines and reciprocals are counted apart (not converted to flops).
-#   forward, per (bin, mode, row) evaluated: Hermite interpolation 20, orbit and the nine
-#     constellation projections 86, phasor and polarization factors 28, three link pairs 195
-#   forward, per (bin, row) evaluated: TDI constants 21, X, Y, Z -> A, E, T 72, residual and
+#   forward, per (bin, mode, row) evaluated: Hermite interpolation 20, orbit angle, the six
+#     scalar products and the nine link projections 76, pair phasors, total phase and
+#     polarization factors 33, three link pairs (sincs, n.e.n, link terms) 132, link sums 36;
+#     transcendentals: 4 sincos, 3 reciprocals
+#   forward, per (bin, row) evaluated: TDI constants 29, X, Y, Z -> A, E, T 72, residual and
 #     accumulation 24
-#   adjoint (lnL + gradient adds): per (bin, mode, row) the recomputed forward 305 and the
-#     adjoint 510; per (bin, row) 139
+#   adjoint (lnL + gradient adds): per (bin, mode, row) the recomputed forward 273 and the
+#     adjoint 488 (link pairs 174, constellation and time 129, waveform side and node-table
+#     contraction 185); per (bin, row) 139
 # "evaluated" = the bin lies at or below the mode's cut (the (bin, row) pairs: below the row's
 # largest cut); the kernel skips the rest, whose response is exactly zero.
-OPS_BM_FWD, OPS_BR_FWD = 329, 117
-OPS_BM_ADJ, OPS_BR_ADJ = 815, 139
+OPS_BM_FWD, OPS_BR_FWD = 297, 125
+OPS_BM_ADJ, OPS_BR_ADJ = 761, 139
 PEAK_FP64 = 78.6e12  # MI355X FP64 vector, FLOP/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lnl_traffic.json")
 
