@@ -250,6 +250,7 @@ def main():
     ap.add_argument("--precision", default="all", choices=("all", "fp64", "mixed"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule of the fused kernel")
+    ap.add_argument("--schedule", default=None, help="fixed schedule 'tiles_per_group,blocks_per_cu' (no autotune)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -280,7 +281,11 @@ def main():
           f"setup {setup_s:.1f}s", file=sys.stderr, flush=True)
 
     tune = None
-    if not args.no_autotune:
+    if args.schedule:
+        t, b = (int(v) for v in args.schedule.split(","))
+        core.set_schedule(t, b)
+        tune = {"tiles_per_group": t, "blocks_per_cu": b, "fixed": True}
+    elif not args.no_autotune:
         # schedule chosen by the autotuner on a 512-row sample of the batch (untimed)
         t0 = time.perf_counter()
         best, times = core.autotune(params[:min(512, args.particles)], grad=grad,
