@@ -40,7 +40,7 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int TILE = 256;      // bins per tile = threads per block
-constexpr int CHUNK = 16;      // parameter rows per work item
+constexpr int CHUNK = 32;      // parameter rows per work item
 constexpr int TILES_PER_GROUP = 32;  // default schedule: tiles per work item ...
 constexpr int BLOCKS_PER_CU = 4;     // ... and persistent blocks launched per CU
 constexpr int MAX_MODES = 6;         // modes per context (ModeList)
