@@ -7,3 +7,6 @@ __all__ = ["WhittleLikelihoodHip", "WhittleLnL", "LikelihoodBilbyInterfaceHip", 
 from .fisher import fisher_matrix, sample_from_fisher, wrap_angle_params
 
 __all__ += ["fisher_matrix", "sample_from_fisher", "wrap_angle_params"]
+from .nuts import TemperedSMCNUTS, nuts_step, softabs_inverse_metric
+
+__all__ += ["TemperedSMCNUTS", "nuts_step", "softabs_inverse_metric"]

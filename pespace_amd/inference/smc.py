@@ -391,7 +391,12 @@ class TemperedSMC:
         state["beta"].copy_(beta_new)
         state["accept"].copy_(acc / self.num_mcmc_steps)
         state["delta"].copy_(delta)
+        self.n_like_evals += self.num_particles * self.num_mcmc_steps
         return state
+
+    def _physical(self, x):
+        """Physical sampled parameters of the particles (the particles themselves here)."""
+        return x
 
     def run(self, max_levels=1000, callback=None):
         """Levels until beta = 1; returns the reference's result fields (without file output)."""
@@ -400,7 +405,6 @@ class TemperedSMC:
         levels = []
         for _ in range(max_levels):
             self.level(state, self.draw())
-            self.n_like_evals += self.num_particles * self.num_mcmc_steps
             rec = {"beta": float(state["beta"]), "delta": float(state["delta"]),
                    "acceptance": float(state["accept"]), "log_evidence": float(state["log_evidence"])}
             levels.append(rec)
@@ -408,7 +412,7 @@ class TemperedSMC:
                 callback(rec)
             if rec["beta"] >= 1.0:
                 break
-        rows = self.prior.rows(state["x"])
+        rows = self.prior.rows(self._physical(state["x"]))
         n = self.num_particles
         return {"posteriors": {k: rows[:, j] for j, k in enumerate(self.prior.names)},
                 "log_likelihoods": state["logl"], "log_evidence": float(state["log_evidence"]),
