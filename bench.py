@@ -290,7 +290,7 @@ def main():
         t0 = time.perf_counter()
         best, times = core.autotune(params[:min(512, args.particles)], grad=grad,
                                     precision="fp64" if args.precision == "all" else args.precision,
-                                    candidates=[(t, b) for t in (8, 16, 32, 64) for b in (2, 4)], reps=1)
+                                    candidates=[(t, b) for t in (8, 16, 32, 64) for b in (2, 3, 4)], reps=1)
         tune = {"tiles_per_group": best[0], "blocks_per_cu": best[1],
                 "candidates_ms": {f"{t}x{b}": round(ms, 2) for (t, b), ms in times.items()}}
         print(f"[bench] autotune {time.perf_counter() - t0:.1f}s: best {best}; "

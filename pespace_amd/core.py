@@ -146,12 +146,13 @@ class LikelihoodCore:
         with HIP events, keep the fastest, and return it with every candidate's mean time (ms).
 
         The default candidates cover work items of 8-64 tiles and 2-8 persistent blocks per CU
-        (with two waves per SIMD, 2 blocks per CU fill the machine once; more add queue slack).
+        (the gradient kernel holds two blocks of four waves per CU, the lnL-only kernel three;
+        blocks beyond those wait and find the work queue drained).
         """
         self._build()
         p = self._params(params)
         P = p.shape[0]
-        cands = candidates or [(t, b) for t in (8, 16, 32, 64) for b in (2, 4, 8)]
+        cands = candidates or [(t, b) for t in (8, 16, 32, 64) for b in (2, 3, 4, 8)]
         lnl = torch.empty(P, dtype=torch.float64, device=p.device)
         g = torch.empty(P, N_PARAMS, dtype=torch.float64, device=p.device) if grad else None
         times = {}
