@@ -508,28 +508,65 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
   }
 }
 
-// Sum 12 per-lane values over the wave with 21 exchanges instead of 12 x 6: the first two
-// butterfly steps trade halves of the value set (lanes with the step's bit set keep the upper
-// half), the last four reduce the remaining three values.  Afterwards the lanes of group
-// q = lane / 16 hold the totals of values 3q .. 3q+2 (in z).
-__device__ inline void wave_sum12(const double x[12], int lane, double z[3]) {
-  const bool h5 = (lane & 32) != 0, h4 = (lane & 16) != 0;
+// Cross-lane moves without LDS (gfx950 VALU permutes and DPP).
+//   half_swap_sum<32>(a, b): v_permlane32_swap on (a, b) leaves lanes 0-31 with (own a, a of
+//     lane + 32) and lanes 32-63 with (b of lane - 32, own b); their sum is a + partner's a in
+//     the lower half and b + partner's b in the upper half.
+//   half_swap_sum<16>: the same with v_permlane16_swap between rows 2k and 2k + 1 of 16 lanes.
+//   dpp_mov<CTRL>: a DPP move of a double (both 32-bit halves) within rows of 16 lanes.
+template <int W>
+__device__ inline double half_swap_sum(double a, double b) {
+  const int alo = __double2loint(a), ahi = __double2hiint(a);
+  const int blo = __double2loint(b), bhi = __double2hiint(b);
+  if constexpr (W == 32) {
+    const auto lo = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+    return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+  } else {
+    const auto lo = __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+    return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+  }
+}
+
+template <int CTRL>
+__device__ inline double dpp_mov(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum of one value over the wave (every lane gets the total, in a fixed order).
+__device__ inline double wave_sum(double v) {
+  v = half_swap_sum<32>(v, v);
+  v = half_swap_sum<16>(v, v);
+  v += dpp_mov<0x128>(v);  // row_ror:8
+  v += dpp_mov<0x124>(v);  // row_ror:4
+  v += dpp_mov<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_mov<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  return v;
+}
+
+// Sum 12 per-lane values over the wave with 9 half-exchanges and 12 in-row moves instead of
+// 12 x 6 exchanges: the first two steps trade halves of the value set (lanes 32-63 keep values
+// 6-11, then rows 1 and 3 keep the upper three of their six), the last four reduce the
+// remaining three values within each row of 16 lanes (rotations by 8 and 4, then the quad
+// permutes lane ^ 2 and lane ^ 1).  Afterwards every lane of row q = lane / 16 holds the totals
+// of values 3q .. 3q+2 (in z), summed in a fixed order.
+__device__ inline void wave_sum12(const double x[12], double z[3]) {
   double y[6];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) {
-    const double keep = h5 ? x[6 + i] : x[i], send = h5 ? x[i] : x[6 + i];
-    y[i] = keep + __shfl_xor(send, 32, 64);
-  }
+  for (int i = 0; i < 6; ++i) y[i] = half_swap_sum<32>(x[i], x[6 + i]);
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const double keep = h4 ? y[3 + i] : y[i], send = h4 ? y[i] : y[3 + i];
-    z[i] = keep + __shfl_xor(send, 16, 64);
-  }
+  for (int i = 0; i < 3; ++i) z[i] = half_swap_sum<16>(y[i], y[3 + i]);
 #pragma unroll
-  for (int off = 8; off > 0; off >>= 1) {
+  for (int i = 0; i < 3; ++i) z[i] += dpp_mov<0x128>(z[i]);  // row_ror:8
 #pragma unroll
-    for (int i = 0; i < 3; ++i) z[i] += __shfl_xor(z[i], off, 64);
-  }
+  for (int i = 0; i < 3; ++i) z[i] += dpp_mov<0x124>(z[i]);  // row_ror:4
+#pragma unroll
+  for (int i = 0; i < 3; ++i) z[i] += dpp_mov<0x4E>(z[i]);   // quad_perm [2, 3, 0, 1]
+#pragma unroll
+  for (int i = 0; i < 3; ++i) z[i] += dpp_mov<0xB1>(z[i]);   // quad_perm [1, 0, 3, 2]
 }
 
 // A row that the fused kernel evaluates on a tile starting at f0: a mode of the row reaches
@@ -706,7 +743,7 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
       x[0] = v;
 #pragma unroll
       for (int i = 0; i < PES_N_PARAMS; ++i) x[1 + i] = G[i];
-      wave_sum12(x, lane, z);
+      wave_sum12(x, z);
       if ((lane & 15) == 0) {
         const int q = lane >> 4;
 #pragma unroll

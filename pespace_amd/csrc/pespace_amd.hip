@@ -255,11 +255,6 @@ __device__ inline void mode_at_bin(const double* __restrict__ nodes, size_t base
   tf = hp.ds / (2.0 * PI * b.du * b.f);
 }
 
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
 
 struct BinData {
   double f, s, w[3];
