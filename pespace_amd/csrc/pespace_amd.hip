@@ -24,6 +24,13 @@
 
 using namespace pes;
 
+namespace pes {
+struct ModeList {  // the modes of a context (ids into kModeM etc.), passed to kernels by value
+  int n;
+  int id[6];
+};
+}  // namespace pes
+
 namespace {
 
 thread_local std::string g_err;
@@ -58,11 +65,6 @@ enum PConst {
 };
 enum { MF_FPR = 0, MF_FPI, MF_FCR, MF_FCI, MF_CUT, MF_DFPR, MF_DFPI, MF_DFCR, MF_DFCI };
 
-struct ModeList {
-  int n;
-  int id[6];
-};
-
 __device__ inline bool finite_d(double x) { return isfinite(x); }
 
 // unit complex factor of a mode: conj(phase of leading PN H_lm) * exp(-i (m-2) pi/8)
@@ -82,6 +84,7 @@ __device__ inline Cplx mode_factor(int id) {
   return cmul(u, {c, s});
 }
 
+#ifndef PES_LNL_FWD_UNIT
 // ---- k_setup ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_setup(const double* __restrict__ params, int P, ModeList modes, double f_ref,
                         double* __restrict__ setup, double* __restrict__ pconst, int* __restrict__ work_counter) {
@@ -224,6 +227,7 @@ __global__ __launch_bounds__(128) void k_nodes(const double* __restrict__ setup,
     }
   }
 }
+#endif  // PES_LNL_FWD_UNIT
 
 // ---- Hermite interpolation between node j and j+1 ---------------------------------------
 struct Herm {
@@ -273,6 +277,7 @@ __device__ inline int tile_detector(const DetectorDev* dets, int n_det, int tile
 
 #include "hotpath.hpp"
 
+#ifndef PES_LNL_FWD_UNIT
 // One block per (64 rows, output): each of the 4 waves sums every 4th bin group in order, then
 // the four wave sums are added in a fixed order, so results do not depend on scheduling.
 constexpr int REDUCE_WAVES = 4;  // the final sum below adds exactly four wave sums
@@ -429,8 +434,37 @@ __global__ void k_whittle_final(const double* __restrict__ work, int nb, double 
   }
 }
 
+#endif  // PES_LNL_FWD_UNIT
+
+#define PES_LNL_PARAMS                                                                                       \
+  const DetectorDev *dets, int n_det, int n_tiles, const double *bin_f, const int *bin_j, const double *bin_s, \
+      const double *bin_w, const double *dre, const double *dim, int NB, const double *tile_f0,               \
+      const double *tile_dd, int tiles_per_group, const double *pconst, const double *nodes,                 \
+      const double *nodes_grad, int NNS, double du, ModeList modes, int P, const int *tile_jr,               \
+      int *work_counter, double *partial
+#define PES_LNL_NAMES                                                                                     \
+  dets, n_det, n_tiles, bin_f, bin_j, bin_s, bin_w, dre, dim, NB, tile_f0, tile_dd, tiles_per_group, pconst, \
+      nodes, nodes_grad, NNS, du, modes, P, tile_jr, work_counter, partial
+
 }  // namespace
 
+// The forward-only instantiations of the fused kernel live in their own compilation unit
+// (lnl_fwd.hip, this file compiled with PES_LNL_FWD_UNIT defined), so that they can be
+// scheduled with their own compiler options (three waves per SIMD against two for the
+// gradient kernel).  This launcher is that unit's only export.
+namespace pes {
+hipError_t launch_lnl_fwd(int precision, int grid, hipStream_t st, PES_LNL_PARAMS);
+}
+
+#ifdef PES_LNL_FWD_UNIT
+hipError_t pes::launch_lnl_fwd(int precision, int grid, hipStream_t st, PES_LNL_PARAMS) {
+  if (precision == 0)
+    hipLaunchKernelGGL((k_lnl<0, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES);
+  else
+    hipLaunchKernelGGL((k_lnl<1, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES);
+  return hipGetLastError();
+}
+#else
 // ================================= host side ================================================
 struct pes_context {
   int device = 0;
@@ -809,12 +843,12 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
   ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre,       \
       ctx->d_dim, ctx->NB, ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, ctx->d_pconst, ctx->d_nodes,   \
       ctx->d_nodes_grad, NNS, ctx->du, ctx->modes, P, ctx->d_tile_jr, ctx->d_work_counter, ctx->d_partial
-  switch (precision * 2 + (grad ? 1 : 0)) {
-    case 0: hipLaunchKernelGGL((k_lnl<0, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
-    case 1: hipLaunchKernelGGL((k_lnl<0, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
-    case 2: hipLaunchKernelGGL((k_lnl<1, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
-    default: hipLaunchKernelGGL((k_lnl<1, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS); break;
-  }
+  if (!grad)
+    HIP_TRY(launch_lnl_fwd(precision, grid, st, PES_LNL_ARGS));
+  else if (precision == 0)
+    hipLaunchKernelGGL((k_lnl<0, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+  else
+    hipLaunchKernelGGL((k_lnl<1, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
 #undef PES_LNL_ARGS
   HIP_TRY(hipGetLastError());
   if (ctx->profiling) {
@@ -964,3 +998,4 @@ extern "C" int pes_profile_work(pes_context* ctx, double* bin_mode_rows, double*
   }
   return PES_OK;
 }
+#endif  // PES_LNL_FWD_UNIT
