@@ -240,10 +240,9 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
   g.hc = mul(Cx{mf[MF_FCR], mf[MF_FCI]}, g.hm);
 }
 
-// sinc(xL +- a) with a = xL k.n, their derivatives, and delta = sinc(xL-a) - sinc(xL+a)
-// (cos a, sin a given)
+// sinc(xL +- a) with a = xL k.n and their derivatives (cos a, sin a given)
 __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, double ca, double sa, double& sp,
-                                 double& sm, double& dsp, double& dsm, double& delta) {
+                                 double& sm, double& dsp, double& dsm) {
   const double a = xL * kn;
   const double xp = xL + a, xm = xL - a;
   const double sinp = fma(sxL, ca, cxL * sa), cosp = fma(cxL, ca, -sxL * sa);
@@ -277,7 +276,6 @@ __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, d
       dsm = (cosm - sm) * im;
     }
   }
-  delta = sm - sp;
 }
 
 // link pair a: link vector n_a = (xd[ia] - xd[ib]) / L with (ia, ib) = (2,1), (0,2), (1,0),
@@ -287,7 +285,7 @@ __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, d
 struct PairV {
   double np, nq, kn, Pp, Pc;
   Cx z, C;  // pair phasor z_a; link term C = (n.h.n) z_a (without the common (-i xL) exp(-i xL))
-  double sp, sm, dsp, dsm, dl;
+  double sp, sm, dsp, dsm;
 };
 
 template <int PREC>
@@ -311,16 +309,17 @@ __device__ inline void pair_eval(const DetectorDev& det, const BinPre& b, const 
     w = mul(g.z1, conjx(g.z0));
   }
   v.C = mul(nhn, v.z);
-  sinc_pair(b.xL, b.sxL, b.cxL, v.kn, w.re, w.im, v.sp, v.sm, v.dsp, v.dsm, v.dl);
+  sinc_pair(b.xL, b.sxL, b.cxL, v.kn, w.re, w.im, v.sp, v.sm, v.dsp, v.dsm);
 }
 
 struct LinksT {
   Cx l[6];
-  Cx tl;  // sum over pairs of delta_a C_a (T channel, before pf (1-D)/sqrt3)
 };
 
-// TDI outputs: A, E from X, Y, Z (tdi.py:1784-1785); T from the pair sum, which equals
-// (X + Y + Z)/sqrt3 = pf (1 - D) sum_a (sinc_- - sinc_+)_a C_a / sqrt3 without cancellation
+// TDI outputs: A, E from X, Y, Z (tdi.py:1784-1785); T from the differences of the two links
+// of each pair: X + Y + Z = pf (1 - D) ((l1 - l0) + (l3 - l2) + (l5 - l4)), so
+// T = (X + Y + Z)/sqrt3 = pfT ((l1 - l0) + (l3 - l2) + (l5 - l4)) without the cancellation
+// of the three channel sums (the factor pf (1 - D) / sqrt3 is formed from its product form)
 __device__ inline void tdi_out(const DetectorDev& det, const TdiC& t, const LinksT& L, Cx out[3]) {
   const Cx* l = L.l;
   const Cx X = mul(t.pf, add(sub(l[5], l[0]), mul(t.D, sub(l[4], l[1]))));
@@ -330,7 +329,7 @@ __device__ inline void tdi_out(const DetectorDev& det, const TdiC& t, const Link
     const double r2 = 0.70710678118654752440, r6 = 0.40824829046386301637;
     out[0] = scale(sub(Z, X), r2);
     out[1] = scale(add(sub(X, scale(Y, 2.0)), Z), r6);
-    out[2] = mul(t.pfT, L.tl);
+    out[2] = mul(t.pfT, add(add(sub(l[1], l[0]), sub(l[3], l[2])), sub(l[5], l[4])));
   } else {
     out[0] = X;
     out[1] = Y;
@@ -346,7 +345,6 @@ __device__ inline void response_row(const DetectorDev& det, const double* __rest
   LinksT L;
 #pragma unroll
   for (int i = 0; i < 6; ++i) L.l[i] = {0.0, 0.0};
-  L.tl = {0.0, 0.0};
   for (int mi = 0; mi < modes.n; ++mi) {
     const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
     if (b.f > mf[MF_CUT]) continue;
@@ -359,7 +357,6 @@ __device__ inline void response_row(const DetectorDev& det, const double* __rest
       const int lp = a == 0 ? 2 : (a == 1 ? 4 : 0), lm = lp + 1;
       L.l[lp] = axpy(v.sp, v.C, L.l[lp]);
       L.l[lm] = axpy(v.sm, v.C, L.l[lm]);
-      L.tl = axpy(v.dl, v.C, L.tl);
     }
   }
   tdi_out(det, tdc, L, out);
@@ -370,18 +367,18 @@ __device__ inline void response_row(const DetectorDev& det, const double* __rest
 // and the propagation direction (contracted with their lambda/beta/psi derivatives once per
 // row); the rest are parameter adjoints.
 struct AdjAcc {
-  double glam, gbet, gpsi;  // lambda, beta, psi (through p, q, k)
+  V3 pb, qb, kb;   // adjoints of p, q, k, summed over the modes
   double gint[4];  // Mc, q, chi1, chi2
   double gh_re;    // sum Re(G h): d lnL / d d_L = -gh_re / d_L
   double mphi;     // sum m phibar: d lnL / d phi_ref = -mphi
   double iota, tc;
 };
 
-// wl: adjoints of the six links; wT: adjoint of the T pair sum; mid: mode id
+// wl: adjoints of the six links (the T channel's folded in); mid: mode id
 template <int PREC, int KIND>
 __device__ inline void mode_adjoint(const DetectorDev& det, const double* __restrict__ mf, int mid,
                                     const double* __restrict__ n0, const double* __restrict__ g0, const BinPre& b,
-                                    const RowC& rc, const Cx wl[6], Cx wT, const double* __restrict__ pc,
+                                    const RowC& rc, const Cx wl[6], const double* __restrict__ pc,
                                     AdjAcc& acc) {
   ModeG g;
   mode_geom<PREC, KIND>(det, mf, n0, b, rc, g);
@@ -396,9 +393,8 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
     const Cx wp = wl[lp], wm = wl[lm];
     const double gp = fma(wp.re, v.C.re, wp.im * v.C.im);  // Re[conj(w) C]
     const double gm = fma(wm.re, v.C.re, wm.im * v.C.im);
-    const double gT = fma(wT.re, v.C.re, wT.im * v.C.im);
-    knb[a] = b.xL * (v.dsp * gp - v.dsm * gm - (v.dsm + v.dsp) * gT);
-    const Cx W = {fma(v.sp, wp.re, fma(v.sm, wm.re, v.dl * wT.re)), fma(v.sp, wp.im, fma(v.sm, wm.im, v.dl * wT.im))};
+    knb[a] = b.xL * (v.dsp * gp - v.dsm * gm);
+    const Cx W = {fma(v.sp, wp.re, v.sm * wm.re), fma(v.sp, wp.im, v.sm * wm.im)};
     // through exp(+i pi f k.xd_a): -pi f Im(conj(W) C)
     kxb[a] = -b.pif * (W.re * v.C.im - W.im * v.C.re);
     const Cx V = mul(conjx(W), v.z);  // adjoint of n.h.n
@@ -409,8 +405,9 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
     Hp = axpy(v.Pp, V, Hp);
     Hc = axpy(v.Pc, V, Hc);
   }
-  // constellation adjoints: p, q, k directly, and the time through the orbit angle
-  V3 pb, qb, kb;
+  // constellation adjoints: p, q, k directly (accumulated over the modes and contracted with
+  // their lambda, beta, psi derivatives once per bin and row), and the time through the orbit
+  // angle
   double tgeo;
   double pUb = 0.0, pVb = 0.0, qUb = 0.0, qVb = 0.0, kUb = 0.0, kVb = 0.0;
   if constexpr (KIND == 0) {
@@ -426,9 +423,12 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
     const double sa = g.sa, ca = g.ca, sc = sa * ca;
     const V3 U = {sc, -fma(ca, ca, 1.0), -SQRT3 * sa};
     const V3 V = {-fma(sa, sa, 1.0), sc, -SQRT3 * ca};
-    pb = {fma(pUb, U.x, pVb * V.x), fma(pUb, U.y, pVb * V.y), fma(pUb, U.z, pVb * V.z)};
-    qb = {fma(qUb, U.x, qVb * V.x), fma(qUb, U.y, qVb * V.y), fma(qUb, U.z, qVb * V.z)};
-    kb = {fma(kUb, U.x, kVb * V.x), fma(kUb, U.y, kVb * V.y), fma(kUb, U.z, kVb * V.z)};
+    acc.pb = {fma(pUb, U.x, fma(pVb, V.x, acc.pb.x)), fma(pUb, U.y, fma(pVb, V.y, acc.pb.y)),
+              fma(pUb, U.z, fma(pVb, V.z, acc.pb.z))};
+    acc.qb = {fma(qUb, U.x, fma(qVb, V.x, acc.qb.x)), fma(qUb, U.y, fma(qVb, V.y, acc.qb.y)),
+              fma(qUb, U.z, fma(qVb, V.z, acc.qb.z))};
+    acc.kb = {fma(kUb, U.x, fma(kVb, V.x, acc.kb.x)), fma(kUb, U.y, fma(kVb, V.y, acc.kb.y)),
+              fma(kUb, U.z, fma(kVb, V.z, acc.kb.z))};
     // dU/dalpha = (c2, s2, -sqrt3 ca), dV/dalpha = (-s2, c2, sqrt3 sa)
     const V3 Ua = {fma(pUb, rc.p.x, fma(qUb, rc.q.x, kUb * rc.k.x)), fma(pUb, rc.p.y, fma(qUb, rc.q.y, kUb * rc.k.y)),
                    fma(pUb, rc.p.z, fma(qUb, rc.q.z, kUb * rc.k.z))};
@@ -448,9 +448,12 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
       kUb = fma(-kns, g.be[a], fma(kxb[a], g.al[a], kUb));
       kVb = fma(kns, g.al[a], fma(kxb[a], g.be[a], kVb));
     }
-    pb = {fma(pUb, det.sgxs, pVb * det.sgxc), fma(pUb, det.sgys, pVb * det.sgyc), pUb * det.sgzs};
-    qb = {fma(qUb, det.sgxs, qVb * det.sgxc), fma(qUb, det.sgys, qVb * det.sgyc), qUb * det.sgzs};
-    kb = {fma(kUb, det.gxs, kVb * det.gxc), fma(kUb, det.gys, kVb * det.gyc), kUb * det.gzs};
+    acc.pb = {fma(pUb, det.sgxs, fma(pVb, det.sgxc, acc.pb.x)), fma(pUb, det.sgys, fma(pVb, det.sgyc, acc.pb.y)),
+              fma(pUb, det.sgzs, acc.pb.z)};
+    acc.qb = {fma(qUb, det.sgxs, fma(qVb, det.sgxc, acc.qb.x)), fma(qUb, det.sgys, fma(qVb, det.sgyc, acc.qb.y)),
+              fma(qUb, det.sgzs, acc.qb.z)};
+    acc.kb = {fma(kUb, det.gxs, fma(kVb, det.gxc, acc.kb.x)), fma(kUb, det.gys, fma(kVb, det.gyc, acc.kb.y)),
+              fma(kUb, det.gzs, acc.kb.z)};
     // d al_a / d theta = be_a, d be_a / d theta = -al_a
     const double th = fma(g.pU, pVb, fma(-g.pV, pUb, fma(g.qU, qVb, fma(-g.qV, qUb, fma(g.kU, kVb, -g.kV * kUb)))));
     tgeo = det.omega_rot * th;
@@ -461,20 +464,8 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
   const Cx Gh = mul(Gm, g.hm);
   const double phib = Gh.im;  // adjoint of the total phase Psi + 2 pi f (t_c + k.x0)
   const double kx0b = 2.0 * b.pif * phib;
-  kb.x = fma(kx0b * AU_SEC, g.ca, kb.x);
-  kb.y = fma(kx0b * AU_SEC, g.sa, kb.y);
-  // p, q, k adjoints contracted with their lambda, beta, psi derivatives (row constants)
-  {
-    const V3 dlp = {pc[PC_DLP], pc[PC_DLP + 1], pc[PC_DLP + 2]};
-    const V3 dlq = {pc[PC_DLQ], pc[PC_DLQ + 1], pc[PC_DLQ + 2]};
-    const V3 dlk = {pc[PC_DLK], pc[PC_DLK + 1], pc[PC_DLK + 2]};
-    acc.glam += dot3(pb, dlp) + dot3(qb, dlq) + dot3(kb, dlk);
-    const V3 dbp = {pc[PC_DBP], pc[PC_DBP + 1], pc[PC_DBP + 2]};
-    const V3 dbq = {pc[PC_DBQ], pc[PC_DBQ + 1], pc[PC_DBQ + 2]};
-    const V3 dbk = {pc[PC_DBK], pc[PC_DBK + 1], pc[PC_DBK + 2]};
-    acc.gbet += dot3(pb, dbp) + dot3(qb, dbq) + dot3(kb, dbk);
-    acc.gpsi += dot3(pb, rc.q) - dot3(qb, rc.p);
-  }
+  acc.kb.x = fma(kx0b * AU_SEC, g.ca, acc.kb.x);
+  acc.kb.y = fma(kx0b * AU_SEC, g.sa, acc.kb.y);
   const double kx0_t = AU_SEC * kOmegaYear * fma(-rc.k.x, g.sa, rc.k.y * g.ca);
   const double tb = fma(kx0b, kx0_t, tgeo);  // adjoint of t = t_f + t_c
   acc.gh_re += Gh.re;
@@ -716,15 +707,21 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
         wl[3] = sub(WZ, DWY);
         wl[4] = sub(DWX, WZ);
         wl[5] = sub(WX, DWZ);
+        // T = pfT sum over pairs of (link lm - link lp): its adjoint moves onto the links
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          wl[2 * a] = sub(wl[2 * a], wT);
+          wl[2 * a + 1] = add(wl[2 * a + 1], wT);
+        }
         AdjAcc aa;
-        aa.glam = aa.gbet = aa.gpsi = 0.0;
+        aa.pb = aa.qb = aa.kb = V3{0.0, 0.0, 0.0};
 #pragma unroll
         for (int i = 0; i < 4; ++i) aa.gint[i] = 0.0;
         aa.gh_re = aa.mphi = aa.iota = aa.tc = 0.0;
         for (int mi = 0; mi < modes.n; ++mi) {
           const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
           if (b.f > mf[MF_CUT]) continue;
-          mode_adjoint<PREC, KIND>(det, mf, modes.id[mi], nb + mi * nstride, gb + mi * gstride, b, rc, wl, wT, pc,
+          mode_adjoint<PREC, KIND>(det, mf, modes.id[mi], nb + mi * nstride, gb + mi * gstride, b, rc, wl, pc,
                                    aa);
         }
 #pragma unroll
@@ -732,9 +729,17 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
         G[4] = -aa.gh_re * pc[PC_INVDL];
         G[5] = -aa.mphi;
         G[6] = aa.iota;
-        G[7] = aa.glam;
-        G[8] = aa.gbet;
-        G[9] = aa.gpsi;
+        {  // p, q, k adjoints contracted with their lambda, beta, psi derivatives (row constants)
+          const V3 dlp = {pc[PC_DLP], pc[PC_DLP + 1], pc[PC_DLP + 2]};
+          const V3 dlq = {pc[PC_DLQ], pc[PC_DLQ + 1], pc[PC_DLQ + 2]};
+          const V3 dlk = {pc[PC_DLK], pc[PC_DLK + 1], pc[PC_DLK + 2]};
+          G[7] = dot3(aa.pb, dlp) + dot3(aa.qb, dlq) + dot3(aa.kb, dlk);
+          const V3 dbp = {pc[PC_DBP], pc[PC_DBP + 1], pc[PC_DBP + 2]};
+          const V3 dbq = {pc[PC_DBQ], pc[PC_DBQ + 1], pc[PC_DBQ + 2]};
+          const V3 dbk = {pc[PC_DBK], pc[PC_DBK + 1], pc[PC_DBK + 2]};
+          G[8] = dot3(aa.pb, dbp) + dot3(aa.qb, dbq) + dot3(aa.kb, dbk);
+          G[9] = dot3(aa.pb, rc.q) - dot3(aa.qb, rc.p);
+        }
         G[10] = aa.tc;
       }
     }
