@@ -19,6 +19,32 @@ from .noise import FrequencyDomainNoiseModel, available_noise_models
 
 BACKENDS = ("hip",)
 
+# DataInformation's fields in the reference's order (tdi.py:103-118): the data_info datasets
+_DATA_INFO_KEYS = ("channels", "duration", "delta_time", "start_time", "minimum_frequency", "maximum_frequency",
+                   "sampling_frequency", "delta_frequency", "time_series_length", "time_samples_array",
+                   "full_frequency_series_length", "full_frequency_samples_array", "frequency_mask_array",
+                   "frequency_samples_array", "frequency_series_length")
+
+
+def _require_h5py():
+    try:
+        import h5py
+    except ImportError as error:
+        raise ImportError("TDIChannelData files are HDF5 (the reference's layout); h5py is required to read or "
+                          "write them and is not installed") from error
+    return h5py
+
+
+def _text(v):
+    """A stored string (str, bytes or a 0-d array of either) as str."""
+    if isinstance(v, np.ndarray):
+        v = v.item() if v.ndim == 0 else v
+    return v.decode("utf-8") if isinstance(v, (bytes, np.bytes_)) else str(v)
+
+
+def _texts(v):
+    return tuple(_text(x) for x in np.asarray(v, dtype=object).ravel())
+
 
 @dataclass(frozen=True)
 class DataInformation:
@@ -226,6 +252,86 @@ class TDIChannelData:
                              "contained by input is different with the TDI data.")
         DataContent = namedtuple("DataContent", self._data_info.channels)
         self.fd_data = DataContent(**{c: getattr(self.fd_data, c) + vals[c] for c in self._data_info.channels})
+
+    # ---- files (tdi.py:1022-1292) -------------------------------------------------------
+    # The reference's HDF5 layout: class_attrs/{scaling, backend, _np_fp, _np_cp},
+    # data_info/<every DataInformation field>, tdi_data/<domain>/<channel>,
+    # noise_data/<kind>/<channel>.  The layout logic works on a flat {path: value} tree;
+    # save_to_file / recover_from_file move that tree through h5py.
+
+    def _file_tree(self):
+        """The datasets save_to_file writes, as {path: value} (tdi.py:1039-1088)."""
+        info = self._data_info
+        if not isinstance(info, DataInformation):
+            raise ValueError("only rfft grids (DataInformation) have a file layout; explicit grids do not")
+        tree = {"class_attrs/scaling": np.array(self.scaling), "class_attrs/backend": self.backend,
+                "class_attrs/_np_fp": "float64", "class_attrs/_np_cp": "complex128"}
+        for key in _DATA_INFO_KEYS:
+            tree[f"data_info/{key}"] = np.array(info.channels if key == "channels" else getattr(info, key))
+        if self.fd_data is not None:
+            for chan in info.channels:
+                tree[f"tdi_data/fd_data/{chan}"] = np.asarray(getattr(self.fd_data, chan))
+        if self.fd_noise_power_density is not None:
+            for chan in info.channels:
+                tree[f"noise_data/fd_noise_power_density/{chan}"] = np.asarray(
+                    getattr(self.fd_noise_power_density, chan))
+        return tree
+
+    @staticmethod
+    def _from_file_tree(tree, filename="<tree>"):
+        """Rebuild an instance from a dataset tree, checking every stored derived quantity of
+        the grid against its recomputation (tdi.py:1120-1292).  Data of any stored backend load
+        into a hip container; time-domain and wavelet-domain data are not held by it and are
+        skipped with a warning."""
+        out = TDIChannelData(scaling=bool(np.asarray(tree["class_attrs/scaling"])), backend="hip")
+        fp, cp = _text(tree["class_attrs/_np_fp"]), _text(tree["class_attrs/_np_cp"])
+        assert fp == "float64" and cp == "complex128", (
+            f"Failed to recover TDIChannelData from file {filename}.\n"
+            "The numerical precision of stored data does not match the settings of the current environment.\n"
+            f"The stored data have _np_fp={fp}, and _np_cp={cp}.\n"
+            "While TDIChannelData in the current environment is initialized with _np_fp=float64, and "
+            "_np_cp=complex128.")
+        g = lambda k: tree[f"data_info/{k}"]
+        out.set_data_info(_texts(g("channels")), float(g("duration")), float(g("delta_time")),
+                          float(g("start_time")), float(g("minimum_frequency")), float(g("maximum_frequency")))
+        info = out._data_info
+        for key in _DATA_INFO_KEYS[6:]:
+            mine, stored = getattr(info, key), np.asarray(g(key))
+            same = np.array_equal(mine, stored) if np.ndim(mine) else mine == stored.item()
+            assert same, (f"Failed to recover TDIChannelData from file {filename}, the `{key}` is different "
+                          "from the stored value.")
+        for domain in ("td_data", "wd_data"):
+            if any(k.startswith(f"tdi_data/{domain}/") for k in tree):
+                warnings.warn(f"{filename}: `{domain}` is not held by the hip TDIChannelData and is skipped.")
+        if any(k.startswith("tdi_data/fd_data/") for k in tree):
+            out._store([np.asarray(tree[f"tdi_data/fd_data/{c}"]) for c in info.channels])
+        if any(k.startswith("noise_data/wd_noise_power_density/") for k in tree):
+            raise NotImplementedError("wavelet-domain noise power densities are not supported")
+        if any(k.startswith("noise_data/fd_noise_power_density/") for k in tree):
+            DataContent = namedtuple("DataContent", info.channels)
+            out.fd_noise_power_density = DataContent(**{
+                c: np.asarray(tree[f"noise_data/fd_noise_power_density/{c}"], np.float64) for c in info.channels})
+        out._reset_flag = True
+        return out
+
+    def save_to_file(self, filename: str) -> None:
+        """tdi.py:1022-1088: the instance as an HDF5 file (fails if the file exists)."""
+        h5py = _require_h5py()
+        with h5py.File(filename, "x") as fh:
+            for path, value in self._file_tree().items():
+                if isinstance(value, str) or (isinstance(value, np.ndarray) and value.dtype.kind == "U"):
+                    value = np.array(value, dtype=h5py.string_dtype(encoding="utf-8"))
+                fh.create_dataset(path, data=value)
+
+    @staticmethod
+    def recover_from_file(filename: str) -> "TDIChannelData":
+        """tdi.py:1090-1292: an instance from a file written by save_to_file."""
+        h5py = _require_h5py()
+        tree = {}
+        with h5py.File(filename, "r") as fh:
+            fh.visititems(lambda name, obj: tree.__setitem__(name, obj[()]) if isinstance(obj, h5py.Dataset)
+                          else None)
+        return TDIChannelData._from_file_tree(tree, filename)
 
     @property
     def fd_data_numpy(self):

@@ -3,8 +3,8 @@
 The oracle would take minutes per row at this size, so these tests check what holds at any
 size: with noiseless data equal to the injection, lnL and its gradient vanish at the
 injection; lnL is negative elsewhere; evaluations are deterministic, independent of the
-batch a row sits in and of the kernel schedule; and the mixed path agrees with FP64 at the
-peak.
+batch a row sits in and of the kernel schedule, and the forward-only kernel reproduces the
+gradient kernel's lnL; and the mixed path agrees with FP64 at the peak.
 """
 
 import numpy as np
@@ -61,6 +61,9 @@ def test_determinism_batch_and_schedule_invariance(full):
     l2, g2 = (x.cpu().numpy() for x in core.lnl(rows, grad=True))
     np.testing.assert_array_equal(l1, l2)
     np.testing.assert_array_equal(g1, g2)
+    # the forward-only kernel (its own compilation unit) gives the gradient kernel's lnL
+    l0 = core.lnl(rows).cpu().numpy()
+    np.testing.assert_allclose(l0, l1, rtol=1e-10)
     for i in (0, 7, 19):
         la, ga = (x.cpu().numpy() for x in core.lnl(rows[i:i + 1], grad=True))
         np.testing.assert_array_equal(la, l1[i:i + 1])
