@@ -57,18 +57,18 @@ WORKLOADS = {
 # FMA = 2; sines, cosines and reciprocals are counted apart (not converted to flops).
 #   forward, per (bin, mode, row) evaluated: Hermite interpolation 20, orbit angle, the six
 #     scalar products and the nine link projections 76, pair phasors, total phase and
-#     polarization factors 33, three link pairs (sincs, n.e.n, link terms) 129, link sums 24;
+#     polarization factors 33, three link pairs (sincs, n.e.n, link terms) 126, link sums 24;
 #     transcendentals: 4 sincos, 3 reciprocals
 #   forward, per (bin, row) evaluated: TDI constants 29, X, Y, Z -> A, E, T (T from the
 #     pair differences of the link sums) 82, residual and accumulation 24
-#   adjoint (lnL + gradient adds): per (bin, mode, row) the recomputed forward 270 and the
-#     adjoint 419 (link pairs 144, constellation and time 90, waveform side and node-table
+#   adjoint (lnL + gradient adds): per (bin, mode, row) the recomputed forward 267 and the
+#     adjoint 416 (link pairs 141, constellation and time 90, waveform side and node-table
 #     contraction 185); per (bin, row) 199 (link adjoints with the T channel's folded in, the
 #     p, q, k adjoints contracted with their lambda, beta, psi derivatives)
 # "evaluated" = the bin lies at or below the mode's cut (the (bin, row) pairs: below the row's
 # largest cut); the kernel skips the rest, whose response is exactly zero.
-OPS_BM_FWD, OPS_BR_FWD = 282, 135
-OPS_BM_ADJ, OPS_BR_ADJ = 689, 199
+OPS_BM_FWD, OPS_BR_FWD = 279, 135
+OPS_BM_ADJ, OPS_BR_ADJ = 683, 199
 PEAK_FP64 = 78.6e12  # MI355X FP64 vector, FLOP/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lnl_traffic.json")
 

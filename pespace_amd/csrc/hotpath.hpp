@@ -148,7 +148,7 @@ __device__ inline RowC load_row(const double* __restrict__ pc) {
 // six scalar products per mode instead of nine, and two terms per projection.
 struct ModeG {
   double A, t;      // amplitude, t_f + t_c
-  Cx u, hm, hp, hc; // unit phasor, A u, F+ A u, Fx A u
+  Cx u, hm, hp, hc; // unit phasor, A u, F+ A u, 2 Fx A u
   Cx z0, z1, z01;   // pair phasors exp(i pi f k.xd_a) of nodes 0, 1, and their product
   double sa, ca;    // revolution angle
   double kx0;
@@ -283,7 +283,7 @@ __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, d
 // sinc(xL(1 + k.n_a)), the (-) link lp + 1 gets sinc(xL(1 - k.n_a))
 
 struct PairV {
-  double np, nq, kn, Pp, Pc;
+  double np, nq, kn, Pp, Pc;  // Pp = n.e+.n, Pc = n.ex.n / 2
   Cx z, C;  // pair phasor z_a; link term C = (n.h.n) z_a (without the common (-i xL) exp(-i xL))
   double sp, sm, dsp, dsm;
 };
@@ -293,8 +293,9 @@ __device__ inline void pair_eval(const DetectorDev& det, const BinPre& b, const 
   v.np = g.np[a];
   v.nq = g.nq[a];
   v.kn = g.kn[a];
-  v.Pp = (v.np - v.nq) * (v.np + v.nq);
-  v.Pc = 2.0 * v.np * v.nq;
+  // n.e+.n = (n.p)^2 - (n.q)^2 and n.ex.n / 2 = (n.p)(n.q) (hc carries the factor 2)
+  v.Pp = fma(v.np, v.np, -v.nq * v.nq);
+  v.Pc = v.np * v.nq;
   const Cx nhn = {fma(v.Pp, g.hp.re, v.Pc * g.hc.re), fma(v.Pp, g.hp.im, v.Pc * g.hc.im)};
   // z_a and exp(i xL k.n_a) = z_ia conj(z_ib), with z_2 = conj(z_0 z_1)
   Cx w;
@@ -399,9 +400,10 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
     kxb[a] = -b.pif * (W.re * v.C.im - W.im * v.C.re);
     const Cx V = mul(conjx(W), v.z);  // adjoint of n.h.n
     const double Ppb = V.re * g.hp.re - V.im * g.hp.im;  // Re(V h+)
-    const double Pcb = V.re * g.hc.re - V.im * g.hc.im;  // Re(V hx)
-    npb[a] = 2.0 * fma(v.np, Ppb, v.nq * Pcb);
-    nqb[a] = 2.0 * fma(v.np, Pcb, -v.nq * Ppb);
+    const double Pcb = V.re * g.hc.re - V.im * g.hc.im;  // Re(V hc), hc = 2 hx
+    const double Ppb2 = 2.0 * Ppb;
+    npb[a] = fma(v.np, Ppb2, v.nq * Pcb);
+    nqb[a] = fma(v.np, Pcb, -v.nq * Ppb2);
     Hp = axpy(v.Pp, V, Hp);
     Hc = axpy(v.Pc, V, Hc);
   }

@@ -58,7 +58,7 @@ enum PConst {
   PC_TC = 0, PC_INVDL, PC_VALID, PC_DL,
   PC_P = 4, PC_Q = 7, PC_K = 10,
   PC_DLP = 13, PC_DLQ = 16, PC_DBP = 19, PC_DBQ = 22, PC_DLK = 25, PC_DBK = 28,
-  PC_MODE = 32,  // per mode: Fp re,im, Fc re,im, cut_hz, dFp/diota re,im, dFc/diota re,im
+  PC_MODE = 32,  // per mode: Fp re,im, 2 Fc re,im, cut_hz, dFp/diota re,im, 2 dFc/diota re,im
   PC_MODE_STRIDE = 9,
   PC_FCUT = 32 + 6 * 9,  // largest mode cut [Hz]: the response is zero above it
   PC_STRIDE = 32 + 6 * 9 + 2
@@ -157,9 +157,10 @@ __global__ __launch_bounds__(64) void k_setup(const double* __restrict__ params,
     double kc = 0.5 * (sgn * ym.v - y.v), dkc = 0.5 * (sgn * ym.d[0] - y.d[0]);  // times i
     double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
     Cplx Fp = cscale(eu, kp / dist);
-    Cplx Fc = cmul({0.0, kc / dist}, eu);
+    // the cross factor is stored doubled: the link term's n.e_x.n = 2 (n.p)(n.q) keeps its 2 there
+    Cplx Fc = cmul({0.0, 2.0 * kc / dist}, eu);
     Cplx dFp = cscale(eu, dkp / dist);
-    Cplx dFc = cmul({0.0, dkc / dist}, eu);
+    Cplx dFc = cmul({0.0, 2.0 * dkc / dist}, eu);
     mf[MF_FPR] = Fp.re; mf[MF_FPI] = Fp.im; mf[MF_FCR] = Fc.re; mf[MF_FCI] = Fc.im;
     mf[MF_CUT] = ph.mode_cut(id) / msec;
     fcut = fmax(fcut, mf[MF_CUT]);
@@ -362,7 +363,7 @@ __global__ __launch_bounds__(128) void k_waveform(const DetectorDev* __restrict_
     sincos(Psi, &sp, &cp);
     Cplx h = {A * cp, -A * sp};
     Cplx hp = cmul({mf[MF_FPR], mf[MF_FPI]}, h);
-    Cplx hc = cmul({mf[MF_FCR], mf[MF_FCI]}, h);
+    Cplx hc = cmul({0.5 * mf[MF_FCR], 0.5 * mf[MF_FCI]}, h);  // stored doubled
     size_t idx = (((size_t)p * modes.n + mi) * det.n_bins + k) * 5;
     out[idx] = hp.re;
     out[idx + 1] = hp.im;
