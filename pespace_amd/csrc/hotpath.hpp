@@ -164,10 +164,10 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
                                  const double* __restrict__ n0, const BinPre& b, const RowC& rc, ModeG& g) {
   const double A0 = n0[0], dA0 = n0[1], P0 = n0[2], dP0 = n0[3];
   const double A1 = n0[4], dA1 = n0[5], P1 = n0[6], dP1 = n0[7];
-  g.A = A0 + fma(b.hb1, A1 - A0, fma(b.hb2, dA0, b.hb3 * dA1));
+  g.A = fma(b.hb1, A1 - A0, fma(b.hb2, dA0, fma(b.hb3, dA1, A0)));
   const double dP = P1 - P0;
-  const double Psi = P0 + fma(b.hb1, dP, fma(b.hb2, dP0, b.hb3 * dP1));
-  g.t = fma(b.he1, dP, fma(b.he2, dP0, b.he3 * dP1)) + rc.tc;
+  const double Psi = fma(b.hb1, dP, fma(b.hb2, dP0, fma(b.hb3, dP1, P0)));
+  g.t = fma(b.he1, dP, fma(b.he2, dP0, fma(b.he3, dP1, rc.tc)));
   sincos_tab(fma(kOmegaYear, g.t, det.rev0), &g.sa, &g.ca);
   const double sa = g.sa, ca = g.ca;
   g.kx0 = AU_SEC * fma(rc.k.x, ca, rc.k.y * sa);
@@ -470,14 +470,14 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
   acc.kb.y = fma(kx0b * AU_SEC, g.sa, acc.kb.y);
   const double kx0_t = AU_SEC * kOmegaYear * fma(-rc.k.x, g.sa, rc.k.y * g.ca);
   const double tb = fma(kx0b, kx0_t, tgeo);  // adjoint of t = t_f + t_c
-  acc.gh_re += Gh.re;
+  acc.gh_re = fma(Gm.re, g.hm.re, fma(-Gm.im, g.hm.im, acc.gh_re));  // += Re(Gm hm)
   acc.mphi = fma((double)kModeM[mid], phib, acc.mphi);
   {
     const Cx dFp = {mf[MF_DFPR], mf[MF_DFPI]}, dFc = {mf[MF_DFCR], mf[MF_DFCI]};
-    const Cx tt = mul(add(mul(Hp, dFp), mul(Hc, dFc)), g.hm);
-    acc.iota += tt.re;
+    const Cx tt = add(mul(Hp, dFp), mul(Hc, dFc));
+    acc.iota = fma(tt.re, g.hm.re, fma(-tt.im, g.hm.im, acc.iota));  // += Re(tt hm)
   }
-  acc.tc += fma(2.0 * b.pif, phib, tb);
+  acc.tc = fma(2.0 * b.pif, phib, acc.tc + tb);
   // intrinsic parameters through the node tables: weights of the eight node entries
   const Cx Gu = mul(Gm, g.u);
   const double Ab = Gu.re;
@@ -489,15 +489,14 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
   for (int i = 0; i < 4; ++i) {
     const double* q0 = g0 + 4 * i;
     const double* q1 = g0 + 16 + 4 * i;
-    double s = cA0 * q0[0];
+    double s = fma(cA0, q0[0], acc.gint[i]);
     s = fma(cdA0, q0[1], s);
     s = fma(cP0, q0[2], s);
     s = fma(cdP0, q0[3], s);
     s = fma(cA1, q1[0], s);
     s = fma(cdA1, q1[1], s);
     s = fma(cP1, q1[2], s);
-    s = fma(cdP1, q1[3], s);
-    acc.gint[i] += s;
+    acc.gint[i] = fma(cdP1, q1[3], s);
   }
 }
 
