@@ -122,17 +122,22 @@ __device__ inline void tdi_consts(const DetectorDev& det, const BinPre& b, TdiC&
   t.pfT = {(pfr * o_re - pfi * o_im) * r3, (pfr * o_im + pfi * o_re) * r3};
 }
 
-// one row's extrinsic constants
+// one row's extrinsic constants, and 2 pi f t_c at this lane's bin reduced by the sine
+// table's step once for all modes (the FP64 phase adds each mode's reduced part to it)
 struct RowC {
   double tc;
   V3 p, q, k;
+  double tc_r;  // 2 pi f t_c - tc_k 2 pi / kTrigN, |tc_r| <= pi / kTrigN (ROW_TC phases)
+  int tc_k;     // low bits of the integer tc_k
 };
-__device__ inline RowC load_row(const double* __restrict__ pc) {
+__device__ inline RowC load_row(const double* __restrict__ pc, const BinPre& b) {
   RowC r;
   r.tc = pc[PC_TC];
   r.p = {pc[PC_P], pc[PC_P + 1], pc[PC_P + 2]};
   r.q = {pc[PC_Q], pc[PC_Q + 1], pc[PC_Q + 2]};
   r.k = {pc[PC_K], pc[PC_K + 1], pc[PC_K + 2]};
+  // 2 pi f t_c rounded once, as the reference's own factor
+  tab_reduce(2.0 * b.pif * r.tc, r.tc_r, r.tc_k);
   return r;
 }
 
@@ -159,7 +164,9 @@ struct ModeG {
 
 constexpr double kOmegaYear = 2.0 * PI / YEAR_SI;
 
-template <int PREC, int KIND>
+// ROW_TC (FP64 path): take 2 pi f t_c's reduction from the row constants (computed once per
+// bin and row) instead of forming the exact total phase per mode
+template <int PREC, int KIND, bool ROW_TC = false>
 __device__ inline void mode_geom(const DetectorDev& det, const double* __restrict__ mf,
                                  const double* __restrict__ n0, const BinPre& b, const RowC& rc, ModeG& g) {
   const double A0 = n0[0], dA0 = n0[1], P0 = n0[2], dP0 = n0[3];
@@ -226,10 +233,15 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
     // unit modulus matters: a mean modulus error of the FP32 pair would scale every bin's
     // |h|^2 the same way; the remaining error is a phase error that changes from bin to bin
     sincos_rev_fc(fma(2.0 * b.pif, rc.tc + g.kx0, Psi), &sp, &cp);
+  } else if constexpr (ROW_TC) {
+    // 2 pi f t_c (rounded once, as the reference's own factor, and reduced once per row) plus
+    // Psi + 2 pi f k.x0, each reduced on its own and the remainders added: the total is never
+    // rounded, so the phase's rounding error does not move with Psi or k.x0, and lnL stays as
+    // smooth in the parameters as with three separate factors
+    sincos_tab_sum(rc.tc_r, rc.tc_k, fma(2.0 * b.pif, g.kx0, Psi), &sp, &cp);
   } else {
-    // 2 pi f t_c (rounded once, as the reference's own factor) plus Psi + 2 pi f k.x0, summed
-    // without rounding: the phase's rounding error then does not move with Psi or k.x0, and
-    // lnL stays as smooth in the parameters as with three separate factors
+    // the same sum formed exactly (TwoSum) and reduced per mode: fewer values live across the
+    // mode loops, which the gradient kernel's register budget prefers
     double hi, lo;
     two_sum(2.0 * b.pif * rc.tc, fma(2.0 * b.pif, g.kx0, Psi), hi, lo);
     sincos_tab2(hi, lo, &sp, &cp);
@@ -338,7 +350,7 @@ __device__ inline void tdi_out(const DetectorDev& det, const TdiC& t, const Link
   }
 }
 
-template <int PREC, int KIND>
+template <int PREC, int KIND, bool ROW_TC = false>
 // nb: node entries (j, j+1) of this bin for mode 0; mode mi's are at nb + mi * mstride
 __device__ inline void response_row(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
                                     const double* __restrict__ nb, size_t mstride, const BinPre& b,
@@ -350,7 +362,7 @@ __device__ inline void response_row(const DetectorDev& det, const double* __rest
     const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
     if (b.f > mf[MF_CUT]) continue;
     ModeG g;
-    mode_geom<PREC, KIND>(det, mf, nb + mi * mstride, b, rc, g);
+    mode_geom<PREC, KIND, ROW_TC>(det, mf, nb + mi * mstride, b, rc, g);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       PairV v;
@@ -671,11 +683,11 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
         nstride = (size_t)NNS * 4;
         gstride = (size_t)NNS * 16;
       }
-      const RowC rc = load_row(pc);
+      const RowC rc = load_row(pc, b);
       TdiC tdc;
       tdi_consts(det, b, tdc);
       Cx out[3];
-      response_row<PREC, KIND>(det, pc, modes, nb, nstride, b, rc, tdc, out);
+      response_row<PREC, KIND, !GRAD>(det, pc, modes, nb, nstride, b, rc, tdc, out);
       // data and weights are re-read per row (L1/L2 hits) instead of held in registers
       Cx wch[3];
 #pragma unroll
@@ -859,7 +871,7 @@ __global__ __launch_bounds__(128) void k_response_t(const DetectorDev* __restric
   BinPre b;
   bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
   const double* pc = pconst + (size_t)p * PC_STRIDE;
-  const RowC rc = load_row(pc);
+  const RowC rc = load_row(pc, b);
   TdiC tdc;
   tdi_consts(det, b, tdc);
   Cx o[3];

@@ -113,6 +113,31 @@ __device__ inline void sincos_tab2(double hi, double lo, double* s, double* c) {
   sincos_tab_kr(km, r, s, c);
 }
 
+// x = k 2 pi / kTrigN + r with k the nearest integer to x kTrigN / (2 pi) (|x| < 5e13): r and
+// the low 32 bits of k
+__device__ inline void tab_reduce(double x, double& r, int& k_lo) {
+  const double km = fma(x, kTabScale, kRoundMagic);
+  const double k = km - kRoundMagic;
+  r = fma(-k, kTabStepLo, fma(-k, kTabStepHi, x));
+  k_lo = __double2loint(km);
+}
+
+// sin, cos of a + x with a given reduced (a = ka 2 pi / kTrigN + ra, ka's low bits): x is
+// reduced on its own and the two remainders added, so a + x is never rounded; the remainder
+// is then up to 2 pi / kTrigN, so the sine series takes one more term
+__device__ inline void sincos_tab_sum(double ra, int ka_lo, double x, double* s, double* c) {
+  double rx;
+  int kx_lo;
+  tab_reduce(x, rx, kx_lo);
+  const double r = ra + rx;
+  const double2 t = g_trig[(ka_lo + kx_lo) & (kTrigN - 1)];
+  const double r2 = r * r;
+  const double sr = fma(r * r2, fma(r2, fma(r2, -1.0 / 5040.0, 1.0 / 120.0), -1.0 / 6.0), r);
+  const double cr = fma(r2, fma(r2, fma(r2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+  *s = fma(t.x, cr, t.y * sr);
+  *c = fma(t.y, cr, -t.x * sr);
+}
+
 // a + b = s + e exactly (Knuth's TwoSum, no ordering of |a|, |b| needed)
 __device__ inline void two_sum(double a, double b, double& s, double& e) {
   s = a + b;
