@@ -194,3 +194,40 @@ def test_invalid_rows_and_empty_batch():
     rc = lib().pes_lnl_batch(lk.core._ctx, x.data_ptr(), x.shape[0], out.data_ptr(), None, 2,
                              torch.cuda.current_stream().cuda_stream)
     assert rc != 0 and b"precision" in lib().pes_last_error()
+
+
+@pytest.mark.parametrize("case", ["decreasing", "too_high", "psd", "generation", "orbit", "schedule"])
+def test_detector_validation_at_the_boundary(case):
+    """Grids and settings the kernel cannot take are rejected by pes_context_add_detector (or
+    pes_context_set_schedule) with a message, before any kernel runs; the context is released."""
+    from pespace_amd._lib import PespaceError
+    from pespace_amd.core import LikelihoodCore
+    n = 64
+    f = np.linspace(1e-3, 2e-3, n)
+    psd = np.ones((3, n))
+    kind, gen = 0, 15
+    if case == "decreasing":
+        f[10] = f[9]
+        match = "strictly increasing"
+    elif case == "too_high":
+        f = np.linspace(0.05, 0.2, n)  # pi f L reaches 5.2 for 2.5e9 m arms
+        match = "pi f L"
+    elif case == "psd":
+        psd[1, 7] = 0.0
+        match = "PSD must be positive"
+    elif case == "generation":
+        gen = 17
+        match = "tdi_generation"
+    elif case == "orbit":
+        kind = 2
+        match = "orbit_kind"
+    else:
+        match = "schedule"
+    core = LikelihoodCore(modes=["22"], n_nodes=64, f_node_min=1e-4, f_node_max=0.3)
+    core.add_detector(kind, 2.5e9, 0.0, 0.0, gen, True, f, np.zeros((3, n), complex), psd, (1, 1, 1), df=1e-5)
+    with pytest.raises(PespaceError, match=match):
+        if case == "schedule":
+            core.set_schedule(0, 2)
+        else:
+            core.lnl(inj_row()[None, :])
+    assert case == "schedule" or core._ctx is None
