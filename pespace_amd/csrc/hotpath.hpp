@@ -631,8 +631,7 @@ __device__ inline void stage_nodes_dma(const double* __restrict__ nodes, const d
 template <int PREC, bool GRAD, int KIND, bool STAGED>
 __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, int p0, int p1,
                                  const double* __restrict__ bin_f, const int* __restrict__ bin_j,
-                                 const double* __restrict__ bin_s, const double* __restrict__ bin_w,
-                                 const double* __restrict__ dre, const double* __restrict__ dim, int NB,
+                                 const double* __restrict__ bin_s, const double* __restrict__ bin_rec,
                                  const double* __restrict__ tile_dd, const double* __restrict__ pconst,
                                  const double* __restrict__ nodes, const double* __restrict__ nodes_grad, int NNS,
                                  double du, const ModeList& modes, int lane, int wave, double* acc, int j0,
@@ -661,6 +660,22 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
     double v = 0.0;
     if (pc[PC_VALID] != 0.0) {
       const size_t pbase = (size_t)p * modes.n * NNS;
+      // this bin's weights and data (w, Re d, Im d per channel): re-read per row (cache hits)
+      // rather than held in registers across the tile, and requested before the mode loops so
+      // that their latency is hidden behind them (in the staged path after the barrier and
+      // before the next row's copies, so that no wait for those copies comes first)
+      double rec[BIN_REC];
+      auto load_rec = [&]() {
+        int off = bi * BIN_REC;
+        asm volatile("" : "+v"(off));  // a per-row offset: the loads stay in the row loop
+        const double2* r2 = reinterpret_cast<const double2*>(bin_rec + off);
+#pragma unroll
+        for (int i = 0; i < BIN_REC / 2; ++i) {
+          const double2 t = r2[i];
+          rec[2 * i] = t.x;
+          rec[2 * i + 1] = t.y;
+        }
+      };
       const double* nb;  // node entries of this lane's bin, mode 0, and the per-mode strides
       const double* gb;
       size_t nstride, gstride;
@@ -669,6 +684,7 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
         while (qn < p1 && !row_live(pconst, qn, f0)) ++qn;
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's copies for row p are in LDS
         __syncthreads();                     // ... and every wave's; all waves are past row p - 1
+        load_rec();
         if (qn < p1)
           stage_nodes_dma<GRAD>(nodes, nodes_grad, (size_t)qn * modes.n * NNS + j0, NNS, modes.n, nj,
                                 stage_n + (buf ^ 1) * SN, stage_g + (buf ^ 1) * SG, lane, wave);
@@ -678,6 +694,7 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
         gstride = (size_t)nj * 16;
         buf ^= 1;
       } else {
+        load_rec();
         nb = nodes + (pbase + b.j) * 4;
         gb = nodes_grad + (pbase + b.j) * 16;
         nstride = (size_t)NNS * 4;
@@ -688,12 +705,11 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
       tdi_consts(det, b, tdc);
       Cx out[3];
       response_row<PREC, KIND, !GRAD>(det, pc, modes, nb, nstride, b, rc, tdc, out);
-      // data and weights are re-read per row (L1/L2 hits) instead of held in registers
       Cx wch[3];
 #pragma unroll
       for (int ch = 0; ch < 3; ++ch) {
-        const double wc = bin_w[ch * NB + bi];
-        const double rr = dre[ch * NB + bi] - out[ch].re, ri = dim[ch * NB + bi] - out[ch].im;
+        const double wc = rec[3 * ch];
+        const double rr = rec[3 * ch + 1] - out[ch].re, ri = rec[3 * ch + 2] - out[ch].im;
         v -= wc * fma(rr, rr, ri * ri);
         wch[ch] = {2.0 * wc * rr, 2.0 * wc * ri};
       }
@@ -785,8 +801,7 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
 template <int PREC, bool GRAD>
 __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* __restrict__ dets, int n_det, int n_tiles,
                                               const double* __restrict__ bin_f, const int* __restrict__ bin_j,
-                                              const double* __restrict__ bin_s, const double* __restrict__ bin_w,
-                                              const double* __restrict__ dre, const double* __restrict__ dim, int NB,
+                                              const double* __restrict__ bin_s, const double* __restrict__ bin_rec,
                                               const double* __restrict__ tile_f0, const double* __restrict__ tile_dd,
                                               int tiles_per_group, const double* __restrict__ pconst,
                                               const double* __restrict__ nodes,
@@ -831,7 +846,7 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
       const DetectorDev& det = dets[tile_detector(dets, n_det, tile)];
       const int j0 = tile_jr[2 * tile], nj = tile_jr[2 * tile + 1];
 #define PES_TILE_ARGS                                                                                         \
-  det, tile, f0, p0, p1, bin_f, bin_j, bin_s, bin_w, dre, dim, NB, tile_dd, pconst, nodes, nodes_grad, NNS, du, \
+  det, tile, f0, p0, p1, bin_f, bin_j, bin_s, bin_rec, tile_dd, pconst, nodes, nodes_grad, NNS, du, \
       modes, lane, wave, &acc[0][0][0], j0, nj, stage_n, &stage_g[0], buf
       if (nj <= NJ_STAGE) {
         if (det.kind == 0)

@@ -52,6 +52,7 @@ constexpr int TILES_PER_GROUP = 32;  // default schedule: tiles per work item ..
 constexpr int BLOCKS_PER_CU = 4;     // ... and persistent blocks launched per CU
 constexpr int MAX_MODES = 6;         // modes per context (ModeList)
 constexpr int NJ_STAGE = 32;         // node rows per tile that the fused kernel stages in LDS
+constexpr int BIN_REC = 10;          // doubles per bin of the fused kernel's weight and data record
 
 // ---- per-row constant block ------------------------------------------------------------
 enum PConst {
@@ -439,12 +440,12 @@ __global__ void k_whittle_final(const double* __restrict__ work, int nb, double 
 
 #define PES_LNL_PARAMS                                                                                       \
   const DetectorDev *dets, int n_det, int n_tiles, const double *bin_f, const int *bin_j, const double *bin_s, \
-      const double *bin_w, const double *dre, const double *dim, int NB, const double *tile_f0,               \
+      const double *bin_rec, const double *tile_f0,                                                       \
       const double *tile_dd, int tiles_per_group, const double *pconst, const double *nodes,                 \
       const double *nodes_grad, int NNS, double du, ModeList modes, int P, const int *tile_jr,               \
       int *work_counter, double *partial
 #define PES_LNL_NAMES                                                                                     \
-  dets, n_det, n_tiles, bin_f, bin_j, bin_s, bin_w, dre, dim, NB, tile_f0, tile_dd, tiles_per_group, pconst, \
+  dets, n_det, n_tiles, bin_f, bin_j, bin_s, bin_rec, tile_f0, tile_dd, tiles_per_group, pconst, \
       nodes, nodes_grad, NNS, du, modes, P, tile_jr, work_counter, partial
 
 }  // namespace
@@ -483,7 +484,8 @@ struct pes_context {
   // device constant data
   DetectorDev* d_dets = nullptr;
   int n_tiles = 0, NB = 0;
-  double *d_bin_f = nullptr, *d_bin_s = nullptr, *d_bin_w = nullptr, *d_dre = nullptr, *d_dim = nullptr;
+  double *d_bin_f = nullptr, *d_bin_s = nullptr;
+  double* d_bin_rec = nullptr;  // per bin: (w, Re d, Im d) of the three channels and one pad, 80 B
   int* d_bin_j = nullptr;
   double *d_tile_f0 = nullptr, *d_tile_dd = nullptr;  // first frequency and sum w |d|^2 per tile
   int* d_tile_jr = nullptr;  // per tile: first node index and number of node rows its bins use
@@ -686,15 +688,20 @@ extern "C" int pes_context_finalize(pes_context* ctx) {
   HIP_TRY(hipMalloc(&ctx->d_bin_f, NB * sizeof(double)));
   HIP_TRY(hipMalloc(&ctx->d_bin_s, NB * sizeof(double)));
   HIP_TRY(hipMalloc(&ctx->d_bin_j, NB * sizeof(int)));
-  HIP_TRY(hipMalloc(&ctx->d_bin_w, 3 * (size_t)NB * sizeof(double)));
-  HIP_TRY(hipMalloc(&ctx->d_dre, 3 * (size_t)NB * sizeof(double)));
-  HIP_TRY(hipMalloc(&ctx->d_dim, 3 * (size_t)NB * sizeof(double)));
+  HIP_TRY(hipMalloc(&ctx->d_bin_rec, BIN_REC * (size_t)NB * sizeof(double)));
   HIP_TRY(hipMemcpy(ctx->d_bin_f, bf.data(), NB * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ctx->d_bin_s, bs.data(), NB * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ctx->d_bin_j, bj.data(), NB * sizeof(int), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(ctx->d_bin_w, bw.data(), 3 * (size_t)NB * sizeof(double), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(ctx->d_dre, dre.data(), 3 * (size_t)NB * sizeof(double), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(ctx->d_dim, dim.data(), 3 * (size_t)NB * sizeof(double), hipMemcpyHostToDevice));
+  {
+    std::vector<double> rec(BIN_REC * (size_t)NB, 0.0);
+    for (size_t k = 0; k < (size_t)NB; ++k)
+      for (int ch = 0; ch < 3; ++ch) {
+        rec[k * BIN_REC + 3 * ch] = bw[(size_t)ch * NB + k];
+        rec[k * BIN_REC + 3 * ch + 1] = dre[(size_t)ch * NB + k];
+        rec[k * BIN_REC + 3 * ch + 2] = dim[(size_t)ch * NB + k];
+      }
+    HIP_TRY(hipMemcpy(ctx->d_bin_rec, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   {
     const int nt = ctx->n_tiles;
     std::vector<double> tf0(nt), tdd(nt, 0.0);
@@ -735,9 +742,7 @@ extern "C" void pes_context_destroy(pes_context* ctx) {
   free_dev(ctx->d_bin_f);
   free_dev(ctx->d_bin_s);
   free_dev(ctx->d_bin_j);
-  free_dev(ctx->d_bin_w);
-  free_dev(ctx->d_dre);
-  free_dev(ctx->d_dim);
+  free_dev(ctx->d_bin_rec);
   free_dev(ctx->d_tile_f0);
   free_dev(ctx->d_work_counter);
   free_dev(ctx->d_tile_dd);
@@ -841,8 +846,8 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
   const int n_det = (int)ctx->hdets.size();
   if (grad) nout = 1 + PES_N_PARAMS;
 #define PES_LNL_ARGS                                                                                         \
-  ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_w, ctx->d_dre,       \
-      ctx->d_dim, ctx->NB, ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, ctx->d_pconst, ctx->d_nodes,   \
+  ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_rec,                 \
+      ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, ctx->d_pconst, ctx->d_nodes,   \
       ctx->d_nodes_grad, NNS, ctx->du, ctx->modes, P, ctx->d_tile_jr, ctx->d_work_counter, ctx->d_partial
   if (!grad)
     HIP_TRY(launch_lnl_fwd(precision, grid, st, PES_LNL_ARGS));
