@@ -661,8 +661,8 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
     if (pc[PC_VALID] != 0.0) {
       const size_t pbase = (size_t)p * modes.n * NNS;
       // this bin's weights and data (w, Re d, Im d per channel): re-read per row (cache hits)
-      // rather than held in registers across the tile, and requested before the mode loops so
-      // that their latency is hidden behind them (in the staged path after the barrier and
+      // rather than held in registers across the tile; the gradient kernel requests them
+      // before the mode loops so that their latency is hidden behind them (in the staged path after the barrier and
       // before the next row's copies, so that no wait for those copies comes first)
       double rec[BIN_REC];
       auto load_rec = [&]() {
@@ -684,7 +684,7 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
         while (qn < p1 && !row_live(pconst, qn, f0)) ++qn;
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's copies for row p are in LDS
         __syncthreads();                     // ... and every wave's; all waves are past row p - 1
-        load_rec();
+        if constexpr (GRAD) load_rec();
         if (qn < p1)
           stage_nodes_dma<GRAD>(nodes, nodes_grad, (size_t)qn * modes.n * NNS + j0, NNS, modes.n, nj,
                                 stage_n + (buf ^ 1) * SN, stage_g + (buf ^ 1) * SG, lane, wave);
@@ -694,7 +694,7 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
         gstride = (size_t)nj * 16;
         buf ^= 1;
       } else {
-        load_rec();
+        if constexpr (GRAD) load_rec();
         nb = nodes + (pbase + b.j) * 4;
         gb = nodes_grad + (pbase + b.j) * 16;
         nstride = (size_t)NNS * 4;
@@ -705,6 +705,9 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
       tdi_consts(det, b, tdc);
       Cx out[3];
       response_row<PREC, KIND, !GRAD>(det, pc, modes, nb, nstride, b, rc, tdc, out);
+      // the forward-only kernel (168 registers) requests the record here instead: held across
+      // its mode loop, the record's registers were spilled (measured 0.9% slower)
+      if constexpr (!GRAD) load_rec();
       Cx wch[3];
 #pragma unroll
       for (int ch = 0; ch < 3; ++ch) {
