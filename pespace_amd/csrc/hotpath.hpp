@@ -576,7 +576,8 @@ __device__ inline void wave_sum12(const double x[12], double z[3]) {
 // A row that the fused kernel evaluates on a tile starting at f0: a mode of the row reaches
 // f0 and the row's parameters are valid (the two uniform skips of tile_rows).
 __device__ inline bool row_live(const double* __restrict__ pconst, int p, double f0) {
-  const double* pc = pconst + (size_t)p * PC_STRIDE;
+  // p is uniform: stated, so that the two loads are scalar (no vector-memory wait per row)
+  const double* pc = pconst + (size_t)__builtin_amdgcn_readfirstlane(p) * PC_STRIDE;
   return !(f0 > pc[PC_FCUT]) && pc[PC_VALID] != 0.0;
 }
 
