@@ -819,7 +819,10 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
   int buf = 0;
   __shared__ int next_work;
   trig_table_load();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the forward-only kernel holds the (uniform) wave index in a scalar register: 1.3% faster
+  // there, 0.2% slower with the gradient
+  const int lane = threadIdx.x & 63;
+  const int wave = GRAD ? (int)(threadIdx.x >> 6) : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int n_chunks = (P + CHUNK - 1) / CHUNK;
   const int n_groups = (n_tiles + tiles_per_group - 1) / tiles_per_group;
   const int n_work = n_chunks * n_groups;
