@@ -638,6 +638,11 @@ extern "C" int pes_context_finalize(pes_context* ctx) {
   if (ctx->finalized) return fail(PES_ERR_STATE, "context already finalized");
   if (ctx->hdets.empty()) return fail(PES_ERR_STATE, "no detector added");
   HIP_TRY(hipSetDevice(ctx->device));
+  long long total = 0;
+  for (auto& h : ctx->hdets) total += (long long)(h.d.n_bins + TILE - 1) / TILE * TILE;
+  // bin indices times the record length (BIN_REC doubles) are formed in 32-bit integers
+  if (total > (long long)(INT32_MAX / BIN_REC))
+    return fail(PES_ERR_INVALID, "too many bins in the network (at most 214,748,364 after padding to tiles)");
   int off = 0;
   for (auto& h : ctx->hdets) {
     h.d.bin_offset = off;
