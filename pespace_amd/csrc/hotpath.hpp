@@ -175,7 +175,7 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
   const double dP = P1 - P0;
   const double Psi = fma(b.hb1, dP, fma(b.hb2, dP0, fma(b.hb3, dP1, P0)));
   g.t = fma(b.he1, dP, fma(b.he2, dP0, fma(b.he3, dP1, rc.tc)));
-  sincos_tab(fma(kOmegaYear, g.t, det.rev0), &g.sa, &g.ca);
+  sincos_tab_s(fma(kOmegaYear, g.t, det.rev0), &g.sa, &g.ca);
   const double sa = g.sa, ca = g.ca;
   g.kx0 = AU_SEC * fma(rc.k.x, ca, rc.k.y * sa);
   double kx_0, kx_1;
@@ -196,7 +196,7 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
     kx_1 = fma(det.hsk[1], kU, det.hck[1] * kV);
   } else {
     double s1, c1;
-    sincos_tab(fma(det.omega_rot, g.t, det.rot0), &s1, &c1);
+    sincos_tab_s(fma(det.omega_rot, g.t, det.rot0), &s1, &c1);
     const double cr = -0.5, sr = 0.5 * SQRT3;  // rotation by 2 pi / 3
     g.al[0] = s1;
     g.be[0] = c1;
@@ -222,8 +222,8 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
   }
   {
     double s0, c0, s1, c1;
-    sincos_tab(b.pif * kx_0, &s0, &c0);
-    sincos_tab(b.pif * kx_1, &s1, &c1);
+    sincos_tab_s(b.pif * kx_0, &s0, &c0);
+    sincos_tab_s(b.pif * kx_1, &s1, &c1);
     g.z0 = {c0, s0};
     g.z1 = {c1, s1};
     g.z01 = mul(g.z0, g.z1);

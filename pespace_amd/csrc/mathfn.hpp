@@ -103,6 +103,14 @@ __device__ inline void sincos_tab(double x, double* s, double* c) {
   sincos_tab_kr(km, r, s, c);
 }
 
+// sincos_tab without the second reduction step: the omitted k (2 pi / 256 - kTabStepHi) is at
+// most 3.9e-17 |x| absolute, for arguments of a few radians below the table's own error
+__device__ inline void sincos_tab_s(double x, double* s, double* c) {
+  const double km = fma(x, kTabScale, kRoundMagic);
+  const double k = km - kRoundMagic;
+  sincos_tab_kr(km, fma(-k, kTabStepHi, x), s, c);
+}
+
 // sin, cos of hi + lo (|lo| <= ulp(hi)/2, e.g. the exact sum of two doubles): the exact first
 // reduction step keeps lo's bits
 __device__ inline void sincos_tab2(double hi, double lo, double* s, double* c) {
