@@ -127,7 +127,7 @@ __device__ inline void tdi_consts(const DetectorDev& det, const BinPre& b, TdiC&
 struct RowC {
   double tc;
   V3 p, q, k;
-  double tc_r;  // 2 pi f t_c - tc_k 2 pi / kTrigN, |tc_r| <= pi / kTrigN (ROW_TC phases)
+  double tc_r;  // 2 pi f t_c - tc_k 2 pi / kTrigN, |tc_r| <= pi / kTrigN
   int tc_k;     // low bits of the integer tc_k
 };
 __device__ inline RowC load_row(const double* __restrict__ pc, const BinPre& b) {
@@ -164,9 +164,7 @@ struct ModeG {
 
 constexpr double kOmegaYear = 2.0 * PI / YEAR_SI;
 
-// ROW_TC (FP64 path): take 2 pi f t_c's reduction from the row constants (computed once per
-// bin and row) instead of forming the exact total phase per mode
-template <int PREC, int KIND, bool ROW_TC = false>
+template <int PREC, int KIND>
 __device__ inline void mode_geom(const DetectorDev& det, const double* __restrict__ mf,
                                  const double* __restrict__ n0, const BinPre& b, const RowC& rc, ModeG& g) {
   const double A0 = n0[0], dA0 = n0[1], P0 = n0[2], dP0 = n0[3];
@@ -233,18 +231,12 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
     // unit modulus matters: a mean modulus error of the FP32 pair would scale every bin's
     // |h|^2 the same way; the remaining error is a phase error that changes from bin to bin
     sincos_rev_fc(fma(2.0 * b.pif, rc.tc + g.kx0, Psi), &sp, &cp);
-  } else if constexpr (ROW_TC) {
-    // 2 pi f t_c (rounded once, as the reference's own factor, and reduced once per row) plus
-    // Psi + 2 pi f k.x0, each reduced on its own and the remainders added: the total is never
-    // rounded, so the phase's rounding error does not move with Psi or k.x0, and lnL stays as
-    // smooth in the parameters as with three separate factors
-    sincos_tab_sum(rc.tc_r, rc.tc_k, fma(2.0 * b.pif, g.kx0, Psi), &sp, &cp);
   } else {
-    // the same sum formed exactly (TwoSum) and reduced per mode: fewer values live across the
-    // mode loops, which the gradient kernel's register budget prefers
-    double hi, lo;
-    two_sum(2.0 * b.pif * rc.tc, fma(2.0 * b.pif, g.kx0, Psi), hi, lo);
-    sincos_tab2(hi, lo, &sp, &cp);
+    // 2 pi f t_c (rounded once, as the reference's own factor, and reduced once per bin and
+    // row) plus Psi + 2 pi f k.x0, each reduced on its own and the remainders added: the total
+    // is never rounded, so the phase's rounding error does not move with Psi or k.x0, and lnL
+    // stays as smooth in the parameters as with three separate factors
+    sincos_tab_sum(rc.tc_r, rc.tc_k, fma(2.0 * b.pif, g.kx0, Psi), &sp, &cp);
   }
   g.u = {cp, -sp};
   g.hm = scale(g.u, g.A);
@@ -350,7 +342,7 @@ __device__ inline void tdi_out(const DetectorDev& det, const TdiC& t, const Link
   }
 }
 
-template <int PREC, int KIND, bool ROW_TC = false>
+template <int PREC, int KIND>
 // nb: node entries (j, j+1) of this bin for mode 0; mode mi's are at nb + mi * mstride
 __device__ inline void response_row(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
                                     const double* __restrict__ nb, size_t mstride, const BinPre& b,
@@ -362,7 +354,7 @@ __device__ inline void response_row(const DetectorDev& det, const double* __rest
     const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
     if (b.f > mf[MF_CUT]) continue;
     ModeG g;
-    mode_geom<PREC, KIND, ROW_TC>(det, mf, nb + mi * mstride, b, rc, g);
+    mode_geom<PREC, KIND>(det, mf, nb + mi * mstride, b, rc, g);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       PairV v;
@@ -705,7 +697,7 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
       TdiC tdc;
       tdi_consts(det, b, tdc);
       Cx out[3];
-      response_row<PREC, KIND, !GRAD>(det, pc, modes, nb, nstride, b, rc, tdc, out);
+      response_row<PREC, KIND>(det, pc, modes, nb, nstride, b, rc, tdc, out);
       // the forward-only kernel (168 registers) requests the record here instead: held across
       // its mode loop, the record's registers were spilled (measured 0.9% slower)
       if constexpr (!GRAD) load_rec();

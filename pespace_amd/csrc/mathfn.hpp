@@ -12,7 +12,8 @@
 // k mod 256 without a conversion), sin and cos of the remainder |r| <= pi/256 from short Taylor
 // polynomials (truncation < 5e-18), and the angle-addition formulas with sin, cos of k h read
 // from a 256-entry table of correctly rounded values held in LDS (4 KiB): about 18 VALU
-// instructions and one LDS read instead of ~37, error within ~2 ulp.
+// instructions and one LDS read instead of ~37, error within ~2 ulp.  sincos_tab_s skips the
+// second reduction step (small arguments); sincos_tab_sum takes a + x with a already reduced.
 // rcp_d: hardware reciprocal estimate refined by two Newton steps (correctly rounded to
 // about one ulp).
 #pragma once
@@ -111,16 +112,6 @@ __device__ inline void sincos_tab_s(double x, double* s, double* c) {
   sincos_tab_kr(km, fma(-k, kTabStepHi, x), s, c);
 }
 
-// sin, cos of hi + lo (|lo| <= ulp(hi)/2, e.g. the exact sum of two doubles): the exact first
-// reduction step keeps lo's bits
-__device__ inline void sincos_tab2(double hi, double lo, double* s, double* c) {
-  const double km = fma(hi, kTabScale, kRoundMagic);
-  const double k = km - kRoundMagic;
-  double r = fma(-k, kTabStepHi, hi);
-  r = fma(-k, kTabStepLo, r) + lo;
-  sincos_tab_kr(km, r, s, c);
-}
-
 // x = k 2 pi / kTrigN + r with k the nearest integer to x kTrigN / (2 pi) (|x| < 5e13): r and
 // the low 32 bits of k
 __device__ inline void tab_reduce(double x, double& r, int& k_lo) {
@@ -146,12 +137,6 @@ __device__ inline void sincos_tab_sum(double ra, int ka_lo, double x, double* s,
   *c = fma(t.y, cr, -t.x * sr);
 }
 
-// a + b = s + e exactly (Knuth's TwoSum, no ordering of |a|, |b| needed)
-__device__ inline void two_sum(double a, double b, double& s, double& e) {
-  s = a + b;
-  const double bb = s - a;
-  e = (a - (s - bb)) + (b - bb);
-}
 
 __device__ inline double rcp_d(double x) {
   double r = __builtin_amdgcn_rcp(x);
