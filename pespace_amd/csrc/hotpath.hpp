@@ -4,9 +4,9 @@
 // cosine of the total phase come from (PREC template parameter, PES_PREC_* values):
 //   0 FP64: the phasor of the total phase Psi + 2 pi f (t_c + k.x0) (the
 //     waveform phase, the t_c shift and the barycentric delay, antenna.py:435, 530-533, which
-//     the reference applies as three factors) from one sincos_tab of the exact two-double sum
-//     of 2 pi f t_c and Psi + 2 pi f k.x0: each term keeps the rounding it has as a separate
-//     factor.
+//     the reference applies as three factors) from one table sincos of 2 pi f t_c (reduced
+//     once per bin and row) plus Psi + 2 pi f k.x0 (reduced per mode), the remainders added:
+//     the total is never rounded and each term keeps the rounding it has as a separate factor.
 //   1 MIXED: the phasor of the total phase Psi + 2 pi f (t_c + k.x0) (summed and reduced mod
 //     2 pi in FP64) comes from the FP32 hardware sine/cosine (v_sin_f32, v_cos_f32).  Its
 //     error changes from bin to bin (the phase moves by ~2 pi t df per bin), so it averages
@@ -34,7 +34,10 @@
 //     per bin, inside the TDI prefactor;
 //   * n_a.v = (xd_i.v - xd_j.v)/L for v in {p, q, k} from six scalar products per mode (the
 //     node offsets are two fixed vectors weighted by equally spaced phases, see ModeG);
-//   * n.e+.n = (n.p)^2 - (n.q)^2, n.ex.n = 2 (n.p)(n.q) with e+ = pp - qq, ex = pq + qp.
+//   * n.e+.n = (n.p)^2 - (n.q)^2, n.ex.n = 2 (n.p)(n.q) with e+ = pp - qq, ex = pq + qp (the
+//     factor 2 is carried by the stored cross factor);
+//   * T = (X + Y + Z)/sqrt3 from the differences of the two links of each pair, and its
+//     adjoint folded into the link adjoints.
 // This header is included inside the anonymous namespace of pespace_amd.hip.
 
 struct Cx {
