@@ -611,7 +611,7 @@ __device__ inline void stage_nodes_dma(const double* __restrict__ nodes, const d
 
 // The rows p0 .. p1-1 of one tile (lane <-> bin) on a detector of orbit kind KIND: lnL and,
 // with GRAD, the 11 gradient terms of each row summed over the tile into acc
-// [TILE / 64][CHUNK][NOUT].  Instantiated per orbit kind so that only that kind's detector
+// [TILE / 64][chunk_rows<GRAD>()][NOUT].  Instantiated per orbit kind so that only that kind's detector
 // constants are held in scalar registers.
 //
 // STAGED: the tile's bins use the nj <= NJ_STAGE consecutive node rows j0 .. j0+nj-1 (bins are
@@ -780,17 +780,18 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
       if ((lane & 15) == 0) {
         const int q = lane >> 4;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) acc[(wave * CHUNK + p - p0) * NOUT + 3 * q + i] += z[i];
+        for (int i = 0; i < 3; ++i) acc[(wave * chunk_rows<GRAD>() + p - p0) * NOUT + 3 * q + i] += z[i];
       }
     } else {
       v = wave_sum(v);
-      if (lane == 0) acc[(wave * CHUNK + p - p0) * NOUT] += v;
+      if (lane == 0) acc[(wave * chunk_rows<GRAD>() + p - p0) * NOUT] += v;
     }
   }
 }
 
 // ------------------------------- kernels ---------------------------------------------------
-// Persistent fused kernel.  Work item = (group of tiles_per_group tiles, chunk of CHUNK rows);
+// Persistent fused kernel.  Work item = (group of tiles_per_group tiles, chunk of
+// chunk_rows<GRAD>() rows);
 // lane <-> bin.  A tile whose first frequency lies above every mode's cut of a row has a
 // zero response for that row: its lnL contribution is the precomputed -sum w |d|^2 of the
 // tile and its gradient contribution is zero, so the row skips it.
@@ -808,7 +809,8 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
                                               ModeList modes, int P, const int* __restrict__ tile_jr,
                                               int* __restrict__ work_counter, double* __restrict__ partial) {
   constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
-  __shared__ double acc[TILE / 64][CHUNK][NOUT];
+  constexpr int CH = chunk_rows<GRAD>();  // 64 rows without the gradient: 0.7% faster
+  __shared__ double acc[TILE / 64][CH][NOUT];
   __shared__ double stage_n[2 * MAX_MODES * NJ_STAGE * 4];
   __shared__ typename std::conditional<GRAD, double[2 * MAX_MODES * NJ_STAGE * 16], double[1]>::type stage_g;
   int buf = 0;
@@ -818,7 +820,7 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
   // there, 0.2% slower with the gradient
   const int lane = threadIdx.x & 63;
   const int wave = GRAD ? (int)(threadIdx.x >> 6) : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int n_chunks = (P + CHUNK - 1) / CHUNK;
+  const int n_chunks = (P + CH - 1) / CH;
   const int n_groups = (n_tiles + tiles_per_group - 1) / tiles_per_group;
   const int n_work = n_chunks * n_groups;
   // work items are taken from a counter (zeroed by k_setup earlier on the same stream), so
@@ -830,8 +832,8 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
     const int w = next_work;
     if (w >= n_work) break;
     const int g = w / n_chunks, c = w % n_chunks;
-    const int p0 = c * CHUNK, p1 = min(P, p0 + CHUNK);
-    for (int i = threadIdx.x; i < (TILE / 64) * CHUNK * NOUT; i += TILE) (&acc[0][0][0])[i] = 0.0;
+    const int p0 = c * CH, p1 = min(P, p0 + CH);
+    for (int i = threadIdx.x; i < (TILE / 64) * CH * NOUT; i += TILE) (&acc[0][0][0])[i] = 0.0;
     double chunk_cut = 0.0;
     for (int p = p0; p < p1; ++p) chunk_cut = fmax(chunk_cut, pconst[(size_t)p * PC_STRIDE + PC_FCUT]);
     __syncthreads();

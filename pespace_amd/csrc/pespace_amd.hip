@@ -47,7 +47,9 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int TILE = 256;      // bins per tile = threads per block
-constexpr int CHUNK = 32;      // parameter rows per work item
+constexpr int CHUNK = 32;      // parameter rows per work item of the gradient kernel ...
+template <bool GRAD>
+constexpr int chunk_rows() { return GRAD ? CHUNK : 2 * CHUNK; }  // ... and of the lnL-only one
 constexpr int TILES_PER_GROUP = 32;  // default schedule: tiles per work item ...
 constexpr int BLOCKS_PER_CU = 4;     // ... and persistent blocks launched per CU
 constexpr int MAX_MODES = 6;         // modes per context (ModeList)
@@ -837,7 +839,8 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
   if (rc != PES_OK || P == 0) return rc;
   const int NNS = ctx->n_nodes + 1;
   const int n_groups = n_groups_of(ctx);
-  const int n_chunks = (P + CHUNK - 1) / CHUNK;
+  const int n_chunks = grad ? (P + chunk_rows<true>() - 1) / chunk_rows<true>()
+                            : (P + chunk_rows<false>() - 1) / chunk_rows<false>();
   const int n_work = n_groups * n_chunks;
   int grid = ctx->n_cu * ctx->blocks_per_cu;
   if (grid > n_work) grid = n_work;
