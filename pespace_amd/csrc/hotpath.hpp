@@ -11,7 +11,7 @@
 //     2 pi in FP64) comes from the FP32 hardware sine/cosine (v_sin_f32, v_cos_f32).  Its
 //     error changes from bin to bin (the phase moves by ~2 pi t df per bin), so it averages
 //     out of lnL.
-// Every other sine and cosine (orbit angles, pair phasors) comes from sincos_tab and every
+// Every other sine and cosine (orbit angles, pair phasors) comes from the table sincos and every
 // reciprocal from rcp_d in both modes.  FP32 there does not pay: the orbit angle's error is
 // multiplied by 2 pi f AU/c (~150 rad) in the delay phase, and the transfer-function
 // arguments vary slowly with f, so FP32 errors there are coherent over many bins and bias

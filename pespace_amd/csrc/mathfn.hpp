@@ -7,13 +7,14 @@
 // no argument size needs a fallback (the error of r is ~ulp(r) + |k| 1.5e-33).  Then
 // Taylor polynomials on [-pi/4, pi/4] (coefficients 1/n!, truncation < 1e-19) and the
 // quadrant k mod 4.
-// sincos_tab: the hot-loop variant.  Reduction by h = 2 pi / 256 in the same FMA form (the
+// Table sincos, the hot-loop variants: reduction by h = 2 pi / 256 in the same FMA form (the
 // first FMA is again exact; k is rounded with the 1.5 * 2^52 trick, whose encoding also gives
 // k mod 256 without a conversion), sin and cos of the remainder |r| <= pi/256 from short Taylor
 // polynomials (truncation < 5e-18), and the angle-addition formulas with sin, cos of k h read
-// from a 256-entry table of correctly rounded values held in LDS (4 KiB): about 18 VALU
-// instructions and one LDS read instead of ~37, error within ~2 ulp.  sincos_tab_s skips the
-// second reduction step (small arguments); sincos_tab_sum takes a + x with a already reduced.
+// from a 256-entry table of correctly rounded values held in LDS (4 KiB): about 16 VALU
+// instructions and one LDS read instead of ~37, error within ~2 ulp.  sincos_tab_s (arguments
+// of a few radians) takes one reduction step; sincos_tab_sum takes a + x with a already reduced
+// (tab_reduce, both steps) and reduces x with both steps.
 // rcp_d: hardware reciprocal estimate refined by two Newton steps (correctly rounded to
 // about one ulp).
 #pragma once
@@ -23,7 +24,7 @@
 
 namespace pes {
 
-// sin, cos of 2 pi i / kTrigN in LDS; every kernel that calls sincos_tab fills it first with
+// sin, cos of 2 pi i / kTrigN in LDS; every kernel that uses the table sincos fills it first with
 // trig_table_load()
 __shared__ double2 g_trig[kTrigN];
 
@@ -84,7 +85,7 @@ constexpr double kTabStepLo = 9.567553118338697e-19; // 2 pi / 256 - kTabStepHi,
 // 32 bits of the result's encoding hold n mod 2^32 (two's complement)
 constexpr double kRoundMagic = 6755399441055744.0;
 
-// table part of sincos_tab: km = k + kRoundMagic, r = x - k 2 pi / 256 already formed
+// table part of the table sincos: km = k + kRoundMagic, r = x - k 2 pi / 256 already formed
 __device__ inline void sincos_tab_kr(double km, double r, double* s, double* c) {
   const int idx = __double2loint(km) & (kTrigN - 1);  // k mod 256
   const double2 t = g_trig[idx];
@@ -95,17 +96,8 @@ __device__ inline void sincos_tab_kr(double km, double r, double* s, double* c) 
   *c = fma(t.y, cr, -t.x * sr);
 }
 
-// any |x| < 5e13 (k = nearest integer to x 256 / (2 pi), |k| < 2^51)
-__device__ inline void sincos_tab(double x, double* s, double* c) {
-  const double km = fma(x, kTabScale, kRoundMagic);
-  const double k = km - kRoundMagic;
-  double r = fma(-k, kTabStepHi, x);
-  r = fma(-k, kTabStepLo, r);
-  sincos_tab_kr(km, r, s, c);
-}
-
-// sincos_tab without the second reduction step: the omitted k (2 pi / 256 - kTabStepHi) is at
-// most 3.9e-17 |x| absolute, for arguments of a few radians below the table's own error
+// one reduction step: the omitted k (2 pi / 256 - kTabStepHi) is at most 3.9e-17 |x|
+// absolute, for arguments of a few radians below the table's own error
 __device__ inline void sincos_tab_s(double x, double* s, double* c) {
   const double km = fma(x, kTabScale, kRoundMagic);
   const double k = km - kRoundMagic;
