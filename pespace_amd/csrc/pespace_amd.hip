@@ -47,13 +47,13 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int TILE = 256;      // bins per tile = threads per block
-constexpr int CHUNK = 32;      // parameter rows per work item of the gradient kernel ...
+constexpr int CHUNK = 40;      // parameter rows per work item of the gradient kernel ...
 template <bool GRAD>
 constexpr int chunk_rows() { return GRAD ? CHUNK : 2 * CHUNK; }  // ... and of the lnL-only one
 constexpr int TILES_PER_GROUP = 32;  // default schedule: tiles per work item ...
 constexpr int BLOCKS_PER_CU = 4;     // ... and persistent blocks launched per CU
 constexpr int MAX_MODES = 6;         // modes per context (ModeList)
-constexpr int NJ_STAGE = 32;         // node rows per tile that the fused kernel stages in LDS
+constexpr int NJ_STAGE = 30;         // node rows per tile that the fused kernel stages in LDS
 constexpr int BIN_REC = 10;          // doubles per bin of the fused kernel's weight and data record
 
 // ---- per-row constant block ------------------------------------------------------------
