@@ -130,7 +130,7 @@ class LikelihoodCore:
 
     # -- schedule of the fused kernel ------------------------------------------------------
     def set_schedule(self, tiles_per_group, blocks_per_cu):
-        """Work items of ``tiles_per_group`` 256-bin tiles (x 40 rows, 80 without gradient),
+        """Work items of ``tiles_per_group`` 256-bin tiles (x 40 rows, 160 without gradient),
         ``blocks_per_cu`` persistent blocks per CU."""
         self._build()
         check(lib().pes_context_set_schedule(self._ctx, int(tiles_per_group), int(blocks_per_cu)))

@@ -809,7 +809,7 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
                                               ModeList modes, int P, const int* __restrict__ tile_jr,
                                               int* __restrict__ work_counter, double* __restrict__ partial) {
   constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
-  constexpr int CH = chunk_rows<GRAD>();  // twice the rows without the gradient: 0.7% faster
+  constexpr int CH = chunk_rows<GRAD>();  // 4x the rows without the gradient: 1.3% faster
   __shared__ double acc[TILE / 64][CH][NOUT];
   __shared__ double stage_n[2 * MAX_MODES * NJ_STAGE * 4];
   __shared__ typename std::conditional<GRAD, double[2 * MAX_MODES * NJ_STAGE * 16], double[1]>::type stage_g;

@@ -49,7 +49,7 @@ int fail(int code, const std::string& msg) {
 constexpr int TILE = 256;      // bins per tile = threads per block
 constexpr int CHUNK = 40;      // parameter rows per work item of the gradient kernel ...
 template <bool GRAD>
-constexpr int chunk_rows() { return GRAD ? CHUNK : 2 * CHUNK; }  // ... and of the lnL-only one
+constexpr int chunk_rows() { return GRAD ? CHUNK : 4 * CHUNK; }  // ... and of the lnL-only one
 constexpr int TILES_PER_GROUP = 32;  // default schedule: tiles per work item ...
 constexpr int BLOCKS_PER_CU = 4;     // ... and persistent blocks launched per CU
 constexpr int MAX_MODES = 6;         // modes per context (ModeList)

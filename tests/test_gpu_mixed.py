@@ -55,7 +55,7 @@ def test_mixed_accuracy(modes, gen):
 
 @pytest.mark.parametrize("P", [1, 257, 1001])
 def test_uneven_batches_wide_prior(P):
-    """Batches of uneven size (not multiples of the 40- or 80-row chunks) drawn from the wide prior:
+    """Batches of uneven size (not multiples of the 40- or 160-row chunks) drawn from the wide prior:
     the mixed path against the FP64 path, and the batch against its rows evaluated alone."""
     modes = oph.MODES
     d, labels = make_detector(modes=modes)
