@@ -806,7 +806,7 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
                                               int tiles_per_group, const double* __restrict__ pconst,
                                               const double* __restrict__ nodes,
                                               const double* __restrict__ nodes_grad, int NNS, double du,
-                                              ModeList modes, int P, const int* __restrict__ tile_jr,
+                                              ModeList modes, int P, int rows, const int* __restrict__ tile_jr,
                                               int* __restrict__ work_counter, double* __restrict__ partial) {
   constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
   constexpr int CH = chunk_rows<GRAD>();  // 4x the rows without the gradient: 1.3% faster
@@ -820,7 +820,8 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
   // there, 0.2% slower with the gradient
   const int lane = threadIdx.x & 63;
   const int wave = GRAD ? (int)(threadIdx.x >> 6) : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int n_chunks = (P + CH - 1) / CH;
+  // rows (1 <= rows <= CH) per work item, chosen by the host
+  const int n_chunks = (P + rows - 1) / rows;
   const int n_groups = (n_tiles + tiles_per_group - 1) / tiles_per_group;
   const int n_work = n_chunks * n_groups;
   // work items are taken from a counter (zeroed by k_setup earlier on the same stream), so
@@ -832,7 +833,7 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
     const int w = next_work;
     if (w >= n_work) break;
     const int g = w / n_chunks, c = w % n_chunks;
-    const int p0 = c * CH, p1 = min(P, p0 + CH);
+    const int p0 = c * rows, p1 = min(P, p0 + rows);
     for (int i = threadIdx.x; i < (TILE / 64) * CH * NOUT; i += TILE) (&acc[0][0][0])[i] = 0.0;
     double chunk_cut = 0.0;
     for (int p = p0; p < p1; ++p) chunk_cut = fmax(chunk_cut, pconst[(size_t)p * PC_STRIDE + PC_FCUT]);

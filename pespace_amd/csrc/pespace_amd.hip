@@ -11,6 +11,7 @@
 //   k_reduce    fixed-order sum of the per-(bin group, row) partials
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -444,11 +445,11 @@ __global__ void k_whittle_final(const double* __restrict__ work, int nb, double 
   const DetectorDev *dets, int n_det, int n_tiles, const double *bin_f, const int *bin_j, const double *bin_s, \
       const double *bin_rec, const double *tile_f0,                                                       \
       const double *tile_dd, int tiles_per_group, const double *pconst, const double *nodes,                 \
-      const double *nodes_grad, int NNS, double du, ModeList modes, int P, const int *tile_jr,               \
+      const double *nodes_grad, int NNS, double du, ModeList modes, int P, int rows, const int *tile_jr,     \
       int *work_counter, double *partial
 #define PES_LNL_NAMES                                                                                     \
   dets, n_det, n_tiles, bin_f, bin_j, bin_s, bin_rec, tile_f0, tile_dd, tiles_per_group, pconst, \
-      nodes, nodes_grad, NNS, du, modes, P, tile_jr, work_counter, partial
+      nodes, nodes_grad, NNS, du, modes, P, rows, tile_jr, work_counter, partial
 
 }  // namespace
 
@@ -839,10 +840,14 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
   if (rc != PES_OK || P == 0) return rc;
   const int NNS = ctx->n_nodes + 1;
   const int n_groups = n_groups_of(ctx);
-  const int n_chunks = grad ? (P + chunk_rows<true>() - 1) / chunk_rows<true>()
-                            : (P + chunk_rows<false>() - 1) / chunk_rows<false>();
-  const int n_work = n_groups * n_chunks;
+  // rows per work item: the kernel's chunk, fewer when the grid has too few tile groups to give
+  // every persistent block about four work items (small grids, e.g. 16k bins)
   int grid = ctx->n_cu * ctx->blocks_per_cu;
+  const int max_rows = grad ? chunk_rows<true>() : chunk_rows<false>();
+  const long long fit = (long long)P * n_groups / (4LL * grid);
+  const int rows = (int)std::max(8LL, std::min((long long)max_rows, fit));
+  const int n_chunks = (P + rows - 1) / rows;
+  const int n_work = n_groups * n_chunks;
   if (grid > n_work) grid = n_work;
   int nout = 1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -856,7 +861,7 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
 #define PES_LNL_ARGS                                                                                         \
   ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_rec,                 \
       ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, ctx->d_pconst, ctx->d_nodes,   \
-      ctx->d_nodes_grad, NNS, ctx->du, ctx->modes, P, ctx->d_tile_jr, ctx->d_work_counter, ctx->d_partial
+      ctx->d_nodes_grad, NNS, ctx->du, ctx->modes, P, rows, ctx->d_tile_jr, ctx->d_work_counter, ctx->d_partial
   if (!grad)
     HIP_TRY(launch_lnl_fwd(precision, grid, st, PES_LNL_ARGS));
   else if (precision == 0)
