@@ -63,19 +63,25 @@ int pes_context_create(pes_context** out, int device, int n_modes, const int* mo
 /* One detector: orbit, TDI combination, grid, data and PSD (host arrays, copied).
  * Replaces InterferometerAntenna + TDIChannelData setup (antenna.py:128-202,
  * tdi.py:585-649, 771-837) and the static data of likelihood_jax.py:132-205.
- * orbit_kind: 0 heliocentric (orbit.py:155-241), 1 geocentric (orbit.py:67-152,
- *   TianQin defaults for the rotation and reference source).
+ * orbit_kind: 0 heliocentric (KaplerianHeliocentric, orbit.py:155-241), 1 geocentric
+ *   (KeplerianGeocentric, orbit.py:67-152).  arm_length_m, rotation_initial,
+ *   revolution_initial: both kinds; omega_rotation [rad/s], lambda_ref, beta_ref [rad]: the
+ *   geocentric rotation rate and reference source (orbit.py:77-85), ignored for kind 0.
  * tdi_generation: 15 or 20 (tdi.py:1614-1628); orthogonal: 1 A,E,T / 0 X,Y,Z.
- * f[n_bins]: bin frequencies.  weight[n_bins]: per-bin frequency weight df_k (may
- * be NULL: then the scalar df is used for every bin, tdi.py:130).
+ * f[n_bins]: bin frequencies, strictly increasing, any range (the transfer functions take
+ * every pi f L).  weight[n_bins]: per-bin frequency weight df_k (may be NULL: then the scalar
+ * df is used for every bin, tdi.py:130).
  * data_re/data_im/psd: [3][n_bins] channel-major; channel_mask[3]: which of the
  * three outputs enter lnL (likelihood_ti.py:61, interface_bilby.py:70-75). */
 int pes_context_add_detector(pes_context* ctx, int orbit_kind, double arm_length_m, double rotation_initial,
-                             double revolution_initial, int tdi_generation, int orthogonal, int n_bins,
-                             const double* f, const double* weight, double df, const double* data_re,
-                             const double* data_im, const double* psd, const int* channel_mask);
+                             double revolution_initial, double omega_rotation, double lambda_ref, double beta_ref,
+                             int tdi_generation, int orthogonal, int n_bins, const double* f, const double* weight,
+                             double df, const double* data_re, const double* data_im, const double* psd,
+                             const int* channel_mask);
 
-/* Upload detector data; must be called once after the last add_detector. */
+/* Upload detector data; must be called once after the last add_detector.  With
+ * interpolation nodes (n_nodes > 1) every bin of every detector must lie inside
+ * [f_node_min, f_node_max] (no extrapolation): otherwise PES_ERR_INVALID. */
 int pes_context_finalize(pes_context* ctx);
 
 void pes_context_destroy(pes_context* ctx);
@@ -110,6 +116,43 @@ int pes_waveform(pes_context* ctx, int det, const double* d_params, int P, doubl
 int pes_response_from_waveform(pes_context* ctx, int det, int n_modes, const double* d_wf, double lam, double beta,
                                double psi, double tc, double* d_out, void* stream);
 
+/* The six single links of P rows on detector det, summed over the modes: device out
+ * [P][6][n_bins][2] (re, im) in the order link12, link21, link23, link32, link31, link13,
+ * from the same device code as pes_lnl_batch (FP64).  Replaces the single-link stage
+ * _FDResponseModelMarset2018Taichi.update_single_link_response with a hip waveform
+ * (antenna.py:403-460, 496-543; the links field of antenna.py:399-401). */
+int pes_links(pes_context* ctx, int det, const double* d_params, int P, double* d_out, void* stream);
+
+/* The six single links of detector det for a caller-supplied waveform container (one
+ * parameter set): d_wf as for pes_response_from_waveform, device out [6][n_bins][2].
+ * Replaces update_single_link_response (antenna.py:403-460) and the JAX twin's
+ * get_response_jit (_antenna_jax.py:82-195). */
+int pes_links_from_waveform(pes_context* ctx, int det, int n_modes, const double* d_wf, double lam, double beta,
+                            double psi, double tc, double* d_out, void* stream);
+
+/* TDI combination of detector det from single links: device links [P][6][n_bins][2] ->
+ * device out [P][3][n_bins][2] (A, E, T = (X+Y+Z)/sqrt3 or X, Y, Z, the reference's own
+ * formulas).  Replaces update_tdi_response (tdi.py:1705-1793) and the JAX twin's
+ * get_combination_jit (_tdi_jax.py:70-102). */
+int pes_tdi_from_links(pes_context* ctx, int det, int P, const double* d_links, double* d_out, void* stream);
+
+/* Constellation vectors at n times (device d_t[n]): device out [n][6][3] = n1, n2, n3 (unit
+ * link vectors) and x1, x2, x3 (node positions, SSB frame, seconds) of an orbit given by the
+ * same parameters as pes_context_add_detector.  Replaces get_constellation_vectors
+ * (orbit.py:260-322 geocentric, 363-419 heliocentric; _orbit_jax.py). */
+int pes_orbit_vectors(int orbit_kind, double arm_length_m, double rotation_initial, double revolution_initial,
+                      double omega_rotation, double lambda_ref, double beta_ref, const double* d_t, int n,
+                      double* d_out, void* stream);
+
+/* Scratch for HIP-graph capture.  pes_context_reserve sizes the context's scratch for batches
+ * of up to P rows (with the gradient tables if grad != 0) under the current schedule.  While
+ * pes_context_pin_scratch(ctx, 1) holds, the scratch is never reallocated: a call that would
+ * need more (a larger batch, a first gradient call, a schedule with more tile groups) fails
+ * with PES_ERR_STATE instead of freeing buffers that a captured graph still uses; unpin with
+ * pes_context_pin_scratch(ctx, 0).  No counterpart in the reference (it has no graphs). */
+int pes_context_reserve(pes_context* ctx, int P, int grad);
+int pes_context_pin_scratch(pes_context* ctx, int pinned);
+
 /* Standalone Whittle lnL = -2 df sum_c sum_k |d - h|^2 / S on device arrays
  * (d_data, d_resp: [n_chan][n][2]; d_psd: [n_chan][n]); the result is written to
  * device d_out[0].  d_work: caller-owned device workspace of at least
@@ -124,7 +167,8 @@ int pes_whittle_lnl(const double* d_data, const double* d_resp, const double* d_
  * lnL without gradient), taken from a queue by blocks_per_cu x (number of CUs) persistent
  * blocks of 256 threads.  Defaults
  * 32 and 4; pespace_amd.core.LikelihoodCore.autotune picks them by timing.  Changing the
- * schedule may reallocate scratch at the next call: set it before capturing a HIP graph.
+ * schedule, a larger batch or a first gradient call may reallocate scratch at the next call:
+ * reserve and pin the scratch (below) before capturing a HIP graph.
  * No counterpart in the reference. */
 int pes_context_set_schedule(pes_context* ctx, int tiles_per_group, int blocks_per_cu);
 int pes_context_get_schedule(const pes_context* ctx, int* tiles_per_group, int* blocks_per_cu);

@@ -128,11 +128,13 @@ GEO_LAMBDA_REF = 120.5 / 180 * PI  # orbit.py:83
 GEO_BETA_REF = -4.7 / 180 * PI  # orbit.py:84
 
 
-def constellation(kind, arm_length, rot0, rev0, t):
+def constellation(kind, arm_length, rot0, rev0, t, omega_rotation=GEO_OMEGA_ROT, lambda_ref=GEO_LAMBDA_REF,
+                  beta_ref=GEO_BETA_REF):
     """Return (n1, n2, n3, x1, x2, x3), each (3, ...) arrays, at times t.
 
     helio: orbit.py:189-228 (params), 381-419 (vectors)
-    geo:   orbit.py:128-139 (params), 278-322 (vectors)
+    geo:   orbit.py:128-139 (params), 278-322 (vectors); omega_rotation, lambda_ref and
+           beta_ref are the constructor's arguments (orbit.py:77-85, TianQin defaults)
     """
     t = np.asarray(t, dtype=np.float64)
     L = arm_length / C_SI
@@ -153,10 +155,10 @@ def constellation(kind, arm_length, rot0, rev0, t):
         x0 = np.array([ca, sa, np.zeros_like(ca)]) * AU_SEC
     elif kind == "geo":
         r = L / np.sqrt(3)
-        sl, cl = np.sin(GEO_LAMBDA_REF), np.cos(GEO_LAMBDA_REF)
-        sb, cb = np.sin(GEO_BETA_REF), np.cos(GEO_BETA_REF)
+        sl, cl = np.sin(lambda_ref), np.cos(lambda_ref)
+        sb, cb = np.sin(beta_ref), np.cos(beta_ref)
         alpha = (2 * PI / YEAR_SI) * t + rev0
-        k1 = GEO_OMEGA_ROT * t + rot0
+        k1 = omega_rotation * t + rot0
         kap = [k1, k1 + 2 * PI / 3, k1 + 2 * PI / 3 + 2 * PI / 3]
         xd = []
         for k in kap:
@@ -200,10 +202,12 @@ def single_links(waveform, lam, beta, psi, tc, f, orbit):
     """Marsat-Baker FD single-link response, summed over modes.
 
     waveform: {"plus","cross","tf"} or a mode-keyed dict of those (antenna.py:438-458)
-    orbit:    (kind, arm_length[m], rot0, rev0)
+    orbit:    (kind, arm_length[m], rot0, rev0) or, geocentric, (kind, arm_length, rot0, rev0,
+              omega_rotation, lambda_ref, beta_ref)
     Returns dict link -> complex[N].  antenna.py:429-460 (driver), 496-543 (links).
     """
-    kind, arm, rot0, rev0 = orbit
+    kind, arm, rot0, rev0 = orbit[:4]
+    extra = tuple(orbit[4:])
     eplus, ecross = polarization_tensors(lam, beta, psi)
     _, _, k = polarization_basis(lam, beta, psi)
     f = np.asarray(f, dtype=np.float64)
@@ -216,7 +220,7 @@ def single_links(waveform, lam, beta, psi, tc, f, orbit):
         hp = wf["plus"] * cexp_tshift  # antenna.py:447-449
         hc = wf["cross"] * cexp_tshift
         tf = wf["tf"] + tc
-        n1, n2, n3, x1, x2, x3 = constellation(kind, arm, rot0, rev0, tf)  # :496
+        n1, n2, n3, x1, x2, x3 = constellation(kind, arm, rot0, rev0, tf, *extra)  # :496
 
         def nhn(n):  # antenna.py:499-512
             return (np.einsum("in,ij,jn->n", n, eplus, n) * hp

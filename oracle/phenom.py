@@ -157,9 +157,14 @@ def fit(name, eta, chi):
             + xi * xi * (c[5] + c[6] * eta + c[7] * eta2) + xi * xi * xi * (c[8] + c[9] * eta + c[10] * eta2))
 
 
-def final_spin(eta, chi1, chi2):
+def _delta(eta, delta):
+    """delta = (m1 - m2) / M = sqrt(1 - 4 eta); callers with the masses pass (m1 - m2) / M."""
+    return np.sqrt(1.0 - 4.0 * eta) if delta is None else delta
+
+
+def final_spin(eta, chi1, chi2, delta=None):
     """Husa et al. 2016 eq. 3.6 (FinalSpin0815), m1 >= m2, chi1 on m1."""
-    d = np.sqrt(1.0 - 4.0 * eta)
+    d = _delta(eta, delta)
     m1, m2 = 0.5 * (1.0 + d), 0.5 * (1.0 - d)
     s = m1 * m1 * chi1 + m2 * m2 * chi2
     e2, e3 = eta * eta, eta * eta * eta
@@ -172,9 +177,9 @@ def final_spin(eta, chi1, chi2):
                          + (-0.8676969352555539 + 2.064046835273906 * eta) * s3))
 
 
-def radiated_energy(eta, chi1, chi2):
+def radiated_energy(eta, chi1, chi2, delta=None):
     """Husa et al. 2016 eq. 3.8 (EradRational0815)."""
-    d = np.sqrt(1.0 - 4.0 * eta)
+    d = _delta(eta, delta)
     m1, m2 = 0.5 * (1.0 + d), 0.5 * (1.0 - d)
     s = (m1 * m1 * chi1 + m2 * m2 * chi2) / (m1 * m1 + m2 * m2)
     e2, e3, e4 = eta ** 2, eta ** 3, eta ** 4
@@ -192,7 +197,7 @@ def qnm_freqs(mode, af, erad):
     return fr / (1.0 - erad), fd / (1.0 - erad)
 
 
-def tf2_coeffs(eta, chi1, chi2):
+def tf2_coeffs(eta, chi1, chi2, delta=None):
     """TaylorF2 coefficients of interface_lalsim.py:353-433 (through 3.5PN)."""
     pi = PI
     pi_third = pi ** (1 / 3)
@@ -200,7 +205,7 @@ def tf2_coeffs(eta, chi1, chi2):
     pi_four_thirds = pi * pi_third
     pi_five_thirds = pi_four_thirds * pi_third
     e2, e3 = eta * eta, eta * eta * eta
-    delta = np.sqrt(1.0 - 4.0 * eta)
+    delta = _delta(eta, delta)
     ca = (chi1 - chi2) * 0.5
     cs = (chi1 + chi2) * 0.5
     ca2, ca3, cs2, cs3 = ca * ca, ca ** 3, cs * cs, cs ** 3
@@ -233,10 +238,10 @@ def tf2_coeffs(eta, chi1, chi2):
     return p
 
 
-def amp_pn_prefactors(eta, chi1, chi2):
+def amp_pn_prefactors(eta, chi1, chi2, delta=None):
     """PN amplitude coefficients of the PhenomD inspiral (Khan et al. 2016 eq. 29, 3PN with spins)."""
     pi = PI
-    d = np.sqrt(1.0 - 4.0 * eta)
+    d = _delta(eta, delta)
     dp1 = d + 1.0
     c12, c22 = chi1 * chi1, chi2 * chi2
     e2, e3 = eta * eta, eta ** 3
@@ -271,18 +276,18 @@ class PhenomD:
         eta = m1 * m2 / (M * M)
         eta = min(eta, 0.25)
         self.eta = eta
-        self.delta = np.sqrt(1.0 - 4.0 * eta)
+        self.delta = (m1 - m2) / M  # = sqrt(1 - 4 eta), without its cancellation near q = 1
         chis, chia = 0.5 * (chi1 + chi2), 0.5 * (chi1 - chi2)
         self.chis, self.chia = chis, chia
         self.chipn = chis * (1.0 - 76.0 * eta / 113.0) + self.delta * chia
-        self.af = final_spin(eta, chi1, chi2)
-        self.erad = radiated_energy(eta, chi1, chi2)
+        self.af = final_spin(eta, chi1, chi2, self.delta)
+        self.erad = radiated_energy(eta, chi1, chi2, self.delta)
         self.fRD, self.fDM = qnm_freqs((2, 2), self.af, self.erad)
         chi = self.chipn
         g = {k: fit(k, eta, chi) for k in FITS}
         self.g = g
-        self.apn = amp_pn_prefactors(eta, chi1, chi2)
-        self.tf2 = tf2_coeffs(eta, chi1, chi2)
+        self.apn = amp_pn_prefactors(eta, chi1, chi2, self.delta)
+        self.tf2 = tf2_coeffs(eta, chi1, chi2, self.delta)
         # amplitude: peak and intermediate collocation (Khan et al. sec. V.B)
         g2, g3 = g["gamma2"], g["gamma3"]
         if g2 <= 1.0:

@@ -23,8 +23,9 @@ SIGNATURES = [
      [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, c_int_p, ctypes.c_double, ctypes.c_int,
       ctypes.c_double, ctypes.c_double]),
     ("pes_context_add_detector", ctypes.c_int,
-     [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-      ctypes.c_int, c_double_p, c_double_p, ctypes.c_double, c_double_p, c_double_p, c_double_p, c_int_p]),
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+      ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_double_p, c_double_p,
+      ctypes.c_double, c_double_p, c_double_p, c_double_p, c_int_p]),
     ("pes_context_finalize", ctypes.c_int, [ctypes.c_void_p]),
     ("pes_context_destroy", None, [ctypes.c_void_p]),
     ("pes_lnl_batch", ctypes.c_int,
@@ -47,6 +48,18 @@ SIGNATURES = [
     ("pes_profile_read", ctypes.c_int, [ctypes.c_void_p, c_int_p, c_double_p]),
     ("pes_profile_work", ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p]),
     ("pes_detector_bins", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("pes_links", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    ("pes_links_from_waveform", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+      ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+    ("pes_tdi_from_links", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("pes_orbit_vectors", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+      ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    ("pes_context_reserve", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("pes_context_pin_scratch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
 ]
 
 

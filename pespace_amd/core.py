@@ -18,6 +18,10 @@ PARAM_NAMES = ("chirp_mass", "mass_ratio", "chi_1", "chi_2", "luminosity_distanc
 N_PARAMS = len(PARAM_NAMES)
 MODE_IDS = {"22": 0, "21": 1, "33": 2, "32": 3, "44": 4, "43": 5}
 PRECISION = {"fp64": 0, "mixed": 1}
+# geocentric orbit defaults of the reference (TianQin: orbit.py:82-84)
+GEO_OMEGA_ROTATION = 2 * np.pi / (3.65 * 86400.0)
+GEO_LAMBDA_REF = 120.5 / 180 * np.pi
+GEO_BETA_REF = -4.7 / 180 * np.pi
 
 
 def _mode_key(m):
@@ -48,8 +52,11 @@ class LikelihoodCore:
 
     # -- detectors ---------------------------------------------------------------------
     def add_detector(self, orbit_kind, arm_length, rotation_initial, revolution_initial, tdi_generation, orthogonal,
-                     frequencies, data, psd, channel_mask, df=None, weight=None):
-        """data: complex (3, N); psd: (3, N) (unused channels may hold ones)."""
+                     frequencies, data, psd, channel_mask, df=None, weight=None, omega_rotation=GEO_OMEGA_ROTATION,
+                     lambda_ref=GEO_LAMBDA_REF, beta_ref=GEO_BETA_REF):
+        """data: complex (3, N); psd: (3, N) (unused channels may hold ones).  omega_rotation,
+        lambda_ref, beta_ref: rotation rate and reference source of a geocentric orbit
+        (orbit.py:77-85; TianQin's by default), unused for heliocentric ones."""
         if self._ctx is not None:
             raise PespaceError("detectors must be added before the first evaluation")
         f = np.ascontiguousarray(frequencies, dtype=np.float64)
@@ -58,7 +65,8 @@ class LikelihoodCore:
         if data.shape != (3, f.size) or psd.shape != (3, f.size):
             raise ValueError("data and psd must have shape (3, n_bins)")
         self._dets.append(dict(kind=int(orbit_kind), arm=float(arm_length), rot0=float(rotation_initial),
-                               rev0=float(revolution_initial), gen=int(tdi_generation), orth=int(bool(orthogonal)),
+                               rev0=float(revolution_initial), omega=float(omega_rotation), lam=float(lambda_ref),
+                               bet=float(beta_ref), gen=int(tdi_generation), orth=int(bool(orthogonal)),
                                f=f, dre=np.ascontiguousarray(data.real), dim=np.ascontiguousarray(data.imag),
                                psd=psd, mask=np.asarray(channel_mask, dtype=np.int32),
                                df=0.0 if df is None else float(df),
@@ -84,7 +92,8 @@ class LikelihoodCore:
                 mask = np.ascontiguousarray(d["mask"], dtype=np.int32)
                 w = d["weight"]
                 check(L.pes_context_add_detector(
-                    h, d["kind"], d["arm"], d["rot0"], d["rev0"], d["gen"], d["orth"], d["f"].size, dptr(d["f"]),
+                    h, d["kind"], d["arm"], d["rot0"], d["rev0"], d["omega"], d["lam"], d["bet"], d["gen"],
+                    d["orth"], d["f"].size, dptr(d["f"]),
                     dptr(w) if w is not None else None, d["df"], dptr(d["dre"]), dptr(d["dim"]), dptr(d["psd"]),
                     mask.ctypes.data_as(c_int_p)))
             check(L.pes_context_finalize(h))
@@ -114,19 +123,54 @@ class LikelihoodCore:
             raise ValueError(f"params must have shape (P, {N_PARAMS})")
         return p.to(device=f"cuda:{self.device}").contiguous()
 
+    @staticmethod
+    def _check_out(t, name, shape, dev):
+        """A caller-supplied output buffer must be a contiguous float64 tensor of exactly ``shape``
+        on the evaluation device: the kernels write through its raw pointer."""
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{name} must be a torch.Tensor")
+        if t.dtype != torch.float64:
+            raise ValueError(f"{name} must be float64, got {t.dtype}")
+        if t.device != dev:
+            raise ValueError(f"{name} must be on {dev}, got {t.device}")
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+
     def lnl(self, params, grad=False, precision="fp64", out=None, grad_out=None):
+        if precision not in PRECISION:
+            raise ValueError(f"precision must be one of {tuple(PRECISION)}, got {precision!r}")
         self._build()
         p = self._params(params)
         P = p.shape[0]
         dev = p.device
+        if out is not None:
+            self._check_out(out, "out", (P,), dev)
         lnl = out if out is not None else torch.empty(P, dtype=torch.float64, device=dev)
         g = None
         if grad:
+            if grad_out is not None:
+                self._check_out(grad_out, "grad_out", (P, N_PARAMS), dev)
             g = grad_out if grad_out is not None else torch.empty(P, N_PARAMS, dtype=torch.float64, device=dev)
         stream = torch.cuda.current_stream(dev).cuda_stream
         check(lib().pes_lnl_batch(self._ctx, p.data_ptr(), P, lnl.data_ptr(), g.data_ptr() if g is not None else None,
                                   PRECISION[precision], stream))
         return (lnl, g) if grad else lnl
+
+    # -- scratch for HIP-graph capture ---------------------------------------------------------
+    def reserve(self, P, grad=True):
+        """Size the context's scratch for batches of up to ``P`` rows (with the gradient tables if
+        ``grad``) under the current schedule."""
+        self._build()
+        check(lib().pes_context_reserve(self._ctx, int(P), int(bool(grad))))
+
+    def pin_scratch(self, pinned=True):
+        """While pinned the scratch is never reallocated: a call that would need more (a larger
+        batch, the first gradient call, a schedule with more tile groups) raises instead of
+        freeing buffers that a captured HIP graph still uses."""
+        self._build()
+        check(lib().pes_context_pin_scratch(self._ctx, int(bool(pinned))))
 
     # -- schedule of the fused kernel ------------------------------------------------------
     def set_schedule(self, tiles_per_group, blocks_per_cu):
@@ -204,22 +248,37 @@ class LikelihoodCore:
                                  torch.cuda.current_stream(p.device).cuda_stream))
         return torch.view_as_complex(out)
 
-    def waveform(self, det, params):
-        """(P, n_modes, N, 5): plus re/im, cross re/im, tf on detector det's bins."""
+    def links(self, det, params):
+        """Complex (P, 6, N) single links (12, 21, 23, 32, 31, 13) of detector det, summed over the
+        modes (antenna.py:496-543), from the same device code as the fused kernel."""
         self._build()
         p = self._params(params)
         n = self._n_bins[det]
-        out = torch.empty(p.shape[0], len(self.modes), n, 5, dtype=torch.float64, device=p.device)
-        check(lib().pes_waveform(self._ctx, det, p.data_ptr(), p.shape[0], out.data_ptr(),
-                                 torch.cuda.current_stream(p.device).cuda_stream))
-        return out
+        out = torch.empty(p.shape[0], 6, n, 2, dtype=torch.float64, device=p.device)
+        check(lib().pes_links(self._ctx, det, p.data_ptr(), p.shape[0], out.data_ptr(),
+                              torch.cuda.current_stream(p.device).cuda_stream))
+        return torch.view_as_complex(out)
 
-    def response_from_container(self, det, waveform, lam, beta, psi, tc):
-        """Complex (3, N) response of detector det to a reference-style container.
-
-        waveform: {"plus","cross","tf"} or a mode-keyed dict of those (numpy or torch).
-        """
+    def tdi_from_links(self, det, links):
+        """TDI channels (P, 3, N) (A, E, T or X, Y, Z) of detector det from single links (P, 6, N)
+        or (6, N) (tdi.py:1705-1793)."""
         self._build()
+        n = self._n_bins[det]
+        dev = torch.device(f"cuda:{self.device}")
+        lk = torch.as_tensor(links, device=dev).to(torch.complex128)
+        squeeze = lk.dim() == 2
+        if squeeze:
+            lk = lk[None]
+        if lk.dim() != 3 or lk.shape[1] != 6 or lk.shape[2] != n:
+            raise ValueError(f"links must have shape (P, 6, {n}) or (6, {n})")
+        lr = torch.view_as_real(lk.contiguous()).contiguous()
+        out = torch.empty(lk.shape[0], 3, n, 2, dtype=torch.float64, device=dev)
+        check(lib().pes_tdi_from_links(self._ctx, det, lk.shape[0], lr.data_ptr(), out.data_ptr(),
+                                       torch.cuda.current_stream(dev).cuda_stream))
+        out = torch.view_as_complex(out)
+        return out[0] if squeeze else out
+
+    def _container(self, det, waveform):
         n = self._n_bins[det]
         modes = [waveform] if {"plus", "cross", "tf"} <= set(waveform.keys()) else list(waveform.values())
         dev = torch.device(f"cuda:{self.device}")
@@ -237,11 +296,60 @@ class LikelihoodCore:
             wf[i, :, 2] = hc.real
             wf[i, :, 3] = hc.imag
             wf[i, :, 4] = tf
+        return wf, len(modes), dev
+
+    def links_from_container(self, det, waveform, lam, beta, psi, tc):
+        """Complex (6, N) single links of detector det for a reference-style container."""
+        self._build()
+        wf, n_modes, dev = self._container(det, waveform)
+        out = torch.empty(6, self._n_bins[det], 2, dtype=torch.float64, device=dev)
+        check(lib().pes_links_from_waveform(self._ctx, det, n_modes, wf.data_ptr(), float(lam), float(beta),
+                                            float(psi), float(tc), out.data_ptr(),
+                                            torch.cuda.current_stream(dev).cuda_stream))
+        return torch.view_as_complex(out)
+
+    def waveform(self, det, params):
+        """(P, n_modes, N, 5): plus re/im, cross re/im, tf on detector det's bins."""
+        self._build()
+        p = self._params(params)
+        n = self._n_bins[det]
+        out = torch.empty(p.shape[0], len(self.modes), n, 5, dtype=torch.float64, device=p.device)
+        check(lib().pes_waveform(self._ctx, det, p.data_ptr(), p.shape[0], out.data_ptr(),
+                                 torch.cuda.current_stream(p.device).cuda_stream))
+        return out
+
+    def response_from_container(self, det, waveform, lam, beta, psi, tc):
+        """Complex (3, N) response of detector det to a reference-style container.
+
+        waveform: {"plus","cross","tf"} or a mode-keyed dict of those (numpy or torch).
+        """
+        self._build()
+        wf, n_modes, dev = self._container(det, waveform)
+        n = self._n_bins[det]
         out = torch.empty(3, n, 2, dtype=torch.float64, device=dev)
-        check(lib().pes_response_from_waveform(self._ctx, det, len(modes), wf.data_ptr(), float(lam), float(beta),
+        check(lib().pes_response_from_waveform(self._ctx, det, n_modes, wf.data_ptr(), float(lam), float(beta),
                                                float(psi), float(tc), out.data_ptr(),
                                                torch.cuda.current_stream(dev).cuda_stream))
         return torch.view_as_complex(out)
+
+
+def orbit_vectors(orbit_kind, arm_length, rotation_initial, revolution_initial, time,
+                  omega_rotation=GEO_OMEGA_ROTATION, lambda_ref=GEO_LAMBDA_REF, beta_ref=GEO_BETA_REF, device=0):
+    """Constellation vectors at the given times on the GPU: a (..., 6, 3) float64 tensor holding
+    n1, n2, n3 (unit link vectors) and x1, x2, x3 (node positions in the SSB frame, seconds)
+    (orbit.py:260-322, 363-419)."""
+    if not torch.cuda.is_available():
+        raise PespaceError("pespace_amd's hip backend needs a GPU (torch.cuda.is_available() is False)")
+    dev = torch.device(f"cuda:{int(device)}")
+    t = torch.as_tensor(time, dtype=torch.float64, device=dev)
+    shape = tuple(t.shape)
+    t = t.reshape(-1).contiguous()
+    out = torch.empty(t.numel(), 6, 3, dtype=torch.float64, device=dev)
+    check(lib().pes_orbit_vectors(int(orbit_kind), float(arm_length), float(rotation_initial),
+                                  float(revolution_initial), float(omega_rotation), float(lambda_ref),
+                                  float(beta_ref), t.data_ptr(), t.numel(), out.data_ptr(),
+                                  torch.cuda.current_stream(dev).cuda_stream))
+    return out.reshape(shape + (6, 3))
 
 
 def whittle_lnl(data, response, psd, df):

@@ -345,12 +345,13 @@ __device__ inline void tdi_out(const DetectorDev& det, const TdiC& t, const Link
   }
 }
 
-template <int PREC, int KIND>
+// The six links of one (bin, row) summed over the modes, without the common factor
+// (-i xL) exp(-i xL) of every link term.
 // nb: node entries (j, j+1) of this bin for mode 0; mode mi's are at nb + mi * mstride
-__device__ inline void response_row(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
-                                    const double* __restrict__ nb, size_t mstride, const BinPre& b,
-                                    const RowC& rc, const TdiC& tdc, Cx out[3]) {
-  LinksT L;
+template <int PREC, int KIND>
+__device__ inline void links_row(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
+                                 const double* __restrict__ nb, size_t mstride, const BinPre& b, const RowC& rc,
+                                 LinksT& L) {
 #pragma unroll
   for (int i = 0; i < 6; ++i) L.l[i] = {0.0, 0.0};
   for (int mi = 0; mi < modes.n; ++mi) {
@@ -367,6 +368,14 @@ __device__ inline void response_row(const DetectorDev& det, const double* __rest
       L.l[lm] = axpy(v.sm, v.C, L.l[lm]);
     }
   }
+}
+
+template <int PREC, int KIND>
+__device__ inline void response_row(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
+                                    const double* __restrict__ nb, size_t mstride, const BinPre& b,
+                                    const RowC& rc, const TdiC& tdc, Cx out[3]) {
+  LinksT L;
+  links_row<PREC, KIND>(det, pc, modes, nb, mstride, b, rc, L);
   tdi_out(det, tdc, L, out);
 }
 
@@ -907,3 +916,38 @@ __global__ __launch_bounds__(128) void k_response_t(const DetectorDev* __restric
     out[idx + 1] = o[ch].im;
   }
 }
+
+#ifndef PES_LNL_FWD_UNIT
+// The six single links (12, 21, 23, 32, 31, 13) of P rows on one detector, summed over the
+// modes, with the common factor (-i xL) exp(-i xL) applied: out [P][6][n_bins][2]
+// (antenna.py:496-543 per link, summed over modes as antenna.py:445-458).
+__global__ __launch_bounds__(128) void k_links(const DetectorDev* __restrict__ dets, int det_i,
+                                               const double* __restrict__ bin_f, const int* __restrict__ bin_j,
+                                               const double* __restrict__ bin_s, const double* __restrict__ pconst,
+                                               const double* __restrict__ nodes, int NNS, double du, ModeList modes,
+                                               int P, double* __restrict__ out) {
+  trig_table_load();
+  const DetectorDev det = dets[det_i];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (k >= det.n_bins || p >= P) return;
+  const int bi = det.bin_offset + k;
+  BinPre b;
+  bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
+  const double* pc = pconst + (size_t)p * PC_STRIDE;
+  const RowC rc = load_row(pc, b);
+  LinksT L;
+  const double* nb = nodes + ((size_t)p * modes.n * NNS + b.j) * 4;
+  if (det.kind == 0)
+    links_row<0, 0>(det, pc, modes, nb, (size_t)NNS * 4, b, rc, L);
+  else
+    links_row<0, 1>(det, pc, modes, nb, (size_t)NNS * 4, b, rc, L);
+  const Cx cf = {-b.xL * b.sxL, -b.xL * b.cxL};  // (-i xL) exp(-i xL)
+  for (int i = 0; i < 6; ++i) {
+    const Cx l = mul(cf, L.l[i]);
+    const size_t idx = (((size_t)p * 6 + i) * det.n_bins + k) * 2;
+    out[idx] = l.re;
+    out[idx + 1] = l.im;
+  }
+}
+#endif  // PES_LNL_FWD_UNIT

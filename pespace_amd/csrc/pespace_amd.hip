@@ -409,6 +409,74 @@ __global__ __launch_bounds__(128) void k_response_container(const DetectorDev* _
   }
 }
 
+// ---- the six single links of a caller-supplied container (antenna.py:403-460), their TDI
+// combination (tdi.py:1705-1793, the JAX twin _tdi_jax.py:70-102), and the constellation
+// vectors (orbit.py:260-322, 363-419) ----------------------------------------------------------
+// links: [6][n_bins][2] in the order 12, 21, 23, 32, 31, 13 (antenna.py:24-31)
+__global__ __launch_bounds__(128) void k_links_container(const DetectorDev* __restrict__ dets, int det_i,
+                                                         const double* __restrict__ bin_f, int n_modes,
+                                                         const double* __restrict__ wf, Extrinsic ex,
+                                                         double* __restrict__ out) {
+  const DetectorDev det = dets[det_i];
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= det.n_bins) return;
+  double f = bin_f[det.bin_offset + k];
+  Cplx etc;
+  sincos(-2.0 * PI * f * ex.tc, &etc.im, &etc.re);
+  Links L = {};
+  for (int m = 0; m < n_modes; ++m) {
+    const double* w = wf + ((size_t)m * det.n_bins + k) * 5;
+    Cplx hp = cmul({w[0], w[1]}, etc);
+    Cplx hc = cmul({w[2], w[3]}, etc);
+    add_mode_links(det, f, w[4] + ex.tc, hp, hc, ex.p, ex.q, ex.k, L);
+  }
+  const Cplx l[6] = {L.l12, L.l21, L.l23, L.l32, L.l31, L.l13};
+  for (int i = 0; i < 6; ++i) {
+    out[((size_t)i * det.n_bins + k) * 2] = l[i].re;
+    out[((size_t)i * det.n_bins + k) * 2 + 1] = l[i].im;
+  }
+}
+
+// links [P][6][n_bins][2] -> channels [P][3][n_bins][2]
+__global__ __launch_bounds__(128) void k_tdi_from_links(const DetectorDev* __restrict__ dets, int det_i,
+                                                        const double* __restrict__ bin_f, int P,
+                                                        const double* __restrict__ links, double* __restrict__ out) {
+  const DetectorDev det = dets[det_i];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (k >= det.n_bins || p >= P) return;
+  const double f = bin_f[det.bin_offset + k];
+  const double* lp = links + (size_t)p * 6 * det.n_bins * 2;
+  auto get = [&](int i) { return Cplx{lp[((size_t)i * det.n_bins + k) * 2], lp[((size_t)i * det.n_bins + k) * 2 + 1]}; };
+  Links L = {get(0), get(1), get(2), get(3), get(4), get(5)};
+  Cplx o[3];
+  tdi_combine(det, f, L, o);
+  for (int ch = 0; ch < 3; ++ch) {
+    const size_t idx = (((size_t)p * 3 + ch) * det.n_bins + k) * 2;
+    out[idx] = o[ch].re;
+    out[idx + 1] = o[ch].im;
+  }
+}
+
+// constellation vectors at n times: out [n][6][3] = n1, n2, n3 (unit link vectors) and x1, x2, x3
+// (node positions in the SSB frame, seconds)
+__global__ __launch_bounds__(128) void k_orbit_vectors(DetectorDev det, const double* __restrict__ t, int n,
+                                                       double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Vec3 nv[3], xv[3];
+  constellation(det, t[i], nv, xv);
+  double* o = out + (size_t)i * 18;
+  for (int a = 0; a < 3; ++a) {
+    o[3 * a] = nv[a].x;
+    o[3 * a + 1] = nv[a].y;
+    o[3 * a + 2] = nv[a].z;
+    o[9 + 3 * a] = xv[a].x;
+    o[9 + 3 * a + 1] = xv[a].y;
+    o[9 + 3 * a + 2] = xv[a].z;
+  }
+}
+
 // ---- standalone Whittle lnL: -2 df sum_c sum_k |d - h|^2 / S (likelihood_ti.py:58-67) --------
 constexpr int WHITTLE_BLOCKS = 1024;
 __global__ __launch_bounds__(TILE) void k_whittle_partial(const double* __restrict__ data, const double* __restrict__ resp,
@@ -493,9 +561,11 @@ struct pes_context {
   double *d_tile_f0 = nullptr, *d_tile_dd = nullptr;  // first frequency and sum w |d|^2 per tile
   int* d_tile_jr = nullptr;  // per tile: first node index and number of node rows its bins use
   double* d_node_f = nullptr;  // direct mode only
-  // scratch, grown on demand
+  // scratch, grown on demand; while pinned (a captured HIP graph holds its pointers) it is
+  // never reallocated and a call that would need more is rejected
   int cap_P = 0;
   bool cap_grad = false;
+  bool pinned = false;
   size_t cap_partial = 0;  // doubles
   // schedule of the fused kernel (pes_context_set_schedule)
   int tiles_per_group = TILES_PER_GROUP;
@@ -555,25 +625,17 @@ extern "C" int pes_context_create(pes_context** out, int device, int n_modes, co
   return PES_OK;
 }
 
-extern "C" int pes_context_add_detector(pes_context* ctx, int orbit_kind, double arm_length_m,
-                                        double rotation_initial, double revolution_initial, int tdi_generation,
-                                        int orthogonal, int n_bins, const double* f, const double* weight, double df,
-                                        const double* data_re, const double* data_im, const double* psd,
-                                        const int* channel_mask) {
-  if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
-  if (ctx->finalized) return fail(PES_ERR_STATE, "context already finalized");
+// Orbit constants of one detector (heliocentric: orbit.py:189-228; geocentric: orbit.py:112-139
+// with the rotation rate and reference source given).
+static int fill_orbit(DetectorDev& d, int orbit_kind, double arm_length_m, double rotation_initial,
+                      double revolution_initial, double omega_rotation, double lambda_ref, double beta_ref) {
   if (orbit_kind != 0 && orbit_kind != 1) return fail(PES_ERR_INVALID, "orbit_kind must be 0 or 1");
-  if (tdi_generation != 15 && tdi_generation != 20) return fail(PES_ERR_INVALID, "tdi_generation must be 15 or 20");
-  if (n_bins < 1 || !f || !data_re || !data_im || !psd || !channel_mask)
-    return fail(PES_ERR_INVALID, "detector arrays must be non-NULL and n_bins >= 1");
-  if (!(arm_length_m > 0.0)) return fail(PES_ERR_INVALID, "arm length must be positive");
-  pes_context::HostDet h;
-  DetectorDev& d = h.d;
-  std::memset(&d, 0, sizeof(d));
+  if (!(arm_length_m > 0.0) || !std::isfinite(arm_length_m)) return fail(PES_ERR_INVALID, "arm length must be positive");
+  if (!std::isfinite(rotation_initial) || !std::isfinite(revolution_initial))
+    return fail(PES_ERR_INVALID, "initial orbit phases must be finite");
+  if (orbit_kind == 1 && !(std::isfinite(omega_rotation) && std::isfinite(lambda_ref) && std::isfinite(beta_ref)))
+    return fail(PES_ERR_INVALID, "geocentric rotation rate and reference source must be finite");
   d.kind = orbit_kind;
-  d.tdi_gen = tdi_generation;
-  d.orthogonal = orthogonal ? 1 : 0;
-  d.n_bins = n_bins;
   d.L = arm_length_m / C_SI;
   d.rot0 = rotation_initial;
   d.rev0 = revolution_initial;
@@ -593,11 +655,11 @@ extern "C" int pes_context_add_detector(pes_context* ctx, int orbit_kind, double
       d.hca[i] = d.c3L * d.hck[i];
       d.hsa[i] = d.c3L * d.hsk[i];
     }
-  } else {  // orbit.py:77-139, TianQin defaults
+  } else {  // orbit.py:77-139
     d.rp = d.L / std::sqrt(3.0);
-    double lr = 120.5 / 180.0 * PI, br = -4.7 / 180.0 * PI;
-    d.slam = std::sin(lr); d.clam = std::cos(lr); d.sbet = std::sin(br); d.cbet = std::cos(br);
-    d.omega_rot = 2.0 * PI / (3.65 * DAY_SI);
+    d.slam = std::sin(lambda_ref); d.clam = std::cos(lambda_ref);
+    d.sbet = std::sin(beta_ref); d.cbet = std::cos(beta_ref);
+    d.omega_rot = omega_rotation;
     d.gxs = d.rp * d.sbet * d.clam;
     d.gxc = d.rp * d.slam;
     d.gys = d.rp * d.sbet * d.slam;
@@ -610,6 +672,29 @@ extern "C" int pes_context_add_detector(pes_context* ctx, int orbit_kind, double
     d.sgyc = d.c3L * d.gyc;
     d.sgzs = d.c3L * d.gzs;
   }
+  return PES_OK;
+}
+
+extern "C" int pes_context_add_detector(pes_context* ctx, int orbit_kind, double arm_length_m,
+                                        double rotation_initial, double revolution_initial, double omega_rotation,
+                                        double lambda_ref, double beta_ref, int tdi_generation, int orthogonal,
+                                        int n_bins, const double* f, const double* weight, double df,
+                                        const double* data_re, const double* data_im, const double* psd,
+                                        const int* channel_mask) {
+  if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
+  if (ctx->finalized) return fail(PES_ERR_STATE, "context already finalized");
+  if (tdi_generation != 15 && tdi_generation != 20) return fail(PES_ERR_INVALID, "tdi_generation must be 15 or 20");
+  if (n_bins < 1 || !f || !data_re || !data_im || !psd || !channel_mask)
+    return fail(PES_ERR_INVALID, "detector arrays must be non-NULL and n_bins >= 1");
+  pes_context::HostDet h;
+  DetectorDev& d = h.d;
+  std::memset(&d, 0, sizeof(d));
+  const int orc = fill_orbit(d, orbit_kind, arm_length_m, rotation_initial, revolution_initial, omega_rotation,
+                             lambda_ref, beta_ref);
+  if (orc != PES_OK) return orc;
+  d.tdi_gen = tdi_generation;
+  d.orthogonal = orthogonal ? 1 : 0;
+  d.n_bins = n_bins;
   h.f.assign(f, f + n_bins);
   h.w.assign(3 * (size_t)n_bins, 0.0);
   h.dre.assign(data_re, data_re + 3 * (size_t)n_bins);
@@ -617,9 +702,6 @@ extern "C" int pes_context_add_detector(pes_context* ctx, int orbit_kind, double
   for (int k = 0; k < n_bins; ++k) {
     if (!(f[k] > 0.0) || !std::isfinite(f[k])) return fail(PES_ERR_INVALID, "frequencies must be positive");
     if (k > 0 && !(f[k] > f[k - 1])) return fail(PES_ERR_INVALID, "frequencies must be strictly increasing");
-    // the transfer-function sines take arguments up to pi f L (no range reduction above 4)
-    if (PI * f[k] * d.L > 4.0)
-      return fail(PES_ERR_INVALID, "pi f L must not exceed 4 (grid too high for this arm length)");
     double dfk = weight ? weight[k] : df;
     for (int ch = 0; ch < 3; ++ch) {
       if (!channel_mask[ch]) continue;
@@ -679,6 +761,19 @@ extern "C" int pes_context_finalize(pes_context* ctx) {
       bs[k] = 0.0;
     }
   } else {
+    // every bin must lie inside the node range: the cubic Hermite would extrapolate outside it
+    const double tol = 1e-6;  // in node intervals (a relative frequency tolerance of ~1e-6 du)
+    for (int k = 0; k < NB; ++k) {
+      const double x = (std::log(bf[k]) - ctx->u0) / ctx->du;
+      if (!(x >= -tol && x <= (double)(ctx->n_nodes - 1) + tol)) {
+        char msg[256];
+        std::snprintf(msg, sizeof msg,
+                      "bin frequency %.17g Hz lies outside the interpolation node range [%.17g, %.17g] Hz: "
+                      "widen f_node_min / f_node_max to cover every detector grid",
+                      bf[k], ctx->f_node_min, ctx->f_node_max);
+        return fail(PES_ERR_INVALID, msg);
+      }
+    }
     for (int k = 0; k < NB; ++k) {
       double x = (std::log(bf[k]) - ctx->u0) / ctx->du;
       double jf = std::floor(x);
@@ -780,6 +875,11 @@ static int n_groups_of(const pes_context* ctx) {
 static int ensure_scratch(pes_context* ctx, int P, bool grad) {
   {
     const size_t need = (size_t)n_groups_of(ctx) * (1 + PES_N_PARAMS) * (size_t)(P > ctx->cap_P ? P : ctx->cap_P);
+    const bool grow = need > ctx->cap_partial || P > ctx->cap_P || (grad && !ctx->cap_grad);
+    if (grow && ctx->pinned)
+      return fail(PES_ERR_STATE,
+                  "scratch is pinned (a captured HIP graph holds its pointers) and this call needs more: "
+                  "reserve the largest batch, the gradient and the schedule before pinning");
     if (need > ctx->cap_partial) {
       free_dev(ctx->d_partial);
       ctx->d_partial = nullptr;
@@ -927,12 +1027,37 @@ extern "C" int pes_waveform(pes_context* ctx, int det, const double* d_params, i
   return PES_OK;
 }
 
+static Extrinsic extrinsic_of(double lam, double beta, double psi, double tc);
+
 extern "C" int pes_response_from_waveform(pes_context* ctx, int det, int n_modes, const double* d_wf, double lam,
                                           double beta, double psi, double tc, double* d_out, void* stream) {
   if (!ctx || !ctx->finalized) return fail(PES_ERR_STATE, "context missing or not finalized");
   if (det < 0 || det >= (int)ctx->hdets.size()) return fail(PES_ERR_INVALID, "bad detector index");
   if (n_modes < 1 || !d_wf || !d_out) return fail(PES_ERR_INVALID, "waveform/out must be non-NULL, n_modes >= 1");
   HIP_TRY(hipSetDevice(ctx->device));
+  const Extrinsic ex = extrinsic_of(lam, beta, psi, tc);
+  int n = ctx->hdets[det].d.n_bins;
+  hipLaunchKernelGGL(k_response_container, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream, ctx->d_dets,
+                     det, ctx->d_bin_f, n_modes, d_wf, ex, d_out);
+  HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" int pes_links(pes_context* ctx, int det, const double* d_params, int P, double* d_out, void* stream) {
+  if (!ctx || det < 0 || det >= (int)ctx->hdets.size()) return fail(PES_ERR_INVALID, "bad detector index");
+  if (P > 0 && !d_out) return fail(PES_ERR_INVALID, "out is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = prepare(ctx, d_params, P, false, st);
+  if (rc != PES_OK || P == 0) return rc;
+  const int n = ctx->hdets[det].d.n_bins;
+  hipLaunchKernelGGL(k_links, dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det, ctx->d_bin_f,
+                     ctx->d_bin_j, ctx->d_bin_s, ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1, ctx->du, ctx->modes,
+                     P, d_out);
+  HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+static Extrinsic extrinsic_of(double lam, double beta, double psi, double tc) {
   Extrinsic ex;
   double sl = std::sin(lam), cl = std::cos(lam), sb = std::sin(beta), cb = std::cos(beta);
   double sp = std::sin(psi), cp = std::cos(psi);
@@ -940,10 +1065,63 @@ extern "C" int pes_response_from_waveform(pes_context* ctx, int det, int n_modes
   ex.q = {-sl * sp - cl * sb * cp, cl * sp - sl * sb * cp, cb * cp};   // antenna.py:77-83
   ex.k = {-cb * cl, -cb * sl, -sb};                                    // antenna.py:112-114
   ex.tc = tc;
-  int n = ctx->hdets[det].d.n_bins;
-  hipLaunchKernelGGL(k_response_container, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream, ctx->d_dets,
-                     det, ctx->d_bin_f, n_modes, d_wf, ex, d_out);
+  return ex;
+}
+
+extern "C" int pes_links_from_waveform(pes_context* ctx, int det, int n_modes, const double* d_wf, double lam,
+                                       double beta, double psi, double tc, double* d_out, void* stream) {
+  if (!ctx || !ctx->finalized) return fail(PES_ERR_STATE, "context missing or not finalized");
+  if (det < 0 || det >= (int)ctx->hdets.size()) return fail(PES_ERR_INVALID, "bad detector index");
+  if (n_modes < 1 || !d_wf || !d_out) return fail(PES_ERR_INVALID, "waveform/out must be non-NULL, n_modes >= 1");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int n = ctx->hdets[det].d.n_bins;
+  hipLaunchKernelGGL(k_links_container, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream, ctx->d_dets, det,
+                     ctx->d_bin_f, n_modes, d_wf, extrinsic_of(lam, beta, psi, tc), d_out);
   HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" int pes_tdi_from_links(pes_context* ctx, int det, int P, const double* d_links, double* d_out,
+                                  void* stream) {
+  if (!ctx || !ctx->finalized) return fail(PES_ERR_STATE, "context missing or not finalized");
+  if (det < 0 || det >= (int)ctx->hdets.size()) return fail(PES_ERR_INVALID, "bad detector index");
+  if (P < 0) return fail(PES_ERR_INVALID, "P must be >= 0");
+  if (P > 0 && (!d_links || !d_out)) return fail(PES_ERR_INVALID, "links/out must be non-NULL");
+  if (P == 0) return PES_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int n = ctx->hdets[det].d.n_bins;
+  hipLaunchKernelGGL(k_tdi_from_links, dim3((n + 127) / 128, P), dim3(128), 0, (hipStream_t)stream, ctx->d_dets,
+                     det, ctx->d_bin_f, P, d_links, d_out);
+  HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" int pes_orbit_vectors(int orbit_kind, double arm_length_m, double rotation_initial,
+                                 double revolution_initial, double omega_rotation, double lambda_ref, double beta_ref,
+                                 const double* d_t, int n, double* d_out, void* stream) {
+  if (n < 0) return fail(PES_ERR_INVALID, "n must be >= 0");
+  if (n > 0 && (!d_t || !d_out)) return fail(PES_ERR_INVALID, "times/out must be non-NULL");
+  DetectorDev d;
+  std::memset(&d, 0, sizeof(d));
+  const int rc = fill_orbit(d, orbit_kind, arm_length_m, rotation_initial, revolution_initial, omega_rotation,
+                            lambda_ref, beta_ref);
+  if (rc != PES_OK || n == 0) return rc;
+  hipLaunchKernelGGL(k_orbit_vectors, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream, d, d_t, n, d_out);
+  HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" int pes_context_reserve(pes_context* ctx, int P, int grad) {
+  if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
+  if (!ctx->finalized) return fail(PES_ERR_STATE, "context not finalized");
+  if (P < 1) return fail(PES_ERR_INVALID, "P must be >= 1");
+  HIP_TRY(hipSetDevice(ctx->device));
+  return ensure_scratch(ctx, P, grad != 0);
+}
+
+extern "C" int pes_context_pin_scratch(pes_context* ctx, int pinned) {
+  if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
+  ctx->pinned = pinned != 0;
   return PES_OK;
 }
 

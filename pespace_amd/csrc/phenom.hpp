@@ -309,8 +309,7 @@ struct Phenom {
 
 // ---- setup from physical parameters -------------------------------------------------
 template <class S>
-PES_HD S final_spin(const S& eta, const S& chi1, const S& chi2) {
-  S d = dsqrt(1.0 - 4.0 * eta);
+PES_HD S final_spin(const S& eta, const S& d, const S& chi1, const S& chi2) {
   S m1 = 0.5 * (1.0 + d), m2 = 0.5 * (1.0 - d);
   S s = m1 * m1 * chi1 + m2 * m2 * chi2;
   S e2 = eta * eta, e3 = e2 * eta;
@@ -324,8 +323,7 @@ PES_HD S final_spin(const S& eta, const S& chi1, const S& chi2) {
 }
 
 template <class S>
-PES_HD S radiated_energy(const S& eta, const S& chi1, const S& chi2) {
-  S d = dsqrt(1.0 - 4.0 * eta);
+PES_HD S radiated_energy(const S& eta, const S& d, const S& chi1, const S& chi2) {
   S m1 = 0.5 * (1.0 + d), m2 = 0.5 * (1.0 - d);
   S s = (m1 * m1 * chi1 + m2 * m2 * chi2) / (m1 * m1 + m2 * m2);
   S e2 = eta * eta, e3 = e2 * eta, e4 = e3 * eta;
@@ -377,7 +375,9 @@ PES_HD void phenom_setup(Phenom<S>& P, const S& Mc, const S& q, const S& chi1_in
   S M = m1 + m2;
   S eta = m1 * m2 / (M * M);
   if (value_of(eta) > 0.25) eta = lift<S>(0.25);
-  S delta = dsqrt(1.0 - 4.0 * eta);
+  // delta = (m1 - m2) / M = sqrt(1 - 4 eta), formed from the masses: the same value (without the
+  // cancellation of 1 - 4 eta near equal masses) and a finite derivative at q = 1
+  S delta = (m1 - m2) / M;
   S chis = 0.5 * (chi1 + chi2), chia = 0.5 * (chi1 - chi2);
   S chipn = chis * (1.0 - 76.0 * eta / 113.0) + delta * chia;
   S* c = P.c;
@@ -388,8 +388,8 @@ PES_HD void phenom_setup(Phenom<S>& P, const S& Mc, const S& q, const S& chi1_in
   c[SU_DELTA] = delta;
   c[SU_CHIS] = chis;
   c[SU_CHIA] = chia;
-  S af = final_spin(eta, chi1, chi2);
-  S erad = radiated_energy(eta, chi1, chi2);
+  S af = final_spin(eta, delta, chi1, chi2);
+  S erad = radiated_energy(eta, delta, chi1, chi2);
   S inv1me = 1.0 / (1.0 - erad);
   c[SU_FRD] = qnm_interp(kQnmFring[0], af) * inv1me;
   c[SU_FDM] = qnm_interp(kQnmFdamp[0], af) * inv1me;

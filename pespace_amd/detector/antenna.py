@@ -1,12 +1,18 @@
 """Detector facade and the Marsat-Baker FD response model, hip backend.
 
 Mirrors pespace/detector/antenna.py:117-379: ``InterferometerAntenna`` owns the
-TDI data, orbit, response model and TDI combination; ``update_detector_response``
-computes the A/E/T (or X/Y/Z) response of a waveform on the GPU and
-``inject_signal`` adds it to the data.  The waveform may be a hip waveform model
-(``pespace_amd.waveform.IMRPhenomD/IMRPhenomHM``, evaluated inside the fused
-kernel) or any reference-style container ``{"plus","cross","tf"}`` / mode-keyed
-dict of those (antenna.py:438-458).
+TDI data, orbit, response model and TDI combination.  As in the reference,
+``update_detector_response`` runs two stages on the GPU: the response model's
+``update_single_link_response`` stores the six single links on the detector
+(``single_link_response``, read back by ``single_link_response_array``), and the TDI
+combination's ``update_tdi_response`` combines them into A/E/T (or X/Y/Z);
+``inject_signal`` adds the result to the data.  The JAX backend's pure functions
+``get_response_jit`` (six links) and ``get_combination_jit`` (three channels) are
+provided too (_antenna_jax.py:82-195, _tdi_jax.py:70-102).  The waveform may be a
+hip waveform model (``pespace_amd.waveform.IMRPhenomD/IMRPhenomHM``, evaluated on
+the device) or any reference-style container ``{"plus","cross","tf"}`` / mode-keyed
+dict of those (antenna.py:438-458).  The likelihood itself never materialises the
+links: the fused kernel takes them straight into the residual.
 """
 
 from __future__ import annotations
@@ -19,6 +25,7 @@ import torch
 from .orbit import OrbitModelBase, available_orbit_models
 
 BACKENDS = ("hip",)
+LINK_NAMES = ("link12", "link21", "link23", "link32", "link31", "link13")  # antenna.py:24-31
 
 
 class FDResponseModelMarset2018:
@@ -32,9 +39,18 @@ class FDResponseModelMarset2018:
 
     def init_single_link_response_model(self, detector):
         self.detector = detector
+        detector.single_link_response = None
+
+    def get_response_jit(self, waveform, lam, beta, psi, tc):
+        """Six single links (link12, link21, link23, link32, link31, link13), each a complex (N,)
+        tensor on the GPU (_antenna_jax.py:82-195); nothing is stored on the detector."""
+        links = self.detector._links(waveform, lam, beta, psi, tc)
+        return tuple(links[i] for i in range(6))
 
     def update_single_link_response(self, waveform, lam, beta, psi, tc):
-        self.detector._update_response(waveform, lam, beta, psi, tc)
+        """antenna.py:403-460: the links of ``waveform`` stored as ``detector.single_link_response``
+        (complex (6, N) tensor)."""
+        self.detector.single_link_response = self.detector._links(waveform, lam, beta, psi, tc)
 
 
 class InterferometerAntenna:
@@ -81,19 +97,21 @@ class InterferometerAntenna:
             self._core_key = key
         return self._core
 
-    def _update_response(self, waveform, lam, beta, psi, tc):
+    def _links(self, waveform, lam, beta, psi, tc):
+        """Complex (6, N) single links of a hip waveform model or a reference-style container."""
         from ..waveform.phenom import PhenomWaveformBase
-        labels = self.tdi_combination.labels
         if isinstance(waveform, PhenomWaveformBase):
             row = waveform.parameter_row(lam, beta, psi, tc)
             core = self._response_core(waveform.modes, waveform.reference_frequency, waveform.n_nodes,
                                        waveform.f_node_min, waveform.f_node_max)
-            out = core.response(0, row[None, :])[0]
-        else:
-            core = self._response_core(("22",), 0.0, 2, None, None)
-            out = core.response_from_container(0, waveform, lam, beta, psi, tc)
-        DataContent = namedtuple("DataContent", labels)
-        self.tdi_response = DataContent(*[out[i] for i in range(3)])
+            return core.links(0, row[None, :])[0]
+        core = self._response_core(("22",), 0.0, 2, None, None)
+        return core.links_from_container(0, waveform, lam, beta, psi, tc)
+
+    def _tdi_from_links(self, links):
+        """Complex (3, N) TDI channels of this detector's combination from (6, N) links."""
+        core = self._core if self._core is not None else self._response_core(("22",), 0.0, 2, None, None)
+        return core.tdi_from_links(0, links)
 
     def update_detector_response(self, waveform, lam, beta, psi, tc):
         """antenna.py:204-229."""
@@ -113,6 +131,14 @@ class InterferometerAntenna:
         labels = self.tdi_combination.labels
         DataContent = namedtuple("DataContent", labels)
         return DataContent(*[getattr(self.tdi_response, c).detach().cpu().numpy() for c in labels])
+
+    @property
+    def single_link_response_array(self):
+        """The stored single links as a NamedTuple of complex numpy arrays (antenna.py:316-323)."""
+        if self.single_link_response is None:
+            raise ValueError("no single-link response: call update_detector_response first")
+        DataContent = namedtuple("DataContent", LINK_NAMES)
+        return DataContent(*[self.single_link_response[i].detach().cpu().numpy() for i in range(6)])
 
 
 def add_detector_to_core(core, det, data_channels):
@@ -135,4 +161,4 @@ def add_detector_to_core(core, det, data_channels):
     return core.add_detector(orb.kind, orb.arm_length, orb.rotation_initial, orb.revolution_initial,
                              det.tdi_combination.generation_code, det.tdi_combination.orthogonal, f, data, psd, mask,
                              df=td.data_info.delta_frequency if uniform else 0.0,
-                             weight=None if uniform else weight)
+                             weight=None if uniform else weight, **getattr(orb, "geo_params", {}))

@@ -371,7 +371,22 @@ class FDMichelsonConstantEqualArm:
 
     def init_tdi_combination_model(self, detector):
         self.detector = detector
+        detector.tdi_response = None
+
+    def get_combination_jit(self, single_link_response):
+        """Three channels (A, E, T or X, Y, Z) from the six links (link12, link21, link23, link32,
+        link31, link13), each a complex (N,) tensor on the GPU (_tdi_jax.py:70-102)."""
+        import torch
+        links = torch.stack([torch.as_tensor(x) for x in single_link_response])
+        out = self.detector._tdi_from_links(links)
+        return tuple(out[i] for i in range(3))
 
     def update_tdi_response(self):
-        """The TDI combination is fused into the response kernel; nothing to do here."""
-        return None
+        """tdi.py:1705-1793: combine the detector's stored single links into ``tdi_response``."""
+        from collections import namedtuple
+        links = getattr(self.detector, "single_link_response", None)
+        if links is None:
+            raise ValueError("no single-link response to combine: update the single-link response first")
+        out = self.detector._tdi_from_links(links)
+        DataContent = namedtuple("DataContent", self.labels)
+        self.detector.tdi_response = DataContent(*[out[i] for i in range(3)])

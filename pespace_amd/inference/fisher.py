@@ -45,24 +45,64 @@ def _noiseless_core(core, row):
         resp = core.response(i, r)[0].cpu().numpy()
         data = np.where(np.asarray(d["mask"])[:, None] != 0, resp, 0.0)
         nc.add_detector(d["kind"], d["arm"], d["rot0"], d["rev0"], d["gen"], d["orth"], d["f"], data, d["psd"],
-                        d["mask"], df=d["df"], weight=d["weight"])
+                        d["mask"], df=d["df"], weight=d["weight"], omega_rotation=d["omega"],
+                        lambda_ref=d["lam"], beta_ref=d["bet"])
     return nc
 
 
+def _in_domain(row):
+    """The rows the kernel evaluates (the others give lnL = -inf): chirp mass and distance
+    positive, mass ratio in (0, 1], |chi| < 1."""
+    return row[0] > 0.0 and 0.0 < row[1] <= 1.0 and abs(row[2]) < 1.0 and abs(row[3]) < 1.0 and row[4] > 0.0
+
+
 def _minus_hessian(nc, row0, h):
-    """-d^2 lnL / d row_i d row_j at row0 from central differences of the adjoint gradient,
-    Richardson-combined over steps h and h / 2 (per coordinate)."""
+    """-d^2 lnL / d row_i d row_j at row0 from differences of the adjoint gradient,
+    Richardson-combined over steps h and h / 2 (per coordinate).
+
+    Central differences where row0 +- h stay in the domain; at its edge (e.g. an equal-mass
+    fiducial point, q = 1) the one-sided second-order stencil
+    D(h) = s (-3 g(0) + 4 g(s h) - g(2 s h)) / (2 h) pointing inwards (s = +-1)."""
     n = row0.size
-    rows = []
+    side = np.zeros(n)  # 0: central, +1 / -1: one-sided towards +h / -h
+    rows = [row0.copy()]
     for k in range(n):
-        for s in (h[k], -h[k], 0.5 * h[k], -0.5 * h[k]):
+        up, dn = row0.copy(), row0.copy()
+        up[k] += h[k]
+        dn[k] -= 2.0 * h[k]
+        ok_up, ok_dn = _in_domain(up), _in_domain(dn)
+        if not ok_up:
+            side[k] = -1.0
+        elif not ok_dn:
+            up2 = row0.copy()
+            up2[k] += 2.0 * h[k]
+            side[k] = 1.0 if _in_domain(up2) else 0.0
+        if side[k] == 0.0:
+            steps = (h[k], -h[k], 0.5 * h[k], -0.5 * h[k])
+        else:
+            sh = side[k] * h[k]
+            steps = (sh, 2.0 * sh, 0.5 * sh, sh)
+        for st in steps:
             r = row0.copy()
-            r[k] += s
+            r[k] += st
+            if not _in_domain(r):
+                raise ValueError(f"Fisher matrix: the fiducial {PARAM_NAMES[k]} = {row0[k]!r} lies too close to "
+                                 f"both ends of its domain for a step of {h[k]:.3g}")
             rows.append(r)
-    _, g = nc.lnl(np.array(rows), grad=True)
-    g = g.cpu().numpy().reshape(n, 4, n)
-    d1 = (g[:, 0] - g[:, 1]) / (2.0 * h[:, None])
-    d2 = (g[:, 2] - g[:, 3]) / h[:, None]
+    lnl, g = nc.lnl(np.array(rows), grad=True)
+    lnl = lnl.cpu().numpy()
+    g = g.cpu().numpy()
+    if not np.all(np.isfinite(lnl)):
+        bad = sorted({PARAM_NAMES[(i - 1) // 4] for i in np.nonzero(~np.isfinite(lnl))[0] if i > 0})
+        raise ValueError(f"Fisher matrix: lnL is not finite at the fiducial point or its shifts along {bad}")
+    g0 = g[0]
+    g = g[1:].reshape(n, 4, n)
+    hk = h[:, None]
+    cen = side == 0.0
+    d1 = np.where(cen[:, None], (g[:, 0] - g[:, 1]) / (2.0 * hk),
+                  side[:, None] * (-3.0 * g0[None, :] + 4.0 * g[:, 0] - g[:, 1]) / (2.0 * hk))
+    d2 = np.where(cen[:, None], (g[:, 2] - g[:, 3]) / hk,
+                  side[:, None] * (-3.0 * g0[None, :] + 4.0 * g[:, 2] - g[:, 3]) / hk)
     H = (4.0 * d2 - d1) / 3.0  # H[k, :] = d grad / d row_k
     F = -H
     return 0.5 * (F + F.T)

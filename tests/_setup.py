@@ -63,7 +63,7 @@ def make_detector(orbit="LISA_analytic", psd_model="LISA_SciRDv1", gen="1.5", T=
         channels = labels
     psd = odet.psd(psd_model, f, labels, False, gen)
     noise = odet.noise_realization(seed, labels, n_t, mask, psd, df)
-    orb = odet.ORBITS[orbit]
+    orb = odet.ORBITS[orbit] if isinstance(orbit, str) else tuple(orbit)
     d0 = olk.Detector(f, None, psd, df, orb, gen, channels, orthogonal)
     data = dict(noise)
     if inject:
@@ -73,9 +73,11 @@ def make_detector(orbit="LISA_analytic", psd_model="LISA_SciRDv1", gen="1.5", T=
 
 
 def add_to_core(core, d, labels):
-    kind, arm, rot0, rev0 = d.orbit
+    kind, arm, rot0, rev0 = d.orbit[:4]
+    geo = dict(zip(("omega_rotation", "lambda_ref", "beta_ref"), d.orbit[4:]))
     data = np.stack([d.data[c] for c in labels])
     psd = np.stack([d.psd[c] for c in labels])
     mask = [1 if c in d.channels else 0 for c in labels]
     gen = 15 if d.tdi_generation == "1.5" else 20
-    return core.add_detector(ORBIT_KIND[kind], arm, rot0, rev0, gen, d.orthogonal, d.f, data, psd, mask, df=d.df)
+    return core.add_detector(ORBIT_KIND[kind], arm, rot0, rev0, gen, d.orthogonal, d.f, data, psd, mask, df=d.df,
+                             **geo)

@@ -125,7 +125,9 @@ def test_hip_graph_capture_of_lnl_grad_step(precision):
     x = torch.as_tensor(batches[0], device="cuda").contiguous()
     lnl = torch.empty(P, dtype=torch.float64, device="cuda")
     g = torch.empty(P, 11, dtype=torch.float64, device="cuda")
-    core.lnl(x, grad=True, precision=precision, out=lnl, grad_out=g)  # sizes the scratch
+    core.reserve(P, grad=True)  # sizes the scratch, which stays pinned while the graph lives
+    core.pin_scratch(True)
+    core.lnl(x, grad=True, precision=precision, out=lnl, grad_out=g)
     torch.cuda.synchronize()
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
@@ -138,6 +140,12 @@ def test_hip_graph_capture_of_lnl_grad_step(precision):
         torch.cuda.synchronize()
         assert torch.equal(lnl, l_e)
         assert torch.equal(g, g_e)
+    # a larger batch would reallocate the scratch under the graph: rejected while pinned
+    from pespace_amd._lib import PespaceError
+    with pytest.raises(PespaceError, match="pinned"):
+        core.lnl(np.vstack(batches), grad=True, precision=precision)
+    core.pin_scratch(False)
+    del graph
 
 
 def test_schedules_agree_and_autotune_picks_one():
@@ -196,7 +204,7 @@ def test_invalid_rows_and_empty_batch():
     assert rc != 0 and b"precision" in lib().pes_last_error()
 
 
-@pytest.mark.parametrize("case", ["decreasing", "too_high", "psd", "generation", "orbit", "schedule"])
+@pytest.mark.parametrize("case", ["decreasing", "node_range", "psd", "generation", "orbit", "schedule"])
 def test_detector_validation_at_the_boundary(case):
     """Grids and settings the kernel cannot take are rejected by pes_context_add_detector (or
     pes_context_set_schedule) with a message, before any kernel runs; the context is released."""
@@ -209,9 +217,9 @@ def test_detector_validation_at_the_boundary(case):
     if case == "decreasing":
         f[10] = f[9]
         match = "strictly increasing"
-    elif case == "too_high":
-        f = np.linspace(0.05, 0.2, n)  # pi f L reaches 5.2 for 2.5e9 m arms
-        match = "pi f L"
+    elif case == "node_range":
+        f = np.linspace(0.05, 0.4, n)  # above the node range [1e-4, 0.3] Hz: no extrapolation
+        match = "outside the interpolation node range"
     elif case == "psd":
         psd[1, 7] = 0.0
         match = "PSD must be positive"

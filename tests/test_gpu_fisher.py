@@ -97,3 +97,19 @@ def test_fisher_distance_scaling_and_batch():
     assert Fb.shape == (2, 11, 11)
     np.testing.assert_allclose(Fb[0], F1, rtol=1e-12)
     np.testing.assert_allclose(Fb[1], F2, rtol=1e-12)
+
+
+def test_fisher_at_the_mass_ratio_edge():
+    """An equal-mass fiducial point (q = 1, the edge of the mass-ratio domain) takes a one-sided
+    stencil along q; the matrix is finite, positive definite and continuous with the one just
+    inside the domain (central stencils)."""
+    from pespace_amd.inference.fisher import fisher_matrix
+    _, lk = _setup()
+    p1 = dict(INJ, mass_ratio=1.0)
+    p0 = dict(INJ, mass_ratio=1.0 - 1e-5)
+    F1 = fisher_matrix(lk, p1)
+    F0 = fisher_matrix(lk, p0)
+    assert np.all(np.isfinite(F1))
+    assert np.all(np.linalg.eigvalsh(_normalised(F1)) > 0)
+    np.testing.assert_allclose(np.diag(F1), np.diag(F0), rtol=1e-3)
+    np.testing.assert_allclose(_normalised(F1), _normalised(F0), atol=1e-3)

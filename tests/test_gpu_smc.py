@@ -80,6 +80,7 @@ def test_smc_level_hip_graph_replays_match_eager():
                     covariance_regularization=REG)
     state = s.init_state()
     s.level(state, s.draw())  # one eager level: sizes every buffer, moves beta off 0
+    lk.core.pin_scratch(True)  # the captured level keeps the context's scratch pointers
     static_state = _clone(state)
     rand = s.draw()
     static_rand = {k: v.clone() for k, v in rand.items()}
@@ -108,6 +109,7 @@ def test_smc_level_hip_graph_replays_match_eager():
         for k in ("x", "logl", "logprior", "beta", "log_evidence", "accept", "delta"):
             assert torch.equal(static_state[k], eager[k]), (rep, k)
         state = eager
+    lk.core.pin_scratch(False)
 
 
 def test_smc_with_nuts_mutation_on_hip_likelihood():

@@ -59,6 +59,39 @@ def test_c_abi_argument_validation():
     assert L.pes_context_set_schedule(None, 32, 4) == 1
     t, b = ctypes.c_int(), ctypes.c_int()
     assert L.pes_context_get_schedule(None, ctypes.byref(t), ctypes.byref(b)) == 1
+    assert L.pes_links(None, 0, None, 1, None, None) == 1
+    assert L.pes_links_from_waveform(None, 0, 1, None, 0.0, 0.0, 0.0, 0.0, None, None) == 3
+    assert L.pes_tdi_from_links(None, 0, 1, None, None, None) == 3
+    assert L.pes_orbit_vectors(2, 2.5e9, 0.0, 0.0, 0.0, 0.0, 0.0, None, 1, None, None) == 1
+    assert L.pes_orbit_vectors(0, 2.5e9, 0.0, 0.0, 0.0, 0.0, 0.0, None, -1, None, None) == 1
+    assert L.pes_orbit_vectors(0, -1.0, 0.0, 0.0, 0.0, 0.0, 0.0, None, 0, None, None) == 1
+    assert b"arm length" in L.pes_last_error()
+    assert L.pes_context_reserve(None, 8, 1) == 1
+    assert L.pes_context_pin_scratch(None, 1) == 1
+
+
+def test_lnl_output_buffers_are_checked():
+    """Caller-supplied lnL / gradient buffers go to the kernels as raw pointers: wrong dtype,
+    device, shape or layout is rejected on the host."""
+    import torch
+
+    from pespace_amd.core import LikelihoodCore
+    dev = torch.device("cuda:0")
+    chk = LikelihoodCore._check_out
+    good = torch.empty(4, dtype=torch.float64)
+    with pytest.raises(ValueError, match="float64"):
+        chk(torch.empty(4, dtype=torch.float32), "out", (4,), good.device)
+    with pytest.raises(ValueError, match="must be on"):
+        chk(good, "out", (4,), dev)
+    with pytest.raises(ValueError, match="shape"):
+        chk(torch.empty(3, dtype=torch.float64), "out", (4,), good.device)
+    with pytest.raises(ValueError, match="shape"):
+        chk(torch.empty(4, 10, dtype=torch.float64), "grad_out", (4, 11), good.device)
+    with pytest.raises(ValueError, match="contiguous"):
+        chk(torch.empty(11, 4, dtype=torch.float64).t(), "grad_out", (4, 11), good.device)
+    with pytest.raises(TypeError):
+        chk(np.empty(4), "out", (4,), good.device)
+    chk(good, "out", (4,), good.device)
 
 
 def test_product_grid_psd_noise_match_oracle():
@@ -114,7 +147,15 @@ def test_antenna_and_orbit_errors():
     with pytest.raises(ValueError):
         KaplerianHeliocentric(2.5e9, backend="jax")
     with pytest.raises(ValueError):
-        KeplerianGeocentric(1.7e8, omega_rotation=1e-5)
+        KeplerianGeocentric(1.7e8, backend="taichi")
+    # orbit_params follow orbit.py:112-139 / 189-228; any geocentric rotation and reference source
+    geo = KeplerianGeocentric(1.7e8, 0.4, 1.1, omega_rotation=1e-5, lambda_ref=0.7, beta_ref=0.3)
+    op = geo.orbit_params
+    assert op.omega_rotation == 1e-5 and op.slam_ref == np.sin(0.7) and op.cbeta_ref == np.cos(0.3)
+    assert op.r_det_sec == op.static_arm_sec / np.sqrt(3) and geo.arm_length_sec(5.0) == 1.7e8 / 299792458.0
+    hel = KaplerianHeliocentric(2.5e9, 0.3, -0.2)
+    assert hel.orbit_params.r_prime == 2.5e9 / 299792458.0 / (2 * np.sqrt(3))
+    assert abs(hel.orbit_params.ck3 - np.cos(0.3 + 4 * np.pi / 3)) < 1e-15
     td = TDIChannelData()
     with pytest.raises(ValueError):
         InterferometerAntenna("x", td, "nope", FDResponseModelMarset2018(), FDMichelsonConstantEqualArm())
