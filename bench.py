@@ -7,7 +7,10 @@ precision modes timed back to back.  ``value`` is the rate of the fastest mode
 whose |dlnL| against the FP64 path stays below 1e-3 on every row (FP64 itself
 when the mixed mode does not); both rates and the mixed mode's |dlnL| and
 gradient error against the FP64 path are reported beside it (gradient errors
-relative to the largest |grad| of each parameter over the batch).
+relative to the largest |grad| of each parameter over the batch).  On the
+network workloads the default run adds the mixed-precision configuration:
+uneven batches of 256, 3001 and 8192 rows from the wide prior, both modes
+(``mixed_prior_batches``).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload network_hm_1yr|lisa_22_1yr]
                     [--no-grad] [--particles P] [--precision all|fp64|mixed]
@@ -122,17 +125,17 @@ def prior_cloud(P, seed=2027):
 
 
 def particle_cloud(P, seed=2026, rel=1e-3):
-    """Gaussian of width rel*|theta| around the injection; angles wrapped (fisher.py:297-310)."""
+    """Gaussian of width rel*|theta| around the injection; angles wrapped by the reference's
+    wrap_angle_params (fisher.py:235-311: longitude, phase mod 2 pi, polarization mod pi,
+    latitude and inclination folded by reflection); the mass ratio kept in (0, 1]."""
     from pespace_amd.core import PARAM_NAMES
+    from pespace_amd.inference.fisher import wrap_angle_params
     c = np.array([INJ[k] for k in PARAM_NAMES])
     rng = np.random.default_rng(seed)
     rows = c + rel * np.abs(c) * rng.standard_normal((P, c.size))
     rows[:, 1] = np.clip(rows[:, 1], 1e-3, 1.0)
-    rows[:, 5] = np.mod(rows[:, 5], 2 * PI)
-    rows[:, 7] = np.mod(rows[:, 7], 2 * PI)
-    rows[:, 9] = np.mod(rows[:, 9], PI)
-    rows[:, 8] = np.clip(rows[:, 8], -PI / 2, PI / 2)
-    return rows
+    w = wrap_angle_params({k: rows[:, i] for i, k in enumerate(PARAM_NAMES)})
+    return np.column_stack([np.asarray(w[k], dtype=np.float64) for k in PARAM_NAMES])
 
 
 def time_steps(core, params, steps, warmup, grad, precision):
@@ -154,6 +157,32 @@ def time_steps(core, params, steps, warmup, grad, precision):
     core.set_profiling(False)
     return dict(dt=dt, kernel_ms=kms / max(n, 1), bm=bm, br=br, lnl=lnl.cpu().numpy(),
                 grad=(g.cpu().numpy() if grad else None))
+
+
+def prior_sweep(core, sizes=(256, 3001, 8192), seed=2027):
+    """The mixed-precision configuration: uneven batches from the wide prior (t_c anywhere in the
+    year, total mass 1e5-1e7 Msun) on the same network, lnL + gradient in both modes (one timed
+    step after one warm-up each); per batch the rates, |dlnL| of mixed against FP64 (max, p99),
+    the gradient error relative to the batch's largest |g| per parameter (max, p99)."""
+    import torch
+    out = []
+    for P in sizes:
+        params = torch.as_tensor(prior_cloud(P, seed=seed + P), device="cuda")
+        r = {prec: time_steps(core, params, 1, 1, True, prec) for prec in ("fp64", "mixed")}
+        d = np.abs(r["mixed"]["lnl"] - r["fp64"]["lnl"])
+        g64, gmx = r["fp64"]["grad"], r["mixed"]["grad"]
+        rel = np.abs(gmx - g64) / np.maximum(np.abs(g64).max(axis=0), 1e-300)
+        e = {"particles": P, "fp64_evals_per_s": P / r["fp64"]["dt"], "mixed_evals_per_s": P / r["mixed"]["dt"],
+             "over_fp64": r["fp64"]["dt"] / r["mixed"]["dt"], "dlnl_max": float(d.max()),
+             "dlnl_p99": float(np.percentile(d, 99)), "grad_rel_err_max": float(rel.max()),
+             "grad_rel_err_p99": float(np.percentile(rel, 99)),
+             "finite": bool(np.all(np.isfinite(r["fp64"]["lnl"])) and np.all(np.isfinite(g64))
+                            and np.all(np.isfinite(gmx)))}
+        print(f"[bench] prior batch P={P}: fp64 {e['fp64_evals_per_s']:.1f}/s, mixed {e['mixed_evals_per_s']:.1f}/s, "
+              f"|dlnl| max {e['dlnl_max']:.2e}, grad rel max {e['grad_rel_err_max']:.2e}", file=sys.stderr,
+              flush=True)
+        out.append(e)
+    return out
 
 
 def traffic_of(workload, precision, grad):
@@ -250,6 +279,8 @@ def main():
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--precision", default="all", choices=("all", "fp64", "mixed"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-prior-sweep", action="store_true",
+                    help="skip the mixed vs FP64 sweep over uneven wide-prior batches (256, 3001, 8192 rows)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule of the fused kernel")
     ap.add_argument("--schedule", default=None, help="fixed schedule 'tiles_per_group,blocks_per_cu' (no autotune)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -334,7 +365,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == "fp64" else "f64+f32trig",
+        "dtype": "f64" if head == "fp64" else "f64 (lnL), f64+f32 sin/cos/rcp (adjoint)",
         "data": "synthetic (seeded noise + injected signal, random particle cloud)",
         "config": {"workload": args.workload, "particles": P, "bins_per_detector": nbins, "modes": list(wf.modes),
                    "nodes": wf.n_nodes, "gradient": grad, "precision": head,
@@ -343,8 +374,8 @@ def main():
                      "frac": achieved / PEAK_FP64, "traffic": traffic,
                      "work": {"bin_mode_rows": h["bm"], "bin_rows": h["br"], "flops": flops},
                      "note": "algorithmic FP64 flops of the fused lnL(+grad) kernel over its HIP-event time; "
-                             "sines/cosines/reciprocals not counted as flops; all arithmetic is FP64 in every "
-                             "precision mode"},
+                             "sines/cosines/reciprocals not counted as flops; all algebra is FP64 in every "
+                             "precision mode (mixed takes FP32 sin/cos/rcp in the adjoint only)"},
     }
     for prec in precs:
         line[f"{prec}_evals_per_s"] = K * P / res[prec]["dt"]
@@ -363,6 +394,9 @@ def main():
                 acc["grad_rel_err_max"] = float(np.max(rel))
                 acc["grad_rel_err_p99"] = float(np.percentile(rel, 99))
             line[f"{prec}_vs_fp64"] = acc
+    if grad and not args.no_prior_sweep and WORKLOADS[args.workload]["detectors"] == ("LISA", "Taiji", "TianQin") \
+            and WORKLOADS[args.workload]["grid"] == "year":
+        line["mixed_prior_batches"] = prior_sweep(core)
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(dets, wf, rows)

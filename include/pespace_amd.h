@@ -39,9 +39,12 @@ extern "C" {
 #define PES_P_POLARIZATION 9
 #define PES_P_COALESCENCE_TIME 10   /* seconds                       */
 
-/* precision modes of pes_lnl_batch / pes_response.  All arithmetic and accumulation is FP64;
- * PES_PREC_MIXED takes the sine/cosine of the total phase Psi + 2 pi f (t_c + k.x0) (reduced in
- * FP64) from the FP32 hardware units.  Any other value is rejected with PES_ERR_INVALID. */
+/* precision modes of pes_lnl_batch / pes_response.  lnL and the response are the FP64 forward in
+ * both modes (identical results).  PES_PREC_MIXED changes the gradient only: the adjoint's
+ * recompute of each mode takes the pair phasors and the total-phase phasor (phase reduced in
+ * FP64) from the FP32 hardware sine/cosine and the sinc denominators from the FP32 reciprocal,
+ * so the gradient carries FP32-level relative errors (~1e-7).  Any other value is rejected
+ * with PES_ERR_INVALID. */
 #define PES_PREC_FP64 0
 #define PES_PREC_MIXED 1
 

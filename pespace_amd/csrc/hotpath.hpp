@@ -1,25 +1,24 @@
 // Fused per-(bin, row) forward response and adjoint.
 //
-// Arithmetic is FP64 in every precision mode; the two modes differ only in where the sine and
-// cosine of the total phase come from (PREC template parameter, PES_PREC_* values):
-//   0 FP64: the phasor of the total phase Psi + 2 pi f (t_c + k.x0) (the
-//     waveform phase, the t_c shift and the barycentric delay, antenna.py:435, 530-533, which
-//     the reference applies as three factors) from one table sincos of 2 pi f t_c (reduced
-//     once per bin and row) plus Psi + 2 pi f k.x0 (reduced per mode), the remainders added:
-//     the total is never rounded and each term keeps the rounding it has as a separate factor.
-//   1 MIXED: the phasor of the total phase Psi + 2 pi f (t_c + k.x0) (summed and reduced mod
-//     2 pi in FP64) comes from the FP32 hardware sine/cosine (v_sin_f32, v_cos_f32).  Its
-//     error changes from bin to bin (the phase moves by ~2 pi t df per bin), so it averages
-//     out of lnL.
-// Every other sine and cosine (orbit angles, pair phasors) comes from the table sincos and every
-// reciprocal from rcp_d in both modes.  FP32 there does not pay: the orbit angle's error is
-// multiplied by 2 pi f AU/c (~150 rad) in the delay phase, and the transfer-function
-// arguments vary slowly with f, so FP32 errors there are coherent over many bins and bias
-// |h|^2 (|dlnL| ~ 1 far from the peak), while FP32 sines cost about as many issue cycles as
-// the table sincos.  Without packed FP32, an FP32 FMA issues at the FP64 rate on gfx950, so
-// FP32 pays only in the transcendental units; FP64 algebra also keeps per-row constants
-// (polarization factors, 1/d_L, node values) from carrying an FP32 rounding that would bias
-// every bin of a row the same way.
+// The forward response, the residual and lnL are FP64 in every precision mode (PREC template
+// parameter, PES_PREC_* values), so lnL does not depend on the mode.  The modes differ in the
+// adjoint's recompute of each mode's response (the gradient only):
+//   0 FP64: every sine and cosine from the FP64 table sincos, every reciprocal from rcp_d; the
+//     phasor of the total phase Psi + 2 pi f (t_c + k.x0) (the waveform phase, the t_c shift and
+//     the barycentric delay, antenna.py:435, 530-533, which the reference applies as three
+//     factors) from one table sincos of 2 pi f t_c (reduced once per bin and row) plus
+//     Psi + 2 pi f k.x0 (reduced per mode), the remainders added: the total is never rounded and
+//     each term keeps the rounding it has as a separate factor.
+//   1 MIXED: in the adjoint's recompute, the pair phasors and the total-phase phasor (its phase
+//     summed and reduced in FP64) come from the FP32 hardware sine/cosine and the sinc
+//     denominators from the FP32 reciprocal; the orbit angle keeps the FP64 table sincos (its
+//     error is multiplied by 2 pi f AU/c ~ 150 rad in the delay phase) and all algebra stays
+//     FP64.  The gradient then carries FP32-level relative errors (~1e-7).
+// Why FP32 goes no further: without packed FP32 (off by build rule) an FP32 FMA issues at the
+// FP64 rate on gfx950, and the mode loops are ~90% FMA/add/mul, so FP32 pays only in the
+// transcendental units; an FP32 forward would bias lnL coherently (per-row constants and the
+// slowly varying transfer functions round the same way over many bins: |dlnL| ~ eps SNR^2,
+// ~0.4 far from the peak), far outside the |dlnL| < 1e-3 bound.
 //
 // Algebraic rearrangements relative to antenna.py:496-543 (same values):
 //   * the constellation offsets sum to zero, so k.(x_i + x_j) = 2 k.x0 - k.xd_a for link
@@ -57,7 +56,9 @@ __device__ inline double dot3(const V3& a, const V3& b) { return fma(a.x, b.x, f
 
 template <int PREC>
 struct Trig {
-  static constexpr bool kFp32Phase = PREC >= 1;  // one FP32 sincos of the total phase
+  // FP32 hardware sine/cosine for the pair phasors and the total phase, FP32 reciprocals for the
+  // sinc denominators (the adjoint's recompute in the mixed mode; the forward is always PREC 0)
+  static constexpr bool kFp32 = PREC >= 1;
 };
 
 // per-bin quantities shared by every row and mode of a tile lane (kept small: they stay in
@@ -223,16 +224,20 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
   }
   {
     double s0, c0, s1, c1;
-    sincos_tab_s(b.pif * kx_0, &s0, &c0);
-    sincos_tab_s(b.pif * kx_1, &s1, &c1);
+    if constexpr (Trig<PREC>::kFp32) {
+      sincos_f32(b.pif * kx_0, &s0, &c0);
+      sincos_f32(b.pif * kx_1, &s1, &c1);
+    } else {
+      sincos_tab_s(b.pif * kx_0, &s0, &c0);
+      sincos_tab_s(b.pif * kx_1, &s1, &c1);
+    }
     g.z0 = {c0, s0};
     g.z1 = {c1, s1};
     g.z01 = mul(g.z0, g.z1);
   }
   double sp, cp;
-  if constexpr (Trig<PREC>::kFp32Phase) {
-    // unit modulus matters: a mean modulus error of the FP32 pair would scale every bin's
-    // |h|^2 the same way; the remaining error is a phase error that changes from bin to bin
+  if constexpr (Trig<PREC>::kFp32) {
+    // the total phase summed and reduced to revolutions in FP64, its sine and cosine in FP32
     sincos_rev_fc(fma(2.0 * b.pif, rc.tc + g.kx0, Psi), &sp, &cp);
   } else {
     // 2 pi f t_c (rounded once, as the reference's own factor, and reduced once per bin and
@@ -248,6 +253,14 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
 }
 
 // sinc(xL +- a) with a = xL k.n and their derivatives (cos a, sin a given)
+template <int PREC>
+__device__ inline double rcp_prec(double x) {
+  if constexpr (Trig<PREC>::kFp32)
+    return rcp_f32(x);
+  else
+    return rcp_d(x);
+}
+template <int PREC>
 __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, double ca, double sa, double& sp,
                                  double& sm, double& dsp, double& dsm) {
   const double a = xL * kn;
@@ -257,7 +270,7 @@ __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, d
   const double thr = 1e-2;
   const bool small_p = fabs(xp) < thr, small_m = fabs(xm) < thr;
   if (!small_p && !small_m) {
-    const double inv = rcp_d(xp * xm);
+    const double inv = rcp_prec<PREC>(xp * xm);
     const double ip = xm * inv, im = xp * inv;
     sp = sinp * ip;
     dsp = (cosp - sp) * ip;
@@ -269,7 +282,7 @@ __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, d
       sp = 1.0 - x2 * (1.0 / 6.0 - x2 * (1.0 / 120.0 - x2 * (1.0 / 5040.0)));
       dsp = -xp * (1.0 / 3.0 - x2 * (1.0 / 30.0 - x2 * (1.0 / 840.0)));
     } else {
-      const double ip = rcp_d(xp);
+      const double ip = rcp_prec<PREC>(xp);
       sp = sinp * ip;
       dsp = (cosp - sp) * ip;
     }
@@ -278,7 +291,7 @@ __device__ inline void sinc_pair(double xL, double sxL, double cxL, double kn, d
       sm = 1.0 - x2 * (1.0 / 6.0 - x2 * (1.0 / 120.0 - x2 * (1.0 / 5040.0)));
       dsm = -xm * (1.0 / 3.0 - x2 * (1.0 / 30.0 - x2 * (1.0 / 840.0)));
     } else {
-      const double im = rcp_d(xm);
+      const double im = rcp_prec<PREC>(xm);
       sm = sinm * im;
       dsm = (cosm - sm) * im;
     }
@@ -317,7 +330,7 @@ __device__ inline void pair_eval(const DetectorDev& det, const BinPre& b, const 
     w = mul(g.z1, conjx(g.z0));
   }
   v.C = mul(nhn, v.z);
-  sinc_pair(b.xL, b.sxL, b.cxL, v.kn, w.re, w.im, v.sp, v.sm, v.dsp, v.dsm);
+  sinc_pair<PREC>(b.xL, b.sxL, b.cxL, v.kn, w.re, w.im, v.sp, v.sm, v.dsp, v.dsm);
 }
 
 struct LinksT {
@@ -709,7 +722,7 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
       TdiC tdc;
       tdi_consts(det, b, tdc);
       Cx out[3];
-      response_row<PREC, KIND>(det, pc, modes, nb, nstride, b, rc, tdc, out);
+      response_row<0, KIND>(det, pc, modes, nb, nstride, b, rc, tdc, out);  // FP64 in every mode
       // the forward-only kernel (168 registers) requests the record here instead: held across
       // its mode loop, the record's registers were spilled (measured 0.9% slower)
       if constexpr (!GRAD) load_rec();

@@ -138,6 +138,17 @@ __device__ inline double rcp_d(double x) {
   return fma(r, e, r);
 }
 
+// Hardware FP32 sine/cosine of a small FP64 argument (|x| well below 256 revolutions; v_sin_f32
+// / v_cos_f32 take revolutions), widened to FP64: absolute error of a few 1e-7.
+__device__ inline void sincos_f32(double x, double* s, double* c) {
+  const float r = (float)(x * kInv2Pi);
+  *s = (double)__builtin_amdgcn_sinf(r);
+  *c = (double)__builtin_amdgcn_cosf(r);
+}
+
+// Hardware FP32 reciprocal widened to FP64 (relative error ~1 FP32 ulp).
+__device__ inline double rcp_f32(double x) { return (double)__builtin_amdgcn_rcpf((float)x); }
+
 // FP64 argument reduced to revolutions in FP64, then the hardware FP32 sine/cosine
 // (v_sin_f32 / v_cos_f32 take revolutions), with the rounding of the argument to FP32 undone
 // to first order and the pair brought back to unit modulus by one Newton step: what remains

@@ -531,10 +531,9 @@ hipError_t launch_lnl_fwd(int precision, int grid, hipStream_t st, PES_LNL_PARAM
 
 #ifdef PES_LNL_FWD_UNIT
 hipError_t pes::launch_lnl_fwd(int precision, int grid, hipStream_t st, PES_LNL_PARAMS) {
-  if (precision == 0)
-    hipLaunchKernelGGL((k_lnl<0, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES);
-  else
-    hipLaunchKernelGGL((k_lnl<1, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES);
+  // lnL alone is the FP64 forward in every precision mode (the modes differ in the adjoint only)
+  (void)precision;
+  hipLaunchKernelGGL((k_lnl<0, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES);
   return hipGetLastError();
 }
 #else
@@ -1004,10 +1003,8 @@ extern "C" int pes_response(pes_context* ctx, int det, const double* d_params, i
 #define PES_RESP_ARGS                                                                                          \
   dim3((n + 127) / 128, P), dim3(128), 0, st, ctx->d_dets, det, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s,     \
       ctx->d_pconst, ctx->d_nodes, ctx->n_nodes + 1, ctx->du, ctx->modes, P, d_out
-  if (precision == PES_PREC_FP64)
-    hipLaunchKernelGGL(k_response_t<0>, PES_RESP_ARGS);
-  else
-    hipLaunchKernelGGL(k_response_t<1>, PES_RESP_ARGS);
+  // the response is the FP64 forward in every precision mode
+  hipLaunchKernelGGL(k_response_t<0>, PES_RESP_ARGS);
 #undef PES_RESP_ARGS
   HIP_TRY(hipGetLastError());
   return PES_OK;

@@ -81,3 +81,44 @@ def add_to_core(core, d, labels):
     gen = 15 if d.tdi_generation == "1.5" else 20
     return core.add_detector(ORBIT_KIND[kind], arm, rot0, rev0, gen, d.orthogonal, d.f, data, psd, mask, df=d.df,
                              **geo)
+
+
+YEAR_SI = 31557600.0
+NETWORK = (("LISA", "LISA_analytic", "LISA_SciRDv1", 1e-4, 0.05),
+           ("Taiji", "Taiji_analytic", "Taiji_TDC", 1e-4, 0.06),
+           ("TianQin", "Tianqin_analytic", "Tianqin_GWSpace", 1e-3, 0.039))
+
+
+def network_core(modes=("22", "21", "33", "32", "44", "43"), T=YEAR_SI, dt=5.0, seed=7, n_nodes=1024,
+                 inj=None):
+    """LISA + Taiji + TianQin on rfft grids of duration T (ragged: 1,574,725 / 1,890,301 /
+    1,199,189 bins for one year at dt = 5 s), seeded noise plus the injection, through the
+    product's host API (the data of the benchmark configurations).  Returns the core."""
+    from pespace_amd.core import LikelihoodCore
+    from pespace_amd.detector.antenna import (FDResponseModelMarset2018, InterferometerAntenna,
+                                              add_detector_to_core)
+    from pespace_amd.detector.orbit import available_orbit_models
+    from pespace_amd.detector.tdi import FDMichelsonConstantEqualArm, TDIChannelData
+    from pespace_amd.waveform.phenom import IMRPhenomHM
+    inj = dict(INJ, coalescence_time=0.8 * T) if inj is None else inj
+    fmin = min(n[3] for n in NETWORK)
+    fmax = max(n[4] for n in NETWORK)
+    dets = []
+    for i, (name, orbit, psd, lo, hi) in enumerate(NETWORK):
+        td = TDIChannelData(backend="hip")
+        td.set_fd_data_from_zero(("A", "E", "T"), T, dt, 0.0, lo, hi)
+        td.set_fd_noise_power_density_from_model(psd, tdi_generation="1.5")
+        det = InterferometerAntenna(name, td, available_orbit_models()[orbit], FDResponseModelMarset2018(),
+                                    FDMichelsonConstantEqualArm("1.5", True))
+        dets.append(det)
+    wf = IMRPhenomHM(dets[0].tdi_data.frequency_samples, 0.0, modes=modes, n_nodes=n_nodes, f_node_min=fmin,
+                     f_node_max=fmax)
+    wf.update_waveform(inj)
+    for i, det in enumerate(dets):
+        det.tdi_data.add_into_fd_data(det.tdi_data.get_fd_noise_realization(seed + i))
+        det.inject_signal(wf.waveform_container, inj["ecliptic_longitude"], inj["ecliptic_latitude"],
+                          inj["polarization"], inj["coalescence_time"])
+    core = LikelihoodCore(modes=wf.modes, n_nodes=n_nodes, f_node_min=fmin, f_node_max=fmax)
+    for det in dets:
+        add_detector_to_core(core, det, data_channels=("A", "E", "T"))
+    return core

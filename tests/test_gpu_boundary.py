@@ -54,7 +54,7 @@ def test_constellation_vectors(orbit):
     ref = odet.constellation(*spec[:4], t, *spec[4:])
     # the rotation / revolution angle omega t + phi0 is rounded once (ulp ~ 1e-13 at ~600 rad):
     # the link vectors agree to a few ulp of the largest angle
-    omega = spec[4] if spec[0] == "geo" else 2 * np.pi / 31557600.0
+    omega = (spec[4] if len(spec) > 4 else odet.GEO_OMEGA_ROT) if spec[0] == "geo" else 2 * np.pi / 31557600.0
     ang = omega * np.abs(t).max() + 2 * np.pi
     for name, got, r in zip(v._fields, v, ref):
         got = got.cpu().numpy()
