@@ -381,9 +381,19 @@ def _m1m2(chirp_mass, mass_ratio):
 
 
 class PhenomHM:
-    """22 = PhenomD; higher modes by the London et al. frequency-map construction."""
+    """22 = PhenomD; higher modes by the London et al. frequency-map construction.
 
-    def __init__(self, chirp_mass, mass_ratio, chi1, chi2, f_ref, modes=((2, 2),)):
+    construction="c1" (the model the HIP kernels implement): C1 cubic frequency map between
+    f_i and the ringdown, PN amplitude-ratio velocity saturating smoothly above f_i, and a C1
+    raised-cosine taper over the last 10% below every mode's cut, (2,2) included.
+    construction="published" (London et al. 2018 as published, for comparison only):
+    piecewise-linear frequency map, amplitude ratio frozen at f_i, no taper (hard cuts).
+    """
+
+    def __init__(self, chirp_mass, mass_ratio, chi1, chi2, f_ref, modes=((2, 2),), construction="c1"):
+        if construction not in ("c1", "published"):
+            raise ValueError(construction)
+        self.construction = construction
         m1, m2 = _m1m2(chirp_mass, mass_ratio)
         self.pd = PhenomD(m1, m2, chi1, chi2)
         pd = self.pd
@@ -413,9 +423,12 @@ class PhenomHM:
         t = (Mf - fi) / w
         P0, P1 = HM_MAP_JOIN, fRD22
         M0, M1 = (2.0 / m) * w, rho * w
-        c2 = 3.0 * (P1 - P0) - 2.0 * M0 - M1
-        c3 = 2.0 * (P0 - P1) + M0 + M1
-        cub = P0 + t * (M0 + t * (c2 + t * c3))
+        if self.construction == "published":  # linear between (f_i, 0.014) and (fRDlm, fRD22)
+            cub = P0 + t * (P1 - P0)
+        else:
+            c2 = 3.0 * (P1 - P0) - 2.0 * M0 - M1
+            c3 = 2.0 * (P0 - P1) + M0 + M1
+            cub = P0 + t * (M0 + t * (c2 + t * c3))
         fv = du.val(Mf)
         return du.where(fv < fi, (2.0 / m) * Mf, du.where(fv >= fr, rho * Mf, cub))
 
@@ -429,8 +442,11 @@ class PhenomHM:
         fv = du.val(Mf)
         vraw = (2.0 * PI * Mf / m) ** (1.0 / 3.0)
         vi = (2.0 * PI * fi / m) ** (1.0 / 3.0)
-        # C1 saturation above f_i: v = vi (2 - exp(1 - vraw/vi)) (slope 1 at the joint, -> 2 vi)
-        v = du.where(fv < fi, vraw, vi * (2.0 - du.exp(1.0 - vraw / vi)))
+        if self.construction == "published":  # frozen at f_i
+            v = du.where(fv < fi, vraw, vi + 0.0 * vraw)
+        else:
+            # C1 saturation above f_i: v = vi (2 - exp(1 - vraw/vi)) (slope 1 at the joint, -> 2 vi)
+            v = du.where(fv < fi, vraw, vi * (2.0 - du.exp(1.0 - vraw / vi)))
         d, eta = pd.delta, pd.eta
         if mode == (2, 1):
             return np.sqrt(2.0) / 3.0 * (d * v - 1.5 * (pd.chia + d * pd.chis) * v * v)
@@ -461,10 +477,12 @@ class PhenomHM:
         A = self.beta(mode, Mf) * self.pd.amplitude(f22) * self.amp_scale
         x = Mf / self.mode_cut(mode)
         xv = du.val(x)
+        P = (0.5 * m) * self.psi22(f22)
+        if self.construction == "published":
+            return A, P
         x0 = 1.0 - TAPER
         arg = (x - x0) * (PI / TAPER)
         w = du.where(xv <= x0, 1.0, du.where(xv >= 1.0, 0.0, 0.5 * (1.0 + du.cos(arg))))
-        P = (0.5 * m) * self.psi22(f22)
         return A * w, P
 
 
@@ -516,14 +534,15 @@ def harmonic_factors(mode, iota, phi_ref):
     return kp, kc
 
 
-def waveform_direct(params, f, modes=((2, 2),), f_ref=0.0):
+def waveform_direct(params, f, modes=((2, 2),), f_ref=0.0, construction="c1"):
     """Reference-style waveform container at frequencies f (Hz), t_c = 0.
 
     params: chirp_mass, mass_ratio, chi_1, chi_2, luminosity_distance [Mpc],
             reference_phase, inclination.
     Returns {mode: {"plus","cross","tf"}} (or the single dict if one mode).
     """
-    hm = PhenomHM(params["chirp_mass"], params["mass_ratio"], params["chi_1"], params["chi_2"], f_ref, modes)
+    hm = PhenomHM(params["chirp_mass"], params["mass_ratio"], params["chi_1"], params["chi_2"], f_ref, modes,
+                  construction)
     dist = params["luminosity_distance"] * 1e6 * PC_SI
     f = np.asarray(f, dtype=np.float64)
     out = {}
@@ -574,9 +593,10 @@ def hermite_coeffs(v, dv, du_):
     return np.stack([P0, M0, c2, c3], axis=-1)
 
 
-def waveform_interpolated(params, f, modes, f_ref, fmin_nodes, fmax_nodes, n_nodes):
+def waveform_interpolated(params, f, modes, f_ref, fmin_nodes, fmax_nodes, n_nodes, construction="c1"):
     """Waveform container from cubic-Hermite interpolation of node tables."""
-    hm = PhenomHM(params["chirp_mass"], params["mass_ratio"], params["chi_1"], params["chi_2"], f_ref, modes)
+    hm = PhenomHM(params["chirp_mass"], params["mass_ratio"], params["chi_1"], params["chi_2"], f_ref, modes,
+                  construction)
     dist = params["luminosity_distance"] * 1e6 * PC_SI
     f = np.asarray(f, dtype=np.float64)
     u0, du_ = node_grid(fmin_nodes, fmax_nodes, n_nodes)

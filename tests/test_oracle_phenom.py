@@ -207,3 +207,24 @@ def test_qnm_interpolation_is_exact_on_grid_and_smooth():
     w = qnm.qnm_branch(3, 3, [0.3025])[0] / (2 * PI)
     assert abs(qnm.interp_table(vals, 0.3025) - w.real) < 1e-8
     assert np.all(np.diff(qnm.interp_table(vals, mid)) > 0)
+
+
+def test_hm_c1_construction_changes_nothing_below_the_joints():
+    """The product's C1 construction departs from London et al.'s published one (piecewise-linear
+    map, ratio frozen at f_i, no taper) only above each mode's inspiral joint f_i = (m/2) 0.014 / M
+    (the (2,2) only inside its taper, the last 10% below its cut): below, the two agree to
+    rounding; above, the higher modes differ (the deviation is measured, not asserted small)."""
+    p = _params(chirp_mass=1.2e6, mass_ratio=0.6, chi_1=0.75, chi_2=0.62)
+    f = np.geomspace(1e-5, 5e-2, 4000)
+    a = ph.waveform_direct(p, f, ph.MODES, 0.0, construction="c1")
+    b = ph.waveform_direct(p, f, ph.MODES, 0.0, construction="published")
+    hm = ph.PhenomHM(p["chirp_mass"], p["mass_ratio"], p["chi_1"], p["chi_2"], 0.0, ph.MODES)
+    Mf = f * hm.Msec
+    for (l, m) in ph.MODES:
+        key = f"{l}{m}"
+        below = Mf < (0.5 * m * ph.HM_MAP_JOIN if (l, m) != (2, 2) else (1 - ph.TAPER) * ph.MF_CUT)
+        np.testing.assert_allclose(a[key]["plus"][below], b[key]["plus"][below], rtol=1e-12, atol=0, err_msg=key)
+        np.testing.assert_allclose(a[key]["tf"][below], b[key]["tf"][below], rtol=1e-12, atol=0, err_msg=key)
+        if (l, m) != (2, 2):
+            above = ~below & (np.abs(b[key]["plus"]) > 0)
+            assert np.max(np.abs(a[key]["plus"][above] - b[key]["plus"][above])) > 1e-3 * np.abs(b[key]["plus"]).max()
