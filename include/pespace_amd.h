@@ -15,6 +15,8 @@
 #ifndef PESPACE_AMD_H
 #define PESPACE_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -99,6 +101,22 @@ void pes_context_destroy(pes_context* ctx);
  * stream: hipStream_t (NULL = default stream). */
 int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, double* d_lnl, double* d_grad, int precision,
                   void* stream);
+
+/* pes_lnl_batch without any device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate (code under HIP-graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results as pes_lnl_batch, but the per-batch scratch
+ * (work counter, per-row setup, node tables and their gradients, per-tile-group partials) lives
+ * in a fixed array of PES_NOALLOC_SCRATCH_BYTES that every GPU holds once for the whole process;
+ * pes_lnl_batch_noalloc_bytes(ctx, P, grad) gives what a batch needs (0 for a bad argument) and
+ * a batch that needs more is rejected with PES_ERR_INVALID before anything is launched.  Every
+ * call on a GPU shares that GPU's array and leaves its contents behind (each call rewrites all it
+ * reads): calls that overlap on one GPU race, so serialise them (one stream, or events).  The
+ * context's profiling (pes_context_set_profiling) does not apply to this entry point.
+ * Example: six modes, 1024 nodes, a 1-yr LISA grid and the gradient take ~1.2 MB per row. */
+#define PES_NOALLOC_SCRATCH_BYTES (64u << 20)
+int pes_lnl_batch_noalloc(pes_context* ctx, const double* d_params, int P, double* d_lnl, double* d_grad,
+                          int precision, void* stream);
+size_t pes_lnl_batch_noalloc_bytes(const pes_context* ctx, int P, int grad);
 
 /* Detector response h_c(f) for P rows on detector det: device out [P][3][n_bins][2]
  * (re, im), computed by the same device code as pes_lnl_batch at the given

@@ -60,6 +60,10 @@ SIGNATURES = [
       ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     ("pes_context_reserve", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("pes_context_pin_scratch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("pes_lnl_batch_noalloc", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+      ctypes.c_void_p]),
+    ("pes_lnl_batch_noalloc_bytes", ctypes.c_size_t, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
 ]
 
 

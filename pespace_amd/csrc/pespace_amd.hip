@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -713,6 +714,50 @@ extern "C" int pes_context_add_detector(pes_context* ctx, int orbit_kind, double
   return PES_OK;
 }
 
+static int n_groups_of(const pes_context* ctx) {
+  return (ctx->n_tiles + ctx->tiles_per_group - 1) / ctx->tiles_per_group;
+}
+
+// ---- the entry point that allocates nothing ---------------------------------------------------
+// Its scratch (work counter, per-row setup and constants, node tables, per-tile-group partials)
+// lives in this file-scope device array, of which every GPU holds its own copy until the process
+// ends.  Host code never uses the array's name as an address: its device address is taken once
+// per device with hipGetSymbolAddress (at pes_context_finalize, with the device current) and
+// kept in g_noalloc_base.
+static __device__ __align__(256) unsigned char scratch[PES_NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static char* g_noalloc_base[NOALLOC_MAX_DEVICES] = {};
+
+// the device address of `scratch` on `device` (which must be current)
+static int noalloc_base(int device, char** out) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(PES_ERR_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_base[device]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_noalloc_base[device] = static_cast<char*>(p);
+  }
+  *out = g_noalloc_base[device];
+  return PES_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// byte offsets of the scratch buffers of a batch of P rows inside the array, and their total
+static size_t noalloc_layout(const pes_context* ctx, int P, bool grad, size_t off[6]) {
+  const size_t NNS = (size_t)ctx->n_nodes + 1, rows = (size_t)P, nm = (size_t)ctx->modes.n;
+  size_t o = 0;
+  off[0] = o; o += align256(sizeof(int));                                  // work counter
+  off[1] = o; o += align256(rows * N_SETUP * 5 * sizeof(double));          // setup
+  off[2] = o; o += align256(rows * PC_STRIDE * sizeof(double));            // per-row constants
+  off[3] = o; o += align256(rows * nm * NNS * 4 * sizeof(double));         // node tables
+  off[4] = o; if (grad) o += align256(rows * nm * NNS * 16 * sizeof(double));  // their gradients
+  off[5] = o; o += align256((size_t)n_groups_of(ctx) * (grad ? 1 + PES_N_PARAMS : 1) * rows * sizeof(double));
+  return o;
+}
+
 static void free_dev(void* p) {
   if (p) (void)hipFree(p);
 }
@@ -833,6 +878,11 @@ extern "C" int pes_context_finalize(pes_context* ctx) {
   }
   if (ctx->direct) ctx->d_node_f = ctx->d_bin_f;
   HIP_TRY(hipMalloc(&ctx->d_work_counter, sizeof(int)));
+  {  // the device address of the no-allocation entry point's scratch, once per device
+    char* base = nullptr;
+    const int rc = noalloc_base(ctx->device, &base);
+    if (rc != PES_OK) return rc;
+  }
   ctx->finalized = true;
   return PES_OK;
 }
@@ -865,9 +915,6 @@ extern "C" int pes_detector_bins(const pes_context* ctx, int det) {
   return ctx->hdets[det].d.n_bins;
 }
 
-static int n_groups_of(const pes_context* ctx) {
-  return (ctx->n_tiles + ctx->tiles_per_group - 1) / ctx->tiles_per_group;
-}
 
 // (re)allocate scratch for P rows; allocation happens only when P, the gradient flag or the
 // number of tile groups grows
@@ -904,28 +951,151 @@ static int ensure_scratch(pes_context* ctx, int P, bool grad) {
   return PES_OK;
 }
 
-static int prepare(pes_context* ctx, const double* d_params, int P, bool grad, hipStream_t st) {
+// Device buffers a batch of P rows works in besides its inputs and outputs.
+struct BatchScratch {
+  double *setup, *pconst, *nodes, *nodes_grad, *partial;
+  int* work_counter;
+};
+
+static BatchScratch context_scratch(const pes_context* ctx) {
+  return {ctx->d_setup, ctx->d_pconst, ctx->d_nodes, ctx->d_nodes_grad, ctx->d_partial, ctx->d_work_counter};
+}
+
+static int check_batch_args(const pes_context* ctx, const double* d_params, int P) {
   if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
   if (!ctx->finalized) return fail(PES_ERR_STATE, "context not finalized");
   if (P < 0) return fail(PES_ERR_INVALID, "P must be >= 0");
   if (P > 0 && !d_params) return fail(PES_ERR_INVALID, "params is NULL");
-  HIP_TRY(hipSetDevice(ctx->device));
-  int rc = ensure_scratch(ctx, P > 0 ? P : 1, grad);
-  if (rc != PES_OK) return rc;
-  if (P == 0) return PES_OK;
-  hipLaunchKernelGGL(k_setup, dim3((P + 63) / 64), dim3(64), 0, st, d_params, P, ctx->modes, ctx->f_ref,
-                     ctx->d_setup, ctx->d_pconst, ctx->d_work_counter);
+  return PES_OK;
+}
+
+// k_setup (which also zeroes the work counter) and k_nodes for P >= 1 rows
+static int launch_setup_nodes(const pes_context* ctx, const BatchScratch& sc, const double* d_params, int P, bool grad,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_setup, dim3((P + 63) / 64), dim3(64), 0, st, d_params, P, ctx->modes, ctx->f_ref, sc.setup,
+                     sc.pconst, sc.work_counter);
   HIP_TRY(hipGetLastError());
   const int NNS = ctx->n_nodes + 1;
   dim3 g((NNS + 127) / 128, P * ctx->modes.n);
   if (grad)
-    hipLaunchKernelGGL(k_nodes<true>, g, dim3(128), 0, st, ctx->d_setup, P, ctx->modes, ctx->n_nodes, ctx->u0,
-                       ctx->du, ctx->d_node_f, ctx->d_nodes, ctx->d_nodes_grad);
+    hipLaunchKernelGGL(k_nodes<true>, g, dim3(128), 0, st, sc.setup, P, ctx->modes, ctx->n_nodes, ctx->u0, ctx->du,
+                       ctx->d_node_f, sc.nodes, sc.nodes_grad);
   else
-    hipLaunchKernelGGL(k_nodes<false>, g, dim3(128), 0, st, ctx->d_setup, P, ctx->modes, ctx->n_nodes, ctx->u0,
-                       ctx->du, ctx->d_node_f, ctx->d_nodes, nullptr);
+    hipLaunchKernelGGL(k_nodes<false>, g, dim3(128), 0, st, sc.setup, P, ctx->modes, ctx->n_nodes, ctx->u0, ctx->du,
+                       ctx->d_node_f, sc.nodes, nullptr);
   HIP_TRY(hipGetLastError());
   return PES_OK;
+}
+
+static int prepare(pes_context* ctx, const double* d_params, int P, bool grad, hipStream_t st) {
+  int rc = check_batch_args(ctx, d_params, P);
+  if (rc != PES_OK) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  rc = ensure_scratch(ctx, P > 0 ? P : 1, grad);
+  if (rc != PES_OK) return rc;
+  if (P == 0) return PES_OK;
+  return launch_setup_nodes(ctx, context_scratch(ctx), d_params, P, grad, st);
+}
+
+// rows per work item of the fused kernel: the kernel's chunk, fewer when the grid has too few
+// tile groups to give every persistent block about four work items (small grids, e.g. 16k bins)
+static int rows_per_item(const pes_context* ctx, int P, bool grad, int grid) {
+  const int max_rows = grad ? chunk_rows<true>() : chunk_rows<false>();
+  const long long fit = (long long)P * n_groups_of(ctx) / (4LL * grid);
+  return (int)std::max(8LL, std::min((long long)max_rows, fit));
+}
+
+// The fused kernel and the reduction, on scratch already set up by launch_setup_nodes.
+static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, double* d_lnl, double* d_grad,
+                             int precision, hipStream_t st, bool profile) {
+  const bool grad = d_grad != nullptr;
+  const int NNS = ctx->n_nodes + 1;
+  const int n_groups = n_groups_of(ctx);
+  int grid = ctx->n_cu * ctx->blocks_per_cu;
+  const int rows = rows_per_item(ctx, P, grad, grid);
+  const int n_chunks = (P + rows - 1) / rows;
+  const int n_work = n_groups * n_chunks;
+  if (grid > n_work) grid = n_work;
+  const int nout = grad ? 1 + PES_N_PARAMS : 1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (profile) {
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, st));
+  }
+  const int n_det = (int)ctx->hdets.size();
+#define PES_LNL_ARGS                                                                                         \
+  ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_rec,                 \
+      ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, sc.pconst, sc.nodes, sc.nodes_grad, NNS, ctx->du, \
+      ctx->modes, P, rows, ctx->d_tile_jr, sc.work_counter, sc.partial
+  if (!grad)
+    HIP_TRY(launch_lnl_fwd(precision, grid, st, PES_LNL_ARGS));
+  else if (precision == 0)
+    hipLaunchKernelGGL((k_lnl<0, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+  else
+    hipLaunchKernelGGL((k_lnl<1, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+#undef PES_LNL_ARGS
+  HIP_TRY(hipGetLastError());
+  if (profile) {
+    HIP_TRY(hipEventRecord(e1, st));
+    ctx->ev_start.push_back(e0);
+    ctx->ev_stop.push_back(e1);
+    if (P > ctx->cap_live) {
+      free_dev(ctx->d_live);
+      ctx->d_live = nullptr;
+      HIP_TRY(hipMalloc(&ctx->d_live, (size_t)P * 2 * sizeof(double)));
+      ctx->cap_live = P;
+    }
+    hipLaunchKernelGGL(k_live, dim3((P + 63) / 64), dim3(64), 0, st, ctx->d_dets, n_det, ctx->d_bin_f, sc.pconst,
+                       ctx->modes, P, ctx->d_live);
+    HIP_TRY(hipGetLastError());
+    ctx->n_live = P;
+  }
+  hipLaunchKernelGGL(k_reduce, dim3((P + 63) / 64, nout), dim3(64 * REDUCE_WAVES), 0, st, sc.partial, n_groups, P,
+                     nout, sc.pconst, d_lnl, d_grad);
+  HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" size_t pes_lnl_batch_noalloc_bytes(const pes_context* ctx, int P, int grad) {
+  if (!ctx || !ctx->finalized || P < 1) return 0;
+  size_t off[6];
+  return noalloc_layout(ctx, P, grad != 0, off);
+}
+
+extern "C" int pes_lnl_batch_noalloc(pes_context* ctx, const double* d_params, int P, double* d_lnl, double* d_grad,
+                                     int precision, void* stream) {
+  if (precision != PES_PREC_FP64 && precision != PES_PREC_MIXED)
+    return fail(PES_ERR_INVALID, "precision must be PES_PREC_FP64 or PES_PREC_MIXED");
+  if (P > 0 && !d_lnl) return fail(PES_ERR_INVALID, "lnl output is NULL");
+  int rc = check_batch_args(ctx, d_params, P);
+  if (rc != PES_OK || P == 0) return rc;
+  const bool grad = d_grad != nullptr;
+  size_t off[6];
+  const size_t need = noalloc_layout(ctx, P, grad, off);
+  if (need > (size_t)PES_NOALLOC_SCRATCH_BYTES) {
+    char msg[256];
+    std::snprintf(msg, sizeof msg,
+                  "pes_lnl_batch_noalloc: %d rows%s need %zu bytes of scratch, more than the fixed %zu "
+                  "(split the batch, or use pes_lnl_batch)",
+                  P, grad ? " with the gradient" : "", need, (size_t)PES_NOALLOC_SCRATCH_BYTES);
+    return fail(PES_ERR_INVALID, msg);
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  char* base = nullptr;
+  rc = noalloc_base(ctx->device, &base);
+  if (rc != PES_OK) return rc;
+  BatchScratch sc;
+  sc.work_counter = reinterpret_cast<int*>(base + off[0]);
+  sc.setup = reinterpret_cast<double*>(base + off[1]);
+  sc.pconst = reinterpret_cast<double*>(base + off[2]);
+  sc.nodes = reinterpret_cast<double*>(base + off[3]);
+  sc.nodes_grad = grad ? reinterpret_cast<double*>(base + off[4]) : nullptr;
+  sc.partial = reinterpret_cast<double*>(base + off[5]);
+  hipStream_t st = (hipStream_t)stream;
+  rc = launch_setup_nodes(ctx, sc, d_params, P, grad, st);
+  if (rc != PES_OK) return rc;
+  return launch_lnl_reduce(ctx, sc, P, d_lnl, d_grad, precision, st, false);
 }
 
 extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, double* d_lnl, double* d_grad,
@@ -937,57 +1107,7 @@ extern "C" int pes_lnl_batch(pes_context* ctx, const double* d_params, int P, do
   bool grad = d_grad != nullptr;
   int rc = prepare(ctx, d_params, P, grad, st);
   if (rc != PES_OK || P == 0) return rc;
-  const int NNS = ctx->n_nodes + 1;
-  const int n_groups = n_groups_of(ctx);
-  // rows per work item: the kernel's chunk, fewer when the grid has too few tile groups to give
-  // every persistent block about four work items (small grids, e.g. 16k bins)
-  int grid = ctx->n_cu * ctx->blocks_per_cu;
-  const int max_rows = grad ? chunk_rows<true>() : chunk_rows<false>();
-  const long long fit = (long long)P * n_groups / (4LL * grid);
-  const int rows = (int)std::max(8LL, std::min((long long)max_rows, fit));
-  const int n_chunks = (P + rows - 1) / rows;
-  const int n_work = n_groups * n_chunks;
-  if (grid > n_work) grid = n_work;
-  int nout = 1;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (ctx->profiling) {
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, st));
-  }
-  const int n_det = (int)ctx->hdets.size();
-  if (grad) nout = 1 + PES_N_PARAMS;
-#define PES_LNL_ARGS                                                                                         \
-  ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_rec,                 \
-      ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, ctx->d_pconst, ctx->d_nodes,   \
-      ctx->d_nodes_grad, NNS, ctx->du, ctx->modes, P, rows, ctx->d_tile_jr, ctx->d_work_counter, ctx->d_partial
-  if (!grad)
-    HIP_TRY(launch_lnl_fwd(precision, grid, st, PES_LNL_ARGS));
-  else if (precision == 0)
-    hipLaunchKernelGGL((k_lnl<0, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
-  else
-    hipLaunchKernelGGL((k_lnl<1, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
-#undef PES_LNL_ARGS
-  HIP_TRY(hipGetLastError());
-  if (ctx->profiling) {
-    HIP_TRY(hipEventRecord(e1, st));
-    ctx->ev_start.push_back(e0);
-    ctx->ev_stop.push_back(e1);
-    if (P > ctx->cap_live) {
-      free_dev(ctx->d_live);
-      ctx->d_live = nullptr;
-      HIP_TRY(hipMalloc(&ctx->d_live, (size_t)P * 2 * sizeof(double)));
-      ctx->cap_live = P;
-    }
-    hipLaunchKernelGGL(k_live, dim3((P + 63) / 64), dim3(64), 0, st, ctx->d_dets, n_det, ctx->d_bin_f,
-                       ctx->d_pconst, ctx->modes, P, ctx->d_live);
-    HIP_TRY(hipGetLastError());
-    ctx->n_live = P;
-  }
-  hipLaunchKernelGGL(k_reduce, dim3((P + 63) / 64, nout), dim3(64 * REDUCE_WAVES), 0, st, ctx->d_partial, n_groups, P,
-                     nout, ctx->d_pconst, d_lnl, d_grad);
-  HIP_TRY(hipGetLastError());
-  return PES_OK;
+  return launch_lnl_reduce(ctx, context_scratch(ctx), P, d_lnl, d_grad, precision, st, ctx->profiling);
 }
 
 extern "C" int pes_response(pes_context* ctx, int det, const double* d_params, int P, double* d_out, int precision,
