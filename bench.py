@@ -196,66 +196,67 @@ def traffic_of(workload, precision, grad):
         return None
 
 
-def _cpu_worker(args):
-    """Evaluate oracle lnL on the sampled detectors for ``budget_s`` seconds (spawned process)."""
-    odets, rows, modes, f_ref, interp, budget_s, max_evals = args
-    from oracle import likelihood as olk
-    n, t0 = 0, time.perf_counter()
-    while True:
-        olk.lnl(olk.as_dict(rows[n % rows.shape[0]]), odets, modes, f_ref, interp)
-        n += 1
-        if time.perf_counter() - t0 > budget_s or n >= max_evals:
-            break
-    return n, time.perf_counter() - t0
-
-
-def cpu_baseline(dets, wf, rows, budget_s=8.0, workers=16):
-    """Numpy FP64 oracle lnL on a bounded bin sample, scaled to the full grid: one core, then
-    ``workers`` processes (spawned, each evaluating rows independently) on the host's cores."""
-    import multiprocessing as mp
-
+def cpu_baseline(dets, wf, rows, n_rows=2):
+    """The oracle's FP64 C++/OpenMP restatement of the reference's per-bin algorithm
+    (oracle/cpu_lnl.cpp: interpolated waveform, t_c shift, orbit, six links, TDI, Whittle sum;
+    the gradient by forward-mode dual numbers) on the same network, data and rows, timed on this
+    host's cores: at 1 core on every 16th bin (lnL + gradient) and every 4th bin (lnL), scaled to
+    the full grid, and with every core of the host's share (at most 16, the CPU share of one GPU)
+    on the full grid.  The per-row node tables (numpy) are prepared before the timed calls."""
+    from oracle import cpu_baseline as cb
     from oracle import likelihood as olk
     modes = tuple((int(m[0]), int(m[1])) for m in wf.modes)
-    total_bins = sum(d.tdi_data.data_info.frequency_series_length for d in dets)
-    stride = 64
     odets = []
     for d in dets:
         td = d.tdi_data
-        sl = slice(0, None, stride)
-        f = np.asarray(td.frequency_samples)[sl]
-        data = {c: np.asarray(getattr(td.fd_data, c))[sl] for c in ("A", "E", "T")}
-        psd = {c: np.asarray(getattr(td.fd_noise_power_density, c))[sl] for c in ("A", "E", "T")}
+        f = np.asarray(td.frequency_samples)
+        data = {c: np.asarray(getattr(td.fd_data, c)) for c in ("A", "E", "T")}
+        psd = {c: np.asarray(getattr(td.fd_noise_power_density, c)) for c in ("A", "E", "T")}
         orb = d.orbit_model
         kind = "helio" if orb.kind == 0 else "geo"
+        geo = (orb.omega_rotation, orb.lambda_ref, orb.beta_ref) if orb.kind == 1 else ()
         df = td.data_info.delta_frequency
-        if df is None:  # explicit grid: the sampled bins carry the weight of the bins they stand for
-            df = np.asarray(td.frequency_weights())[sl] * stride
-        odets.append(olk.Detector(f, data, psd, df,
-                                  (kind, orb.arm_length, orb.rotation_initial, orb.revolution_initial), "1.5"))
-    sample_bins = sum(o.f.size for o in odets)
-    scale = total_bins / sample_bins
-    interp = (wf.f_node_min, wf.f_node_max, wf.n_nodes) if wf.n_nodes > 0 else None
-    task = (odets, rows[:64], modes, wf.reference_frequency, interp, budget_s, 400)
-    n1, t1 = _cpu_worker(task)
-    single = n1 / t1 / scale
+        if df is None:
+            df = np.asarray(td.frequency_weights())
+        odets.append(olk.Detector(f, data, psd, df, (kind, orb.arm_length, orb.rotation_initial,
+                                                       orb.revolution_initial, *geo), "1.5"))
+    net = cb.Network(odets)
+    interp = (wf.f_node_min, wf.f_node_max, wf.n_nodes)
+    ris = [cb.RowInputs(r, modes, wf.reference_frequency, interp, grad=True) for r in rows[:n_rows]]
     try:
         avail = len(os.sched_getaffinity(0))
     except AttributeError:
         avail = os.cpu_count() or 1
-    nw = max(1, min(workers, avail))
-    ctx = mp.get_context("spawn")  # fresh interpreters: nothing GPU-related is inherited
-    with ctx.Pool(nw) as pool:
+    threads = max(1, min(16, avail))
+
+    def timed(grad, stride, nthr):
         t0 = time.perf_counter()
-        res = pool.map(_cpu_worker, [task] * nw)
-        wall = time.perf_counter() - t0
-    n_all = sum(n for n, _ in res)
-    busy = max(t for _, t in res)
-    multi = n_all / busy / scale
-    return dict(value=multi, unit="lnL evals/s (no gradient)", cores=nw, kind="port", single_core_value=single,
-                sample=f"numpy FP64 oracle lnL on every {stride}th bin ({sample_bins} of {total_bins} bins, "
-                       f"{len(modes)} modes, {len(odets)} detectors), scaled linearly to the full grid; "
-                       f"1 core: {n1} evals in {t1:.1f} s; {nw} spawned processes: {n_all} evals, "
-                       f"{busy:.1f} s each ({wall:.1f} s wall incl. start-up)")
+        for ri in ris:
+            cb.lnl_row(net, ri, grad=grad, stride=stride, threads=nthr)
+        return (time.perf_counter() - t0) / len(ris) * 1.0  # seconds per row at this stride
+
+    cb.lnl_row(net, ris[0], grad=False, stride=64, threads=threads)  # warm-up (thread pool, pages)
+    g1 = timed(True, 16, 1) * 16
+    l1 = timed(False, 4, 1) * 4
+    gN = timed(True, 1, threads)
+    lN = timed(False, 1, threads)
+    # (bin, mode) pairs below the mode cuts, for the cost per pair
+    bm = 0
+    for r in rows[:n_rows]:
+        from oracle import phenom as oph
+        hm = oph.PhenomHM(r[0], r[1], r[2], r[3], wf.reference_frequency, modes)
+        bm += sum(int(np.sum(net.f <= hm.mode_cut(md) / hm.Msec)) for md in modes)
+    bm /= n_rows
+    return dict(value=1.0 / gN, unit="lnL+grad evals/s", cores=threads, kind="port",
+                lnl_only_value=1.0 / lN, single_core_value=1.0 / g1, single_core_lnl_only_value=1.0 / l1,
+                ns_per_bin_mode_single_core={"lnl": l1 / bm * 1e9, "lnl_grad": g1 / bm * 1e9},
+                reference_taichi_ns_per_bin=144.0,
+                sample=f"oracle/cpu_lnl.cpp (FP64 C++/OpenMP restatement, gradient by 11-way forward-mode duals) "
+                       f"on {n_rows} rows of the batch, full network ({net.NB} bins, {len(modes)} modes, "
+                       f"{bm:.3g} (bin, mode) pairs below the cuts per row): {threads} threads on the full grid "
+                       f"(lnL+grad {gN:.2f} s/row, lnL {lN:.2f} s/row); 1 core on every 16th bin (lnL+grad) "
+                       f"and every 4th bin (lnL), scaled (x16, x4) to the full grid; the reference's taichi "
+                       f"CPU backend: 144 ns per bin per lnL (XAS, 65,471 bins)")
 
 
 def copy_calibration(n_bytes=1 << 30, reps=3):

@@ -128,3 +128,29 @@ def test_grad_direct_log_grid():
     core.add_detector(0, orb[1], orb[2], orb[3], 15, True, f, data, np.stack([psd[c] for c in labels]), [1, 1, 1],
                       weight=w)
     _fd_check_core(core, np.vstack([inj_row(), particle_cloud(2, rel=1e-5)]))
+
+
+@pytest.mark.parametrize("orbit,psd_model,fmin", [("LISA_analytic", "LISA_SciRDv1", 1e-4),
+                                                  ("Tianqin_analytic", "Tianqin_GWSpace", 1e-3)])
+def test_adjoint_matches_forward_mode_gradient_of_the_cpu_restatement(orbit, psd_model, fmin):
+    """The HIP adjoint against a second, unrelated differentiation of the same lnL: forward-mode
+    dual numbers through the oracle's C++ restatement (node-table derivatives by central
+    differences, ~1e-9 relative)."""
+    import os
+
+    from oracle import build_cpu
+    if not os.path.exists(build_cpu.LIB):
+        build_cpu.build()
+    from oracle import cpu_baseline as cb
+    modes = oph.MODES
+    d, labels = make_detector(modes=modes, orbit=orbit, psd_model=psd_model, fmin=fmin)
+    core = _core(modes, [(d, labels)])
+    rows = np.vstack([particle_cloud(3, seed=41), prior_rows(2, seed=42)])
+    lnl, g = (x.cpu().numpy() for x in core.lnl(rows, grad=True))
+    net = cb.Network([d])
+    for p in range(rows.shape[0]):
+        lc, gc = cb.lnl_row(net, cb.RowInputs(rows[p], modes, 0.0, INTERP, grad=True), grad=True, threads=8)
+        assert abs(lc - lnl[p]) < 1e-10 * abs(lnl[p])
+        scale = np.maximum(np.abs(gc), 1e-6 * np.abs(gc).max())
+        err = np.abs(g[p] - gc) / scale
+        assert np.all(err < 1e-6), (p, err)
