@@ -1,0 +1,36 @@
+// Definitions shared by the kernels of pespace_amd.hip and hotpath.hpp: tile and chunk sizes,
+// the per-row constant block, and the tile -> detector map.  Included inside the anonymous
+// namespace of pespace_amd.hip.
+#pragma once
+
+constexpr int TILE = 256;      // bins per tile = threads per block
+constexpr int CHUNK = 40;      // parameter rows per work item of the gradient kernel ...
+template <bool GRAD>
+constexpr int chunk_rows() { return GRAD ? CHUNK : 4 * CHUNK; }  // ... and of the lnL-only one
+constexpr int TILES_PER_GROUP = 32;  // default schedule: tiles per work item ...
+constexpr int BLOCKS_PER_CU = 4;     // ... and persistent blocks launched per CU
+constexpr int MAX_MODES = 6;         // modes per context (ModeList)
+constexpr int NJ_STAGE = 30;         // node rows per tile that the fused kernel stages in LDS
+constexpr int BIN_REC = 10;          // doubles per bin of the fused kernel's weight and data record
+
+// ---- per-row constant block ------------------------------------------------------------
+enum PConst {
+  PC_TC = 0, PC_INVDL, PC_VALID, PC_DL,
+  PC_P = 4, PC_Q = 7, PC_K = 10,
+  PC_DLP = 13, PC_DLQ = 16, PC_DBP = 19, PC_DBQ = 22, PC_DLK = 25, PC_DBK = 28,
+  PC_MODE = 32,  // per mode: Fp re,im, 2 Fc re,im, cut_hz, dFp/diota re,im, 2 dFc/diota re,im
+  PC_MODE_STRIDE = 9,
+  PC_FCUT = 32 + 6 * 9,  // largest mode cut [Hz]: the response is zero above it
+  PC_STRIDE = 32 + 6 * 9 + 2
+};
+enum { MF_FPR = 0, MF_FPI, MF_FCR, MF_FCI, MF_CUT, MF_DFPR, MF_DFPI, MF_DFCR, MF_DFCI };
+
+// detector owning tile `tile` (detectors are concatenated in tile order)
+__device__ inline int tile_detector(const DetectorDev* dets, int n_det, int tile) {
+  int acc = 0;
+  for (int d = 0; d < n_det; ++d) {
+    acc += dets[d].n_tiles;
+    if (tile < acc) return d;
+  }
+  return n_det - 1;
+}

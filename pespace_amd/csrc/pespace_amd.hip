@@ -26,13 +26,6 @@
 
 using namespace pes;
 
-namespace pes {
-struct ModeList {  // the modes of a context (ids into kModeM etc.), passed to kernels by value
-  int n;
-  int id[6];
-};
-}  // namespace pes
-
 namespace {
 
 thread_local std::string g_err;
@@ -48,27 +41,7 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(PES_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-constexpr int TILE = 256;      // bins per tile = threads per block
-constexpr int CHUNK = 40;      // parameter rows per work item of the gradient kernel ...
-template <bool GRAD>
-constexpr int chunk_rows() { return GRAD ? CHUNK : 4 * CHUNK; }  // ... and of the lnL-only one
-constexpr int TILES_PER_GROUP = 32;  // default schedule: tiles per work item ...
-constexpr int BLOCKS_PER_CU = 4;     // ... and persistent blocks launched per CU
-constexpr int MAX_MODES = 6;         // modes per context (ModeList)
-constexpr int NJ_STAGE = 30;         // node rows per tile that the fused kernel stages in LDS
-constexpr int BIN_REC = 10;          // doubles per bin of the fused kernel's weight and data record
-
-// ---- per-row constant block ------------------------------------------------------------
-enum PConst {
-  PC_TC = 0, PC_INVDL, PC_VALID, PC_DL,
-  PC_P = 4, PC_Q = 7, PC_K = 10,
-  PC_DLP = 13, PC_DLQ = 16, PC_DBP = 19, PC_DBQ = 22, PC_DLK = 25, PC_DBK = 28,
-  PC_MODE = 32,  // per mode: Fp re,im, 2 Fc re,im, cut_hz, dFp/diota re,im, 2 dFc/diota re,im
-  PC_MODE_STRIDE = 9,
-  PC_FCUT = 32 + 6 * 9,  // largest mode cut [Hz]: the response is zero above it
-  PC_STRIDE = 32 + 6 * 9 + 2
-};
-enum { MF_FPR = 0, MF_FPI, MF_FCR, MF_FCI, MF_CUT, MF_DFPR, MF_DFPI, MF_DFCR, MF_DFCI };
+#include "kernel_common.hpp"
 
 __device__ inline bool finite_d(double x) { return isfinite(x); }
 
@@ -271,15 +244,6 @@ struct BinData {
   Cplx d[3];
   int j;
 };
-
-__device__ inline int tile_detector(const DetectorDev* dets, int n_det, int tile) {
-  int acc = 0;
-  for (int d = 0; d < n_det; ++d) {
-    acc += dets[d].n_tiles;
-    if (tile < acc) return d;
-  }
-  return n_det - 1;
-}
 
 #include "hotpath.hpp"
 
@@ -635,43 +599,8 @@ static int fill_orbit(DetectorDev& d, int orbit_kind, double arm_length_m, doubl
     return fail(PES_ERR_INVALID, "initial orbit phases must be finite");
   if (orbit_kind == 1 && !(std::isfinite(omega_rotation) && std::isfinite(lambda_ref) && std::isfinite(beta_ref)))
     return fail(PES_ERR_INVALID, "geocentric rotation rate and reference source must be finite");
-  d.kind = orbit_kind;
-  d.L = arm_length_m / C_SI;
-  d.rot0 = rotation_initial;
-  d.rev0 = revolution_initial;
-  d.invL = 1.0 / d.L;
-  if (orbit_kind == 0) {  // orbit.py:189-228
-    d.rp = d.L / (2.0 * std::sqrt(3.0));
-    double k1 = rotation_initial, k2 = k1 + 2.0 * PI / 3.0, k3 = k2 + 2.0 * PI / 3.0;
-    d.ck[0] = std::cos(k1); d.sk[0] = std::sin(k1);
-    d.ck[1] = std::cos(k2); d.sk[1] = std::sin(k2);
-    d.ck[2] = std::cos(k3); d.sk[2] = std::sin(k3);
-    for (int i = 0; i < 3; ++i) {
-      d.hck[i] = d.rp * d.ck[i];
-      d.hsk[i] = d.rp * d.sk[i];
-    }
-    d.c3L = std::sqrt(3.0) * d.invL;
-    for (int i = 0; i < 3; ++i) {
-      d.hca[i] = d.c3L * d.hck[i];
-      d.hsa[i] = d.c3L * d.hsk[i];
-    }
-  } else {  // orbit.py:77-139
-    d.rp = d.L / std::sqrt(3.0);
-    d.slam = std::sin(lambda_ref); d.clam = std::cos(lambda_ref);
-    d.sbet = std::sin(beta_ref); d.cbet = std::cos(beta_ref);
-    d.omega_rot = omega_rotation;
-    d.gxs = d.rp * d.sbet * d.clam;
-    d.gxc = d.rp * d.slam;
-    d.gys = d.rp * d.sbet * d.slam;
-    d.gyc = -d.rp * d.clam;
-    d.gzs = -d.rp * d.cbet;
-    d.c3L = std::sqrt(3.0) * d.invL;
-    d.sgxs = d.c3L * d.gxs;
-    d.sgxc = d.c3L * d.gxc;
-    d.sgys = d.c3L * d.gys;
-    d.sgyc = d.c3L * d.gyc;
-    d.sgzs = d.c3L * d.gzs;
-  }
+  orbit_constants(d, orbit_kind, arm_length_m, rotation_initial, revolution_initial, omega_rotation, lambda_ref,
+                  beta_ref);
   return PES_OK;
 }
 

@@ -41,6 +41,55 @@ struct DetectorDev {
   double sgxs, sgxc, sgys, sgyc, sgzs;
 };
 
+struct ModeList {  // the modes of a context (ids into kModeM etc.), passed to kernels by value
+  int n;
+  int id[6];
+};
+
+// Host side: the orbit constants of a detector (heliocentric orbit.py:189-228; geocentric
+// orbit.py:112-139 with the given rotation rate and reference source); arguments validated by
+// the caller.
+inline void orbit_constants(DetectorDev& d, int orbit_kind, double arm_length_m, double rotation_initial,
+                            double revolution_initial, double omega_rotation, double lambda_ref, double beta_ref) {
+  d.kind = orbit_kind;
+  d.L = arm_length_m / C_SI;
+  d.rot0 = rotation_initial;
+  d.rev0 = revolution_initial;
+  d.invL = 1.0 / d.L;
+  if (orbit_kind == 0) {  // orbit.py:189-228
+    d.rp = d.L / (2.0 * std::sqrt(3.0));
+    double k1 = rotation_initial, k2 = k1 + 2.0 * PI / 3.0, k3 = k2 + 2.0 * PI / 3.0;
+    d.ck[0] = std::cos(k1); d.sk[0] = std::sin(k1);
+    d.ck[1] = std::cos(k2); d.sk[1] = std::sin(k2);
+    d.ck[2] = std::cos(k3); d.sk[2] = std::sin(k3);
+    for (int i = 0; i < 3; ++i) {
+      d.hck[i] = d.rp * d.ck[i];
+      d.hsk[i] = d.rp * d.sk[i];
+    }
+    d.c3L = std::sqrt(3.0) * d.invL;
+    for (int i = 0; i < 3; ++i) {
+      d.hca[i] = d.c3L * d.hck[i];
+      d.hsa[i] = d.c3L * d.hsk[i];
+    }
+  } else {  // orbit.py:77-139
+    d.rp = d.L / std::sqrt(3.0);
+    d.slam = std::sin(lambda_ref); d.clam = std::cos(lambda_ref);
+    d.sbet = std::sin(beta_ref); d.cbet = std::cos(beta_ref);
+    d.omega_rot = omega_rotation;
+    d.gxs = d.rp * d.sbet * d.clam;
+    d.gxc = d.rp * d.slam;
+    d.gys = d.rp * d.sbet * d.slam;
+    d.gyc = -d.rp * d.clam;
+    d.gzs = -d.rp * d.cbet;
+    d.c3L = std::sqrt(3.0) * d.invL;
+    d.sgxs = d.c3L * d.gxs;
+    d.sgxc = d.c3L * d.gxc;
+    d.sgys = d.c3L * d.gys;
+    d.sgyc = d.c3L * d.gyc;
+    d.sgzs = d.c3L * d.gzs;
+  }
+}
+
 struct Vec3 {
   double x, y, z;
 };
