@@ -165,6 +165,25 @@ int pes_orbit_vectors(int orbit_kind, double arm_length_m, double rotation_initi
                       double omega_rotation, double lambda_ref, double beta_ref, const double* d_t, int n,
                       double* d_out, void* stream);
 
+/* Time domain.  pes_td_links: the six single links of a detector (orbit as in
+ * pes_context_add_detector) at the n times t0 + i dt for a waveform h+, hx sampled at
+ * wave_t0 + m wave_dt (device d_hp, d_hc [wave_n], linearly interpolated, zero outside), as
+ * y_rs(t) = n.[H(t - L - k.x_s) - H(t - k.x_r)].n / (2 (1 - k.n_sr)) with the constellation at t:
+ * device out [6][n] (link12, link21, link23, link32, link31, link13).  Replaces the reference's
+ * time-domain response model (TDResponseModelCornish2003, antenna.py:566-800, commented out there).
+ * pes_td_tdi: Michelson TDI 1.5 / 2.0 of links sampled at spacing dt (device links [6][n_ext]):
+ * output sample i is the combination at link index i + added, added =
+ * pes_td_tdi_added_samples(...) = floor(max_delay L / dt) + 1 (max_delay 3 or 7), each delay
+ * D^m y = y(t - m L) linearly interpolated; device out [3][n_ext - added] (A, E, T or X, Y, Z).
+ * Replaces TDMichelsonConstantEqualArm.update_tdi_response (tdi.py:1388-1600). */
+int pes_td_links(int orbit_kind, double arm_length_m, double rotation_initial, double revolution_initial,
+                 double omega_rotation, double lambda_ref, double beta_ref, const double* d_hp, const double* d_hc,
+                 double wave_t0, double wave_dt, int wave_n, double lam, double beta, double psi, double t0, double dt,
+                 int n, double* d_out, void* stream);
+int pes_td_tdi_added_samples(double arm_length_m, double dt, int tdi_generation);
+int pes_td_tdi(double arm_length_m, double dt, int tdi_generation, int orthogonal, const double* d_links, int n_ext,
+               double* d_out, void* stream);
+
 /* Scratch for HIP-graph capture.  pes_context_reserve sizes the context's scratch for batches
  * of up to P rows (with the gradient tables if grad != 0) under the current schedule.  While
  * pes_context_pin_scratch(ctx, 1) holds, the scratch is never reallocated: a call that would

@@ -64,6 +64,15 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
       ctypes.c_void_p]),
     ("pes_lnl_batch_noalloc_bytes", ctypes.c_size_t, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("pes_td_links", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+      ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+      ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+      ctypes.c_void_p, ctypes.c_void_p]),
+    ("pes_td_tdi_added_samples", ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    ("pes_td_tdi", ctypes.c_int,
+     [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+      ctypes.c_void_p]),
 ]
 
 

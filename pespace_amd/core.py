@@ -352,6 +352,71 @@ def orbit_vectors(orbit_kind, arm_length, rotation_initial, revolution_initial, 
     return out.reshape(shape + (6, 3))
 
 
+def _device(device):
+    if not torch.cuda.is_available():
+        raise PespaceError("pespace_amd's hip backend needs a GPU (torch.cuda.is_available() is False)")
+    return torch.device(f"cuda:{int(device)}") if not isinstance(device, torch.device) else device
+
+
+def td_links(orbit_kind, arm_length, rotation_initial, revolution_initial, hp, hc, wave_t0, wave_dt, lam, beta, psi,
+             t0, dt, n, omega_rotation=GEO_OMEGA_ROTATION, lambda_ref=GEO_LAMBDA_REF, beta_ref=GEO_BETA_REF,
+             out=None, device=0):
+    """Time-domain single links on the GPU: a (6, n) float64 tensor (link12, link21, link23,
+    link32, link31, link13) at the times t0 + i dt, for h+, hx sampled at wave_t0 + m wave_dt
+    (linearly interpolated, zero outside the samples), with the constellation at each sample
+    time (the reference's commented TDResponseModelCornish2003, antenna.py:681-800)."""
+    dev = hp.device if isinstance(hp, torch.Tensor) and hp.is_cuda else _device(device)
+    hp = torch.as_tensor(hp, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+    hc = torch.as_tensor(hc, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+    if hp.numel() != hc.numel():
+        raise ValueError(f"h+ and hx differ in length ({hp.numel()} vs {hc.numel()})")
+    n = int(n)
+    if out is None:
+        out = torch.empty(6, n, dtype=torch.float64, device=dev)
+    else:
+        LikelihoodCore._check_out(out, "out", (6, n), dev)
+    check(lib().pes_td_links(int(orbit_kind), float(arm_length), float(rotation_initial), float(revolution_initial),
+                             float(omega_rotation), float(lambda_ref), float(beta_ref), hp.data_ptr(), hc.data_ptr(),
+                             float(wave_t0), float(wave_dt), hp.numel(), float(lam), float(beta), float(psi),
+                             float(t0), float(dt), n, out.data_ptr(), torch.cuda.current_stream(dev).cuda_stream))
+    return out
+
+
+def td_added_samples(arm_length, dt, tdi_generation):
+    """Link samples the time-domain TDI needs before its first output: int(max_delay L // dt + 1),
+    max_delay 3 (TDI 1.5) or 7 (TDI 2.0) (tdi.py:1445-1455)."""
+    gen = {"1.5": 15, "2.0": 20, 15: 15, 20: 20}.get(tdi_generation)
+    if gen is None:
+        raise ValueError(f"Unsupported generation {tdi_generation}.")
+    added = lib().pes_td_tdi_added_samples(float(arm_length), float(dt), gen)
+    if added < 0:
+        raise ValueError("arm length and dt must be positive")
+    return added
+
+
+def td_tdi(arm_length, dt, tdi_generation, orthogonal, links, out=None):
+    """Time-domain Michelson TDI on the GPU: (3, n_ext - added) float64 (A, E, T or X, Y, Z) from
+    (6, n_ext) links sampled at spacing dt, output sample i at link sample i + added
+    (TDMichelsonConstantEqualArm.update_tdi_response, tdi.py:1457-1600)."""
+    if not (isinstance(links, torch.Tensor) and links.is_cuda):
+        links = torch.as_tensor(links, dtype=torch.float64, device=_device(0))
+    links = links.to(torch.float64).contiguous()
+    if links.dim() != 2 or links.shape[0] != 6:
+        raise ValueError(f"links must be (6, n_ext), got {tuple(links.shape)}")
+    added = td_added_samples(arm_length, dt, tdi_generation)
+    n_out = links.shape[1] - added
+    if n_out < 0:
+        raise ValueError(f"{links.shape[1]} link samples, fewer than the {added} the delays need")
+    gen = 15 if str(tdi_generation) in ("1.5", "15") else 20
+    if out is None:
+        out = torch.empty(3, n_out, dtype=torch.float64, device=links.device)
+    else:
+        LikelihoodCore._check_out(out, "out", (3, n_out), links.device)
+    check(lib().pes_td_tdi(float(arm_length), float(dt), gen, int(bool(orthogonal)), links.data_ptr(),
+                           links.shape[1], out.data_ptr(), torch.cuda.current_stream(links.device).cuda_stream))
+    return out
+
+
 def whittle_lnl(data, response, psd, df):
     """-2 df sum |d - h|^2 / S over channels and bins, on the GPU (deterministic reduction).
 

@@ -442,6 +442,104 @@ __global__ __launch_bounds__(128) void k_orbit_vectors(DetectorDev det, const do
   }
 }
 
+// ---- time domain: single links of a sampled waveform and the Michelson TDI of sampled links ----
+// (the reference's TDResponseModelCornish2003, antenna.py:566-800, and TDMichelsonConstantEqualArm,
+// tdi.py:1388-1600, with its linear interpolation kernel, utils.py:83-100)
+struct TdWave {  // h+, hx sampled at t0 + m dt, m = 0 .. n - 1; zero outside
+  const double* hp;
+  const double* hc;
+  double t0, dt;
+  int n;
+};
+__device__ inline void td_wave_at(const TdWave& w, double t, double& hp, double& hc) {
+  const double x = (t - w.t0) / w.dt;
+  if (!(x >= 0.0) || x > (double)(w.n - 1)) {  // outside the samples (or NaN): no signal
+    hp = hc = 0.0;
+    return;
+  }
+  const int j = min((int)floor(x), w.n - 2);  // x = n - 1 exactly reads the last pair at fraction 1
+  const double fr = x - (double)j;
+  hp = w.hp[j] + (w.hp[j + 1] - w.hp[j]) * fr;
+  hc = w.hc[j] + (w.hc[j + 1] - w.hc[j]) * fr;
+}
+
+// y_rs(t) = n.[H(t - L - k.x_s) - H(t - k.x_r)].n / (2 (1 - k.n_sr)) at t_i = t0 + i dt, with the
+// constellation at t_i; out [6][n] in the order 12, 21, 23, 32, 31, 13 (link_rs: sent from s)
+__global__ __launch_bounds__(128) void k_links_td(DetectorDev det, TdWave w, Extrinsic ex, double t0, double dt, int n,
+                                                  double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double t = t0 + dt * (double)i;
+  Vec3 nv[3], xv[3];
+  constellation(det, t, nv, xv);
+  double Pp[3], Pc[3], kn[3], kx[3];
+  for (int a = 0; a < 3; ++a) {
+    const double np = dot(nv[a], ex.p), nq = dot(nv[a], ex.q);
+    Pp[a] = np * np - nq * nq;  // n.e+.n
+    Pc[a] = 2.0 * np * nq;      // n.ex.n
+    kn[a] = dot(ex.k, nv[a]);
+    kx[a] = dot(ex.k, xv[a]);
+  }
+  double sp[3], sc[3], rp[3], rc[3];  // sent (t - L - k.x) and received (t - k.x) waves at node a
+  for (int a = 0; a < 3; ++a) {
+    td_wave_at(w, t - det.L - kx[a], sp[a], sc[a]);
+    td_wave_at(w, t - kx[a], rp[a], rc[a]);
+  }
+  // link (r, s) lies along n_c with c the third node; 1 - k.n_sr = 1 + k.n_c when s -> r runs
+  // against n_c (links 12, 23, 31) and 1 - k.n_c along it (21, 32, 13)
+  const int R[6] = {0, 1, 1, 2, 2, 0}, S[6] = {1, 0, 2, 1, 0, 2}, C[6] = {2, 2, 0, 0, 1, 1};
+  const double sg[6] = {1.0, -1.0, 1.0, -1.0, 1.0, -1.0};
+  for (int l = 0; l < 6; ++l) {
+    const int r = R[l], s = S[l], c = C[l];
+    const double y = 0.5 * (Pp[c] * (sp[s] - rp[r]) + Pc[c] * (sc[s] - rc[r])) / (1.0 + sg[l] * kn[c]);
+    out[(size_t)l * n + i] = y;
+  }
+}
+
+// TDI of links sampled at t0 + j dt (j < n_ext = n_out + added, added = floor(max_delay L / dt) + 1):
+// output sample i at link index i + added, D^m y at the fractional index i + added - m L / dt by
+// linear interpolation; X = P(D) [y13 - y12 + D (y31 - y21)], Y, Z cyclic, with P = 1 - D^2 (1.5)
+// or (1 - D^2)(1 - D^4) (2.0); out [3][n_out] (A, E, T or X, Y, Z)
+__global__ __launch_bounds__(128) void k_tdi_td(const double* __restrict__ links, int n_ext, int n_out, int added,
+                                                double L_dt, int gen, int orthogonal, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  auto at = [&](int l, int m) {  // D^m y_l at output sample i
+    // x lies in [added - max_delay L / dt, n_ext - 1] with added - max_delay L / dt > 0; at the
+    // top end (m = 0, last sample) the last pair is read at fraction 1
+    const double x = (double)(i + added) - (double)m * L_dt;
+    const int j = min((int)floor(x), n_ext - 2);
+    const double fr = x - (double)j;
+    const double* y = links + (size_t)l * n_ext;
+    return y[j] + (y[j + 1] - y[j]) * fr;
+  };
+  const double P15[3] = {1.0, 0.0, -1.0}, P20[7] = {1.0, 0.0, -1.0, 0.0, -1.0, 0.0, 1.0};
+  const double* P = gen == 20 ? P20 : P15;
+  const int np = gen == 20 ? 7 : 3;
+  // links 0:12 1:21 2:23 3:32 4:31 5:13; channel c: (a, b, c2, e) with X = P (y_a - y_b + D (y_c2 - y_e))
+  const int comb[3][4] = {{5, 0, 4, 1}, {1, 2, 0, 3}, {3, 4, 2, 5}};
+  double ch[3];
+  for (int c = 0; c < 3; ++c) {
+    double v = 0.0;
+    for (int k = 0; k < np; ++k) {
+      if (P[k] == 0.0) continue;
+      v += P[k] * (at(comb[c][0], k) - at(comb[c][1], k) + at(comb[c][2], k + 1) - at(comb[c][3], k + 1));
+    }
+    ch[c] = v;
+  }
+  double o[3];
+  if (orthogonal) {
+    o[0] = (ch[2] - ch[0]) / sqrt(2.0);
+    o[1] = (ch[0] - 2.0 * ch[1] + ch[2]) / sqrt(6.0);
+    o[2] = (ch[0] + ch[1] + ch[2]) / sqrt(3.0);
+  } else {
+    o[0] = ch[0];
+    o[1] = ch[1];
+    o[2] = ch[2];
+  }
+  for (int c = 0; c < 3; ++c) out[(size_t)c * n_out + i] = o[c];
+}
+
 // ---- standalone Whittle lnL: -2 df sum_c sum_k |d - h|^2 / S (likelihood_ti.py:58-67) --------
 constexpr int WHITTLE_BLOCKS = 1024;
 __global__ __launch_bounds__(TILE) void k_whittle_partial(const double* __restrict__ data, const double* __restrict__ resp,
@@ -1153,6 +1251,45 @@ extern "C" int pes_orbit_vectors(int orbit_kind, double arm_length_m, double rot
                             lambda_ref, beta_ref);
   if (rc != PES_OK || n == 0) return rc;
   hipLaunchKernelGGL(k_orbit_vectors, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream, d, d_t, n, d_out);
+  HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" int pes_td_links(int orbit_kind, double arm_length_m, double rotation_initial, double revolution_initial,
+                            double omega_rotation, double lambda_ref, double beta_ref, const double* d_hp,
+                            const double* d_hc, double wave_t0, double wave_dt, int wave_n, double lam, double beta,
+                            double psi, double t0, double dt, int n, double* d_out, void* stream) {
+  if (n < 0 || wave_n < 2) return fail(PES_ERR_INVALID, "td links: n >= 0 and at least two waveform samples");
+  if (!(wave_dt > 0.0) || !(dt > 0.0)) return fail(PES_ERR_INVALID, "td links: sample spacings must be positive");
+  if (n > 0 && (!d_hp || !d_hc || !d_out)) return fail(PES_ERR_INVALID, "td links: arrays must be non-NULL");
+  DetectorDev d;
+  std::memset(&d, 0, sizeof(d));
+  const int rc = fill_orbit(d, orbit_kind, arm_length_m, rotation_initial, revolution_initial, omega_rotation,
+                            lambda_ref, beta_ref);
+  if (rc != PES_OK || n == 0) return rc;
+  const TdWave w{d_hp, d_hc, wave_t0, wave_dt, wave_n};
+  hipLaunchKernelGGL(k_links_td, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream, d, w,
+                     extrinsic_of(lam, beta, psi, 0.0), t0, dt, n, d_out);
+  HIP_TRY(hipGetLastError());
+  return PES_OK;
+}
+
+extern "C" int pes_td_tdi_added_samples(double arm_length_m, double dt, int tdi_generation) {
+  if (!(arm_length_m > 0.0) || !(dt > 0.0) || (tdi_generation != 15 && tdi_generation != 20)) return -1;
+  const int max_delay = tdi_generation == 20 ? 7 : 3;
+  return (int)std::floor(max_delay * (arm_length_m / C_SI) / dt) + 1;
+}
+
+extern "C" int pes_td_tdi(double arm_length_m, double dt, int tdi_generation, int orthogonal, const double* d_links,
+                          int n_ext, double* d_out, void* stream) {
+  const int added = pes_td_tdi_added_samples(arm_length_m, dt, tdi_generation);
+  if (added < 0) return fail(PES_ERR_INVALID, "td tdi: arm length and dt must be positive, generation 15 or 20");
+  const int n_out = n_ext - added;
+  if (n_out < 0) return fail(PES_ERR_INVALID, "td tdi: fewer link samples than the delays need");
+  if (n_out > 0 && (!d_links || !d_out)) return fail(PES_ERR_INVALID, "td tdi: arrays must be non-NULL");
+  if (n_out == 0) return PES_OK;
+  hipLaunchKernelGGL(k_tdi_td, dim3((n_out + 127) / 128), dim3(128), 0, (hipStream_t)stream, d_links, n_ext, n_out,
+                     added, (arm_length_m / C_SI) / dt, tdi_generation, orthogonal ? 1 : 0, d_out);
   HIP_TRY(hipGetLastError());
   return PES_OK;
 }

@@ -1,4 +1,4 @@
-"""Detector facade and the Marsat-Baker FD response model, hip backend.
+"""Detector facade, the Marsat-Baker FD response model and the time-domain response, hip backend.
 
 Mirrors pespace/detector/antenna.py:117-379: ``InterferometerAntenna`` owns the
 TDI data, orbit, response model and TDI combination.  As in the reference,
@@ -12,7 +12,9 @@ provided too (_antenna_jax.py:82-195, _tdi_jax.py:70-102).  The waveform may be 
 hip waveform model (``pespace_amd.waveform.IMRPhenomD/IMRPhenomHM``, evaluated on
 the device) or any reference-style container ``{"plus","cross","tf"}`` / mode-keyed
 dict of those (antenna.py:438-458).  The likelihood itself never materialises the
-links: the fused kernel takes them straight into the residual.
+links: the fused kernel takes them straight into the residual.  ``TDResponseModelCornish2003``
+with ``tdi.TDMichelsonConstantEqualArm`` is the time-domain counterpart (real time series on
+the TDI model's extended sample grid).
 """
 
 from __future__ import annotations
@@ -51,6 +53,66 @@ class FDResponseModelMarset2018:
         """antenna.py:403-460: the links of ``waveform`` stored as ``detector.single_link_response``
         (complex (6, N) tensor)."""
         self.detector.single_link_response = self.detector._links(waveform, lam, beta, psi, tc)
+
+
+class TDResponseModelCornish2003:
+    """Time-domain single-link response (Cornish & Rubbo 2003), the reference's commented
+    TDResponseModelCornish2003 (antenna.py:566-800), evaluated in a HIP kernel.
+
+    At each time t of the TDI model's extended link grid, with the constellation at t,
+    y_rs(t) = n.[H(t - L - k.x_s) - H(t - k.x_r)].n / (2 (1 - k.n_sr)) (link_rs sent from s,
+    received at r; antenna.py:681-800).  The waveform container is the reference's
+    ``{"t0": float, "data": (n, 2) array of [h+, hx]}`` (antenna.py:630-648), sampled at the
+    data's delta_time (or the container's ``"delta_time"``); its ``t0`` is the time of the first
+    sample relative to the coalescence time, so sample m sits at tc + t0 + m dt.  Between samples
+    the waveform is linearly interpolated (the ``"linear"`` kernel, utils.py:83-100) and outside
+    them it is zero, which is what the reference's padding in _ensure_waveform_length supplies.
+    """
+
+    def __init__(self, interpolate_kernel="linear", backend="hip"):
+        if interpolate_kernel != "linear":
+            raise ValueError("only the 'linear' interpolation kernel is implemented (as in the reference)")
+        if backend not in BACKENDS:
+            raise ValueError(f"Invalid backend: {backend}")
+        self.interpolate_kernel = interpolate_kernel
+        self.interpolate_kernel_length = 3
+        self.backend = backend
+        self.detector = None
+
+    def init_single_link_response_model(self, detector):
+        if getattr(detector.tdi_combination, "domain", None) != "td":
+            raise ValueError("the time-domain response needs a time-domain TDI combination "
+                             "(TDMichelsonConstantEqualArm)")
+        self.detector = detector
+        detector.single_link_response = None
+
+    def _links(self, waveform_container, lam, beta, psi, tc):
+        from ..core import td_links
+        det = self.detector
+        info = det.tdi_data.data_info
+        comb = det.tdi_combination
+        data = waveform_container["data"]
+        if isinstance(data, torch.Tensor):
+            data = data.to(torch.float64)
+        else:
+            data = np.asarray(data, dtype=np.float64)
+        if data.ndim != 2 or data.shape[1] != 2:
+            raise ValueError(f"waveform data must be (n, 2) [h+, hx], got {tuple(data.shape)}")
+        wdt = float(waveform_container.get("delta_time", info.delta_time))
+        orb = det.orbit_model
+        t_first = info.start_time - comb.added_time_samples_number * info.delta_time
+        return td_links(orb.kind, orb.arm_length, orb.rotation_initial, orb.revolution_initial, data[:, 0],
+                        data[:, 1], float(tc) + float(waveform_container["t0"]), wdt, lam, beta, psi, t_first,
+                        info.delta_time, comb.extended_time_series_length, **getattr(orb, "geo_params", {}))
+
+    def get_response_jit(self, waveform_container, lam, beta, psi, tc):
+        """Six real (extended_time_series_length,) GPU tensors; nothing is stored."""
+        links = self._links(waveform_container, lam, beta, psi, tc)
+        return tuple(links[i] for i in range(6))
+
+    def update_single_link_response(self, waveform_container, lam, beta, psi, tc):
+        """antenna.py:650-669: the links stored as ``detector.single_link_response`` (real (6, n_ext))."""
+        self.detector.single_link_response = self._links(waveform_container, lam, beta, psi, tc)
 
 
 class InterferometerAntenna:
@@ -119,12 +181,15 @@ class InterferometerAntenna:
         self.tdi_combination.update_tdi_response()
 
     def inject_signal(self, waveform, lam, beta, psi, tc):
-        """antenna.py:231-271."""
+        """antenna.py:231-271 (time-domain models add into the time-domain data)."""
         self.update_detector_response(waveform, lam, beta, psi, tc)
         chans = self.tdi_data.data_info.channels
         DataContent = namedtuple("DataContent", chans)
         resp = self.tdi_response_numpy
-        self.tdi_data.add_into_fd_data(DataContent(**{c: getattr(resp, c) for c in chans}))
+        if getattr(self.tdi_combination, "domain", "fd") == "td":
+            self.tdi_data.add_into_td_data(DataContent(**{c: getattr(resp, c) for c in chans}))
+        else:
+            self.tdi_data.add_into_fd_data(DataContent(**{c: getattr(resp, c) for c in chans}))
 
     @property
     def tdi_response_numpy(self):

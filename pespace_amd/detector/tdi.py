@@ -1,10 +1,10 @@
-"""TDI data container and the FD Michelson TDI model, hip backend.
+"""TDI data container and the Michelson TDI models, hip backend.
 
 Mirrors pespace/detector/tdi.py: ``DataInformation`` (tdi.py:64-160),
-``TDIChannelData`` (tdi.py:162-1376; the FD parts) and
-``FDMichelsonConstantEqualArm`` (tdi.py:1603-1647).  Grid, PSD and noise are
-setup-time host work (numpy, as in the reference); the per-evaluation response,
-TDI combination and likelihood run in the HIP library.
+``TDIChannelData`` (tdi.py:162-1376; the FD and TD parts), ``FDMichelsonConstantEqualArm``
+(tdi.py:1603-1647) and ``TDMichelsonConstantEqualArm`` (tdi.py:1388-1600).  Grid, PSD and
+noise are setup-time host work (numpy, as in the reference); the per-evaluation response, TDI
+combination and likelihood run in the HIP library.
 """
 
 from __future__ import annotations
@@ -119,6 +119,8 @@ class TDIChannelData:
     def _reset(self):
         self.frequency_samples = None
         self.fd_data = None
+        self.time_samples = None
+        self.td_data = None
         self.fd_noise_power_density = None
         self._data_info = None
         self._reset_flag = False
@@ -168,6 +170,68 @@ class TDIChannelData:
             raise ValueError(f"The length of second dimension of input array is {ns}, different from "
                              f"frequency_series_length={self._data_info.frequency_series_length}.")
         self._store(arr)
+
+    def _store_td(self, data_array):
+        chans = self._data_info.channels
+        DataContent = namedtuple("DataContent", chans)
+        self.time_samples = np.asarray(self._data_info.time_samples_array, dtype=np.float64)
+        self.td_data = DataContent(*[np.asarray(a, dtype=np.float64) for a in data_array])
+        self._reset_flag = True
+
+    def set_td_data_from_zero(self, channels, duration, delta_time, start_time=0.0, minimum_frequency=1e-5,
+                              maximum_frequency=0.1):
+        """tdi.py:521-583: zero time series at start_time + n delta_time, n < int(duration // delta_time + 1)."""
+        if self._reset_flag:
+            warnings.warn("You are setting `td_data` with zero value, but you have set TDI data previously; "
+                          "the instance is reset.")
+            self._reset()
+        self.set_data_info(channels, duration, delta_time, start_time, minimum_frequency, maximum_frequency)
+        self._store_td(np.zeros((len(channels), self._data_info.time_series_length)))
+
+    def set_td_data_from_input(self, channels, duration, delta_time, tdi_data_array, start_time=0.0,
+                               minimum_frequency=1e-5, maximum_frequency=0.1):
+        """tdi.py:330-414."""
+        if self._reset_flag:
+            warnings.warn("You are setting `td_data` with input array, but you have set TDI data previously; "
+                          "the instance is reset.")
+            self._reset()
+        self.set_data_info(channels, duration, delta_time, start_time, minimum_frequency, maximum_frequency)
+        arr = np.asarray(tdi_data_array)
+        nc, ns = arr.shape
+        if nc != len(channels):
+            raise ValueError(f"You set channels with {channels}, while the length of first dimension of input "
+                             f"tdi_data_array is {nc}.")
+        if ns != self._data_info.time_series_length:
+            raise ValueError(f"The length of second dimension of input array is {ns}. It is different with the "
+                             f"`time_series_length={self._data_info.time_series_length}` which is set according "
+                             "to the duration and delta_time by `int(duration//delta_time + 1)`.")
+        self._store_td(arr)
+
+    def add_into_td_data(self, input):
+        """Add a per-channel NamedTuple (numpy or torch) of time series into td_data (the
+        reference declares this and raises NotImplementedError, tdi.py:932-940)."""
+        if self.td_data is None:
+            raise ValueError("no `td_data`: set it with set_td_data_from_zero / _from_input first")
+        vals = self._named_arrays(input, self._data_info.time_series_length, "td_data")
+        DataContent = namedtuple("DataContent", self._data_info.channels)
+        self.td_data = DataContent(**{c: getattr(self.td_data, c) + vals[c] for c in self._data_info.channels})
+
+    def _named_arrays(self, input, n, target):
+        if not (isinstance(input, tuple) and hasattr(input, "_fields")):
+            raise TypeError(f"Expected input to be a NamedTuple instance, but got {type(input).__name__}.")
+        vals = {}
+        for c in input._fields:
+            a = getattr(input, c)
+            if hasattr(a, "detach"):
+                a = a.detach().cpu().numpy()
+            vals[c] = np.asarray(a)
+        if not all(v.shape == (n,) for v in vals.values()):
+            raise ValueError(f"Cannot add the input NamedTuple into the `{target}`, since there is at least one "
+                             "array in the input having different length with the TDI data.")
+        if set(input._fields) != set(self._data_info.channels):
+            raise ValueError(f"Cannot add the input NamedTuple of array into the `{target}`, since the channnels "
+                             "contained by input is different with the TDI data.")
+        return vals
 
     def set_fd_data_on_grid(self, channels, frequencies, weights, tdi_data_array=None):
         """Explicit frequency grid with per-bin weights (log-spaced grids)."""
@@ -235,21 +299,7 @@ class TDIChannelData:
 
     def add_into_fd_data(self, input):
         """tdi.py:942-1010: add a per-channel NamedTuple (numpy or torch) into fd_data."""
-        if not (isinstance(input, tuple) and hasattr(input, "_fields")):
-            raise TypeError(f"Expected input to be a NamedTuple instance, but got {type(input).__name__}.")
-        n = self._data_info.frequency_series_length
-        vals = {}
-        for c in input._fields:
-            a = getattr(input, c)
-            if hasattr(a, "detach"):
-                a = a.detach().cpu().numpy()
-            vals[c] = np.asarray(a)
-        if not all(v.shape == (n,) for v in vals.values()):
-            raise ValueError("Cannot add the input NamedTuple into the `fd_data`, since there is at least one "
-                             "array in the input having different length with the TDI data.")
-        if set(input._fields) != set(self._data_info.channels):
-            raise ValueError("Cannot add the input NamedTuple of array into the `fd_data`, since the channnels "
-                             "contained by input is different with the TDI data.")
+        vals = self._named_arrays(input, self._data_info.frequency_series_length, "fd_data")
         DataContent = namedtuple("DataContent", self._data_info.channels)
         self.fd_data = DataContent(**{c: getattr(self.fd_data, c) + vals[c] for c in self._data_info.channels})
 
@@ -268,6 +318,9 @@ class TDIChannelData:
                 "class_attrs/_np_fp": "float64", "class_attrs/_np_cp": "complex128"}
         for key in _DATA_INFO_KEYS:
             tree[f"data_info/{key}"] = np.array(info.channels if key == "channels" else getattr(info, key))
+        if self.td_data is not None:
+            for chan in info.channels:
+                tree[f"tdi_data/td_data/{chan}"] = np.asarray(getattr(self.td_data, chan))
         if self.fd_data is not None:
             for chan in info.channels:
                 tree[f"tdi_data/fd_data/{chan}"] = np.asarray(getattr(self.fd_data, chan))
@@ -281,8 +334,8 @@ class TDIChannelData:
     def _from_file_tree(tree, filename="<tree>"):
         """Rebuild an instance from a dataset tree, checking every stored derived quantity of
         the grid against its recomputation (tdi.py:1120-1292).  Data of any stored backend load
-        into a hip container; time-domain and wavelet-domain data are not held by it and are
-        skipped with a warning."""
+        into a hip container; wavelet-domain data are not held by it and are skipped with a
+        warning."""
         out = TDIChannelData(scaling=bool(np.asarray(tree["class_attrs/scaling"])), backend="hip")
         fp, cp = _text(tree["class_attrs/_np_fp"]), _text(tree["class_attrs/_np_cp"])
         assert fp == "float64" and cp == "complex128", (
@@ -300,9 +353,14 @@ class TDIChannelData:
             same = np.array_equal(mine, stored) if np.ndim(mine) else mine == stored.item()
             assert same, (f"Failed to recover TDIChannelData from file {filename}, the `{key}` is different "
                           "from the stored value.")
-        for domain in ("td_data", "wd_data"):
-            if any(k.startswith(f"tdi_data/{domain}/") for k in tree):
-                warnings.warn(f"{filename}: `{domain}` is not held by the hip TDIChannelData and is skipped.")
+        if any(k.startswith("tdi_data/wd_data/") for k in tree):
+            warnings.warn(f"{filename}: `wd_data` is not held by the hip TDIChannelData and is skipped.")
+        if any(k.startswith("tdi_data/td_data/") for k in tree):
+            arrs = [np.asarray(tree[f"tdi_data/td_data/{c}"]) for c in info.channels]
+            if any(a.shape != (info.time_series_length,) for a in arrs):
+                raise ValueError(f"{filename}: stored td_data do not have time_series_length="
+                                 f"{info.time_series_length} samples")
+            out._store_td(arrs)
         if any(k.startswith("tdi_data/fd_data/") for k in tree):
             out._store([np.asarray(tree[f"tdi_data/fd_data/{c}"]) for c in info.channels])
         if any(k.startswith("noise_data/wd_noise_power_density/") for k in tree):
@@ -388,5 +446,89 @@ class FDMichelsonConstantEqualArm:
         if links is None:
             raise ValueError("no single-link response to combine: update the single-link response first")
         out = self.detector._tdi_from_links(links)
+        DataContent = namedtuple("DataContent", self.labels)
+        self.detector.tdi_response = DataContent(*[out[i] for i in range(3)])
+
+
+class TDMichelsonConstantEqualArm:
+    """Time-domain Michelson TDI 1.5 / 2.0 for constant equal arms, A/E/T or X/Y/Z
+    (tdi.py:1388-1600, marked incomplete there); evaluated in a HIP kernel.
+
+    The single links are sampled on an extended grid that starts ``added_time_samples_number``
+    samples before the data's first time sample (int(max_num_delay L // dt + 1), max_num_delay 3
+    or 7, tdi.py:1436-1455); output sample i combines the links at extended sample
+    i + added_time_samples_number and their delays y(t - m L), linearly interpolated.  Where the
+    reference's loop (tdi.py:1482-1497) takes the base sample i + floor((max_num_delay - m) L / dt)
+    with fraction 1 - frac(m L / dt), this reads at i + added - m L / dt, the delay's
+    definition: the two agree except where frac(m L / dt) > frac(max_num_delay L / dt), where the
+    reference's index is one sample early.
+    """
+
+    domain = "td"
+
+    def __init__(self, generation="1.5", orthogonal=True, backend="hip"):
+        self.generation = str(generation)
+        if self.generation == "1.5":
+            self.max_num_delay = 3
+        elif self.generation == "2.0":
+            self.max_num_delay = 7
+        else:
+            raise ValueError(f"Unsupported generation {self.generation}.")
+        self.orthogonal = bool(orthogonal)
+        self.labels = ("A", "E", "T") if self.orthogonal else ("X", "Y", "Z")
+        if backend not in BACKENDS:
+            raise ValueError(f"Unknown backend: {backend}")
+        self.backend = backend
+        self.added_time_samples_number = None
+        self.extended_time_series_length = None
+        self.detector = None
+
+    @property
+    def generation_code(self):
+        return 15 if self.generation == "1.5" else 20
+
+    def init_tdi_combination_model(self, detector):
+        """tdi.py:1435-1455: sizes of the extended link grid."""
+        from ..core import td_added_samples
+        self.detector = detector
+        detector.tdi_response = None
+        info = detector.tdi_data.data_info
+        if info is None or not isinstance(info, DataInformation):
+            raise ValueError("the time-domain TDI needs the data's time grid: set the TDI data (set_td_data_...) "
+                             "before building the detector")
+        self.added_time_samples_number = td_added_samples(detector.orbit_model.arm_length, info.delta_time,
+                                                          self.generation)
+        self.extended_time_series_length = int(info.time_series_length + self.added_time_samples_number)
+
+    @property
+    def extended_time_samples_array(self):
+        """Times of the extended link grid: start_time + (j - added) delta_time."""
+        info = self.detector.tdi_data.data_info
+        j = np.arange(self.extended_time_series_length, dtype=np.float64) - self.added_time_samples_number
+        return info.start_time + j * info.delta_time
+
+    def get_combination_jit(self, single_link_response):
+        """Three channels (each a real (time_series_length,) GPU tensor) from the six links
+        (link12, link21, link23, link32, link31, link13), each (extended_time_series_length,)."""
+        import torch
+        from ..core import td_tdi
+        links = torch.stack([torch.as_tensor(x) for x in single_link_response])
+        out = self._combine(links, td_tdi)
+        return tuple(out[i] for i in range(3))
+
+    def _combine(self, links, td_tdi):
+        info = self.detector.tdi_data.data_info
+        if links.shape[-1] != self.extended_time_series_length:
+            raise ValueError(f"links have {links.shape[-1]} samples; the extended grid has "
+                             f"{self.extended_time_series_length}")
+        return td_tdi(self.detector.orbit_model.arm_length, info.delta_time, self.generation, self.orthogonal, links)
+
+    def update_tdi_response(self):
+        """tdi.py:1457-1600: combine the detector's stored single links into ``tdi_response``."""
+        from ..core import td_tdi
+        links = getattr(self.detector, "single_link_response", None)
+        if links is None:
+            raise ValueError("no single-link response to combine: update the single-link response first")
+        out = self._combine(links, td_tdi)
         DataContent = namedtuple("DataContent", self.labels)
         self.detector.tdi_response = DataContent(*[out[i] for i in range(3)])

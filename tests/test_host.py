@@ -68,6 +68,22 @@ def test_c_abi_argument_validation():
     assert b"arm length" in L.pes_last_error()
     assert L.pes_context_reserve(None, 8, 1) == 1
     assert L.pes_context_pin_scratch(None, 1) == 1
+    # time domain: validation precedes any device work
+    td_args = [0, 2.5e9, 0.0, 0.0, 0.0, 0.0, 0.0, None, None, 0.0, 1.0]
+    assert L.pes_td_links(*td_args, 1, 0.1, 0.2, 0.3, 0.0, 1.0, 4, None, None) == 1  # one waveform sample
+    assert L.pes_td_links(*td_args, 8, 0.1, 0.2, 0.3, 0.0, 1.0, 4, None, None) == 1  # NULL arrays
+    assert L.pes_td_links(*td_args[:10], -1.0, 8, 0.1, 0.2, 0.3, 0.0, 1.0, 4, None, None) == 1
+    assert L.pes_td_links(*td_args, 8, 0.1, 0.2, 0.3, 0.0, 1.0, -1, None, None) == 1
+    assert L.pes_td_links(2, *td_args[1:], 8, 0.1, 0.2, 0.3, 0.0, 1.0, 0, None, None) == 1  # orbit kind
+    assert L.pes_td_links(*td_args, 8, 0.1, 0.2, 0.3, 0.0, 1.0, 0, None, None) == 0  # nothing to do
+    assert L.pes_td_tdi_added_samples(2.5e9, 5.0, 15) == int(3 * (2.5e9 / 299792458.0) // 5.0 + 1)
+    assert L.pes_td_tdi_added_samples(2.5e9, 5.0, 16) == -1
+    assert L.pes_td_tdi_added_samples(2.5e9, 0.0, 20) == -1
+    assert L.pes_td_tdi(2.5e9, 5.0, 15, 1, None, 3, None, None) == 1  # fewer samples than the delays need
+    assert b"fewer link samples" in L.pes_last_error()
+    assert L.pes_td_tdi(2.5e9, 5.0, 15, 1, None, 10, None, None) == 1  # NULL arrays
+    assert L.pes_td_tdi(2.5e9, 5.0, 21, 1, None, 10, None, None) == 1
+    assert L.pes_td_tdi(2.5e9, 5.0, 15, 1, None, 6, None, None) == 0  # zero outputs
 
 
 def test_lnl_output_buffers_are_checked():
@@ -138,6 +154,66 @@ def test_tdi_channel_data_errors():
         td.add_into_fd_data({"A": np.zeros(n)})
     with pytest.raises(ValueError):
         FDMichelsonConstantEqualArm(generation="3.0")
+
+
+@pytest.mark.parametrize("arm, dt", [(2.5e9, 5.0), (3.0e9, 0.5), (1.7e8, 0.25), (8 * 299792458.0, 2.0),
+                                     (2.5e9, 8.339102379953802)])
+def test_td_added_samples_follow_the_reference(arm, dt):
+    """int(max_num_delay * L // dt + 1) (tdi.py:1445-1450), through the C-ABI, against the oracle."""
+    from oracle import td as otd
+    from pespace_amd.core import td_added_samples
+    L = arm / 299792458.0
+    for gen, m in (("1.5", 3), ("2.0", 7)):
+        assert td_added_samples(arm, dt, gen) == int(m * L // dt + 1) == otd.added_samples(arm, dt, gen)
+    with pytest.raises(ValueError):
+        td_added_samples(arm, dt, "3.0")
+
+
+def test_td_data_container_and_models():
+    from pespace_amd.detector.antenna import FDResponseModelMarset2018, InterferometerAntenna, TDResponseModelCornish2003
+    from pespace_amd.detector.tdi import TDIChannelData, TDMichelsonConstantEqualArm
+    td = TDIChannelData()
+    with pytest.raises(ValueError):
+        td.set_td_data_from_input(("A", "E", "T"), 100.0, 5.0, np.zeros((3, 20)))  # 21 samples expected
+    x = np.arange(63, dtype=np.float64).reshape(3, 21)
+    td.set_td_data_from_input(("A", "E", "T"), 100.0, 5.0, x, start_time=10.0)
+    assert td.data_info.time_series_length == 21
+    np.testing.assert_array_equal(td.time_samples, 10.0 + 5.0 * np.arange(21))
+    DC = namedtuple("DC", ("A", "E", "T"))
+    td.add_into_td_data(DC(*np.ones((3, 21))))
+    np.testing.assert_array_equal(td.td_data.E, x[1] + 1)
+    with pytest.raises(ValueError):
+        td.add_into_td_data(DC(*np.ones((3, 20))))
+    # the reference's file layout holds td_data under tdi_data/td_data/<channel>
+    tree = td._file_tree()
+    assert set(k for k in tree if k.startswith("tdi_data/")) == {f"tdi_data/td_data/{c}" for c in "AET"}
+    back = TDIChannelData._from_file_tree(tree)
+    np.testing.assert_array_equal(back.td_data.T, td.td_data.T)
+    with pytest.warns(UserWarning):
+        td.set_td_data_from_zero(("X", "Y", "Z"), 100.0, 5.0)
+    with pytest.raises(ValueError):
+        TDMichelsonConstantEqualArm(generation="1.0")
+    comb = TDMichelsonConstantEqualArm("2.0", orthogonal=False)
+    assert comb.labels == ("X", "Y", "Z") and comb.max_num_delay == 7 and comb.domain == "td"
+    det = InterferometerAntenna("LISA", td, "LISA_analytic", TDResponseModelCornish2003(), comb)
+    L = 2.5e9 / 299792458.0
+    assert comb.added_time_samples_number == int(7 * L // 5.0 + 1)
+    assert comb.extended_time_series_length == 21 + comb.added_time_samples_number
+    ext = comb.extended_time_samples_array
+    assert ext.size == comb.extended_time_series_length and ext[comb.added_time_samples_number] == 0.0
+    assert np.all(np.diff(ext) == 5.0)
+    assert det.single_link_response is None
+    with pytest.raises(ValueError):
+        TDResponseModelCornish2003(interpolate_kernel=("lagrange", 5))
+    with pytest.raises(ValueError):  # a time-domain response needs a time-domain TDI model
+        from pespace_amd.detector.tdi import FDMichelsonConstantEqualArm
+        td2 = TDIChannelData()
+        td2.set_fd_data_from_zero(("A", "E", "T"), 100.0, 5.0)
+        InterferometerAntenna("LISA", td2, "LISA_analytic", TDResponseModelCornish2003(),
+                              FDMichelsonConstantEqualArm())
+    with pytest.raises(ValueError):  # and a time grid
+        InterferometerAntenna("LISA", TDIChannelData(), "LISA_analytic", FDResponseModelMarset2018(),
+                              TDMichelsonConstantEqualArm())
 
 
 def test_antenna_and_orbit_errors():

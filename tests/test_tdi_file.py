@@ -94,10 +94,20 @@ def test_precision_mismatch_is_rejected():
 
 def test_other_domains():
     tree = _data()._file_tree()
-    tree["tdi_data/td_data/A"] = np.zeros(4)
-    with pytest.warns(UserWarning, match="td_data"):
-        back = TDIChannelData._from_file_tree(tree)
-    assert back.fd_data is not None
+    chans = [k.split("/")[-1] for k in tree if k.startswith("tdi_data/fd_data/")]
+    n_t = int(np.asarray(tree["data_info/time_series_length"]))
+    for i, c in enumerate(chans):  # time-domain data load beside the FD data
+        tree[f"tdi_data/td_data/{c}"] = np.full(n_t, float(i))
+    back = TDIChannelData._from_file_tree(tree)
+    assert back.fd_data is not None and back.td_data is not None
+    np.testing.assert_array_equal(getattr(back.td_data, chans[-1]), np.full(n_t, len(chans) - 1.0))
+    bad = dict(tree)
+    bad[f"tdi_data/td_data/{chans[0]}"] = np.zeros(4)
+    with pytest.raises(ValueError, match="time_series_length"):
+        TDIChannelData._from_file_tree(bad)
+    tree["tdi_data/wd_data/A"] = np.zeros(4)
+    with pytest.warns(UserWarning, match="wd_data"):
+        TDIChannelData._from_file_tree(tree)
     tree["noise_data/wd_noise_power_density/A"] = np.zeros(4)
     with pytest.warns(UserWarning), pytest.raises(NotImplementedError):
         TDIChannelData._from_file_tree(tree)
