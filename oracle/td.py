@@ -41,9 +41,10 @@ def added_samples(arm_length, dt, generation="1.5"):
     return int(np.floor(MAX_DELAY[generation] * (arm_length / C_SI) / dt)) + 1
 
 
-def _interp(y, x):
-    """Linear interpolation of samples y at fractional indices x; 0 outside [0, len - 1]."""
-    ok = (x >= 0) & (x <= y.size - 1)
+def _interp(y, x, zero_outside=True):
+    """Linear interpolation of samples y at fractional indices x; 0 outside [0, len - 1] (or,
+    zero_outside=False, the end pair's line)."""
+    ok = (x >= 0) & (x <= y.size - 1) if zero_outside else np.ones(np.shape(x), bool)
     jj = np.clip(np.floor(np.where(ok, x, 0.0)), 0, y.size - 2).astype(np.int64)
     fr = x - jj
     return np.where(ok, y[jj] + (y[jj + 1] - y[jj]) * fr, 0.0)
@@ -85,8 +86,8 @@ def tdi(link_samples, arm_length, dt, generation="1.5", orthogonal=True):
     r = (arm_length / C_SI) / dt
     i = np.arange(n_out, dtype=np.float64)
 
-    def D(name, m):
-        return _interp(y[name], i + added - m * r)
+    def D(name, m):  # in [added - max_delay r, n_ext - 1]: inside the samples up to rounding
+        return _interp(y[name], i + added - m * r, zero_outside=False)
 
     def chan(a, b, c, e):
         v = np.zeros(n_out)

@@ -505,10 +505,11 @@ __global__ __launch_bounds__(128) void k_tdi_td(const double* __restrict__ links
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_out) return;
   auto at = [&](int l, int m) {  // D^m y_l at output sample i
-    // x lies in [added - max_delay L / dt, n_ext - 1] with added - max_delay L / dt > 0; at the
-    // top end (m = 0, last sample) the last pair is read at fraction 1
+    // x lies in [added - max_delay L / dt, n_ext - 1] with added - max_delay L / dt > 0 up to
+    // rounding (added is floor(max_delay L / dt) + 1); the index is clamped to the samples both
+    // ways, so the top end (m = 0, last sample) reads the last pair at fraction 1
     const double x = (double)(i + added) - (double)m * L_dt;
-    const int j = min((int)floor(x), n_ext - 2);
+    const int j = max(0, min((int)floor(x), n_ext - 2));
     const double fr = x - (double)j;
     const double* y = links + (size_t)l * n_ext;
     return y[j] + (y[j + 1] - y[j]) * fr;
