@@ -259,6 +259,45 @@ def cpu_baseline(dets, wf, rows, n_rows=2):
                        f"CPU backend: 144 ns per bin per lnL (XAS, 65,471 bins)")
 
 
+def td_measure(reps=20, dt=5.0):
+    """Time-domain links (k_links_td) and TDI 1.5 A/E/T (k_tdi_td) over one year of LISA samples
+    at 5 s (6,311,521 outputs plus the delays' lead-in) for a chirping waveform sampled at the
+    same spacing; HIP events on the launch stream around `reps` launches each.  Algorithmic HBM
+    bytes per output sample: links 48 written + 16 read (h+, hx; the neighbouring samples'
+    reads hit the same lines), TDI 48 read + 24 written."""
+    import torch
+    from pespace_amd.core import td_added_samples, td_links, td_tdi
+    arm = 2.5e9
+    n = int(YEAR_SI // dt + 1)
+    added = td_added_samples(arm, dt, "1.5")
+    t = torch.arange(n + added, dtype=torch.float64, device="cuda") * dt
+    ph = 2 * np.pi * (1e-4 * t + 0.5e-12 * t * t)
+    amp = 1e-21 * torch.sin(np.pi * t / t[-1]) ** 2
+    hp, hc = (amp * torch.cos(ph)).contiguous(), (amp * torch.sin(ph)).contiguous()
+    links = torch.empty(6, n + added, dtype=torch.float64, device="cuda")
+    aet = torch.empty(3, n, dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream()
+    out = {}
+    for name, fn, nbytes in (
+            ("links", lambda: td_links(0, arm, 0.0, -np.pi / 9, hp, hc, -added * dt, dt, 1.375, -1.2108, 2.659,
+                                       -added * dt, dt, n + added, out=links), (n + added) * 64),
+            ("tdi", lambda: td_tdi(arm, dt, "1.5", True, links, out=aet), n * 72)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        out[name] = {"us_per_launch": us, "algorithmic_bytes": nbytes, "GB_per_s": nbytes / us * 1e-3,
+                     "frac_of_hbm_peak": nbytes / (us * 1e-6) / 8e12}
+    out["samples"] = n
+    out["samples_per_s"] = n / ((out["links"]["us_per_launch"] + out["tdi"]["us_per_launch"]) * 1e-6)
+    return out
+
+
 def copy_calibration(n_bytes=1 << 30, reps=3):
     """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel calibrate the counters."""
     import torch
@@ -284,6 +323,8 @@ def main():
                     help="skip the mixed vs FP64 sweep over uneven wide-prior batches (256, 3001, 8192 rows)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule of the fused kernel")
     ap.add_argument("--schedule", default=None, help="fixed schedule 'tiles_per_group,blocks_per_cu' (no autotune)")
+    ap.add_argument("--no-td", action="store_true", help="skip the time-domain links/TDI timing")
+    ap.add_argument("--td-only", action="store_true", help="only time the time-domain links and TDI")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -298,6 +339,9 @@ def main():
     from pespace_amd.core import LikelihoodCore
     from pespace_amd.detector.antenna import add_detector_to_core
 
+    if args.td_only:
+        print(json.dumps({"td_response": td_measure()}), flush=True)
+        return
     grad = not args.no_grad
     t_setup = time.perf_counter()
     dets, wf = build_network(args.workload)
@@ -398,6 +442,8 @@ def main():
     if grad and not args.no_prior_sweep and WORKLOADS[args.workload]["detectors"] == ("LISA", "Taiji", "TianQin") \
             and WORKLOADS[args.workload]["grid"] == "year":
         line["mixed_prior_batches"] = prior_sweep(core)
+    if not args.no_td:
+        line["td_response"] = td_measure()
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(dets, wf, rows)

@@ -85,7 +85,13 @@ def test_td_links_and_tdi_one_year():
     links = _td_links_gpu(orb, hp, hc, 0.0, dt, SKY, -added * dt, dt, n + added)
     got = links.cpu().numpy()
     ref = otd.links(orb, hp, hc, 0.0, dt, *SKY, -added * dt, dt, n + added)
-    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.max(np.abs(hp)))
+    # FP64 resolves a delay time t - L - k.x near t ~ 3e7 s to ~4e-9 s; one ulp there moves a
+    # pulse by |dh/dt| 4e-9 s, with |dh/dt| <= 2 pi f_max |h| (about 1e-8 of a link at 1e-4 Hz)
+    t_max = dt * (n + added)
+    f_max = 1e-4 + 1e-12 * t_max
+    tol = 8 * np.finfo(float).eps * t_max * 2 * np.pi * f_max * np.max(np.abs(hp))
+    np.testing.assert_allclose(got, ref, rtol=0, atol=tol)
+    assert tol < 1e-6 * np.max(np.abs(got))
     del ref
     aet = td_tdi(orb[1], dt, "1.5", True, links).cpu().numpy()
     ref = otd.tdi(got, orb[1], dt, "1.5", True)
