@@ -16,7 +16,8 @@ out (pespace/detector/antenna.py:566-800) and TDMichelsonConstantEqualArm
   or 1 - D^2 - D^4 + D^6 (2.0), D^m y(t) = y(t - m L) by linear interpolation between samples;
   the links are sampled on an extended grid that starts ``added`` samples before the output,
   added = int(max_delay L // dt + 1) with max_delay 3 (1.5) or 7 (2.0) (tdi.py:1445-1455).
-  Output sample i reads D^m y at the fractional link index i + added - m L / dt.  (The
+  Output sample i reads D^m y at the fractional link index i + added - m L / dt, whose integer
+  part and fraction are split once per delay (i is an integer).  (The
   reference's loop, tdi.py:1482-1497, takes the base index i + floor((max_delay - m) L / dt)
   with fraction 1 - frac(m L / dt); that equals i + added - m L / dt except where
   frac(m L / dt) > frac(max_delay L / dt), where it reads one sample early.  The restatement
@@ -41,10 +42,9 @@ def added_samples(arm_length, dt, generation="1.5"):
     return int(np.floor(MAX_DELAY[generation] * (arm_length / C_SI) / dt)) + 1
 
 
-def _interp(y, x, zero_outside=True):
-    """Linear interpolation of samples y at fractional indices x; 0 outside [0, len - 1] (or,
-    zero_outside=False, the end pair's line)."""
-    ok = (x >= 0) & (x <= y.size - 1) if zero_outside else np.ones(np.shape(x), bool)
+def _interp(y, x):
+    """Linear interpolation of samples y at fractional indices x; 0 outside [0, len - 1]."""
+    ok = (x >= 0) & (x <= y.size - 1)
     jj = np.clip(np.floor(np.where(ok, x, 0.0)), 0, y.size - 2).astype(np.int64)
     fr = x - jj
     return np.where(ok, y[jj] + (y[jj + 1] - y[jj]) * fr, 0.0)
@@ -84,10 +84,19 @@ def tdi(link_samples, arm_length, dt, generation="1.5", orthogonal=True):
     added = added_samples(arm_length, dt, generation)
     n_out = n_ext - added
     r = (arm_length / C_SI) / dt
-    i = np.arange(n_out, dtype=np.float64)
+    i = np.arange(n_out)
 
-    def D(name, m):  # in [added - max_delay r, n_ext - 1]: inside the samples up to rounding
-        return _interp(y[name], i + added - m * r, zero_outside=False)
+    def D(name, m):
+        """y at the fractional index i + added - m r, split once into an integer base and a
+        fraction (i is an integer); the base lies in [0, added - 1] for m >= 1 (added - m r > 0
+        up to rounding), so every read is inside the samples."""
+        if m == 0:
+            return y[name][i + added]
+        c = added - m * r
+        jb = min(max(int(np.floor(c)), 0), added - 1)
+        fr = c - jb
+        a, b = y[name][i + jb], y[name][i + jb + 1]
+        return a + (b - a) * fr
 
     def chan(a, b, c, e):
         v = np.zeros(n_out)

@@ -448,97 +448,123 @@ __global__ __launch_bounds__(128) void k_orbit_vectors(DetectorDev det, const do
 struct TdWave {  // h+, hx sampled at t0 + m dt, m = 0 .. n - 1; zero outside
   const double* hp;
   const double* hc;
-  double t0, dt;
+  double t0, inv_dt;
   int n;
 };
-__device__ inline void td_wave_at(const TdWave& w, double t, double& hp, double& hc) {
-  const double x = (t - w.t0) / w.dt;
+// h+, hx at the fractional sample x = (t - t0) / dt: linear between samples, zero outside
+__device__ inline void td_wave_at(const TdWave& w, double x, double& hp, double& hc) {
   if (!(x >= 0.0) || x > (double)(w.n - 1)) {  // outside the samples (or NaN): no signal
     hp = hc = 0.0;
     return;
   }
   const int j = min((int)floor(x), w.n - 2);  // x = n - 1 exactly reads the last pair at fraction 1
   const double fr = x - (double)j;
-  hp = w.hp[j] + (w.hp[j + 1] - w.hp[j]) * fr;
-  hc = w.hc[j] + (w.hc[j + 1] - w.hc[j]) * fr;
+  const double p0 = w.hp[j], p1 = w.hp[j + 1], c0 = w.hc[j], c1 = w.hc[j + 1];
+  hp = p0 + (p1 - p0) * fr;
+  hc = c0 + (c1 - c0) * fr;
 }
 
 // y_rs(t) = n.[H(t - L - k.x_s) - H(t - k.x_r)].n / (2 (1 - k.n_sr)) at t_i = t0 + i dt, with the
-// constellation at t_i; out [6][n] in the order 12, 21, 23, 32, 31, 13 (link_rs: sent from s)
-__global__ __launch_bounds__(128) void k_links_td(DetectorDev det, TdWave w, Extrinsic ex, double t0, double dt, int n,
+// constellation at t_i; out [6][n] in the order 12, 21, 23, 32, 31, 13 (link_rs: sent from s).
+// One thread per sample: the orbit (one sincos), six waveform positions in sample units
+// (x = (t - t_w0) / dt - (L + k.x_a) / dt), twelve interpolations and three divisions
+// (1 / (1 +- k.n_c) from 1 / (1 - (k.n_c)^2)).  Neighbouring threads read neighbouring samples.
+__global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Extrinsic ex, double t0, double dt, int n,
                                                   double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double t = t0 + dt * (double)i;
   Vec3 nv[3], xv[3];
   constellation(det, t, nv, xv);
-  double Pp[3], Pc[3], kn[3], kx[3];
+  const double base = (t - w.t0) * w.inv_dt, Ls = det.L * w.inv_dt;
+  double Pp[3], Pc[3], kn[3], sp[3], sc[3], rp[3], rc[3];  // sent (t - L - k.x_a), received (t - k.x_a)
+#pragma unroll
   for (int a = 0; a < 3; ++a) {
     const double np = dot(nv[a], ex.p), nq = dot(nv[a], ex.q);
     Pp[a] = np * np - nq * nq;  // n.e+.n
     Pc[a] = 2.0 * np * nq;      // n.ex.n
     kn[a] = dot(ex.k, nv[a]);
-    kx[a] = dot(ex.k, xv[a]);
-  }
-  double sp[3], sc[3], rp[3], rc[3];  // sent (t - L - k.x) and received (t - k.x) waves at node a
-  for (int a = 0; a < 3; ++a) {
-    td_wave_at(w, t - det.L - kx[a], sp[a], sc[a]);
-    td_wave_at(w, t - kx[a], rp[a], rc[a]);
+    const double xr = base - dot(ex.k, xv[a]) * w.inv_dt;
+    td_wave_at(w, xr - Ls, sp[a], sc[a]);
+    td_wave_at(w, xr, rp[a], rc[a]);
   }
   // link (r, s) lies along n_c with c the third node; 1 - k.n_sr = 1 + k.n_c when s -> r runs
   // against n_c (links 12, 23, 31) and 1 - k.n_c along it (21, 32, 13)
-  const int R[6] = {0, 1, 1, 2, 2, 0}, S[6] = {1, 0, 2, 1, 0, 2}, C[6] = {2, 2, 0, 0, 1, 1};
-  const double sg[6] = {1.0, -1.0, 1.0, -1.0, 1.0, -1.0};
-  for (int l = 0; l < 6; ++l) {
-    const int r = R[l], s = S[l], c = C[l];
-    const double y = 0.5 * (Pp[c] * (sp[s] - rp[r]) + Pc[c] * (sc[s] - rc[r])) / (1.0 + sg[l] * kn[c]);
-    out[(size_t)l * n + i] = y;
+  double up[3], dn[3];  // 0.5 / (1 + k.n_c), 0.5 / (1 - k.n_c)
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double h = 0.5 / fma(-kn[c], kn[c], 1.0);
+    up[c] = (1.0 - kn[c]) * h;
+    dn[c] = (1.0 + kn[c]) * h;
   }
+  auto link = [&](int r, int s, int c, double f) {
+    return f * (Pp[c] * (sp[s] - rp[r]) + Pc[c] * (sc[s] - rc[r]));
+  };
+  const size_t N = (size_t)n;
+  out[i] = link(0, 1, 2, up[2]);          // 12
+  out[N + i] = link(1, 0, 2, dn[2]);      // 21
+  out[2 * N + i] = link(1, 2, 0, up[0]);  // 23
+  out[3 * N + i] = link(2, 1, 0, dn[0]);  // 32
+  out[4 * N + i] = link(2, 0, 1, up[1]);  // 31
+  out[5 * N + i] = link(0, 2, 1, dn[1]);  // 13
 }
 
-// TDI of links sampled at t0 + j dt (j < n_ext = n_out + added, added = floor(max_delay L / dt) + 1):
-// output sample i at link index i + added, D^m y at the fractional index i + added - m L / dt by
-// linear interpolation; X = P(D) [y13 - y12 + D (y31 - y21)], Y, Z cyclic, with P = 1 - D^2 (1.5)
-// or (1 - D^2)(1 - D^4) (2.0); out [3][n_out] (A, E, T or X, Y, Z)
-__global__ __launch_bounds__(128) void k_tdi_td(const double* __restrict__ links, int n_ext, int n_out, int added,
-                                                double L_dt, int gen, int orthogonal, double* __restrict__ out) {
+// Integer and fractional parts of the delays m L / dt on the extended link grid: D^m y at output
+// sample i is y[i + jb[m]] + (y[i + jb[m] + 1] - y[i + jb[m]]) fr[m], with jb[m] + fr[m] =
+// added - m L / dt (jb[0] = added, fr[0] = 0)
+struct TdDelays {
+  int jb[8];
+  double fr[8];
+};
+
+// TDI of links sampled on the extended grid (n_ext = n_out + added samples): output sample i at
+// link sample i + added; X = P(D) [y13 - y12 + D (y31 - y21)], Y, Z cyclic, P = 1 - D^2 (1.5,
+// MAXD = 3) or (1 - D^2)(1 - D^4) (2.0, MAXD = 7); out [3][n_out] (A, E, T or X, Y, Z).  Every
+// link is needed at every delay 0..MAXD (as y_a / y_b of one channel at even m, as y_c / y_e of
+// another at odd m); each delayed value is added into the three channel sums as it is read.
+constexpr int TD_CHAN[3][4] = {{5, 0, 4, 1}, {1, 2, 0, 3}, {3, 4, 2, 5}};  // (a, b, c, e) per X, Y, Z
+__host__ __device__ constexpr double td_coef(int c, int l, int m) {
+  // coefficient of D^m y_l in channel c: p_k (y_a - y_b) at m = k, p_k (y_c - y_e) at m = k + 1,
+  // p = 1, -1, -1, 1 at k = 0, 2, 4, 6
+  const int k = m & ~1;
+  const double p = (k == 0 || k == 6) ? 1.0 : -1.0;
+  const int* ch = TD_CHAN[c];
+  return (m % 2 == 0) ? p * ((ch[0] == l) - (ch[1] == l)) : p * ((ch[2] == l) - (ch[3] == l));
+}
+
+template <int MAXD, bool ORTH>
+__global__ __launch_bounds__(256) void k_tdi_td(const double* __restrict__ links, int n_ext, int n_out, TdDelays dl,
+                                                double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_out) return;
-  auto at = [&](int l, int m) {  // D^m y_l at output sample i
-    // x lies in [added - max_delay L / dt, n_ext - 1] with added - max_delay L / dt > 0 up to
-    // rounding (added is floor(max_delay L / dt) + 1); the index is clamped to the samples both
-    // ways, so the top end (m = 0, last sample) reads the last pair at fraction 1
-    const double x = (double)(i + added) - (double)m * L_dt;
-    const int j = max(0, min((int)floor(x), n_ext - 2));
-    const double fr = x - (double)j;
-    const double* y = links + (size_t)l * n_ext;
-    return y[j] + (y[j + 1] - y[j]) * fr;
-  };
-  const double P15[3] = {1.0, 0.0, -1.0}, P20[7] = {1.0, 0.0, -1.0, 0.0, -1.0, 0.0, 1.0};
-  const double* P = gen == 20 ? P20 : P15;
-  const int np = gen == 20 ? 7 : 3;
-  // links 0:12 1:21 2:23 3:32 4:31 5:13; channel c: (a, b, c2, e) with X = P (y_a - y_b + D (y_c2 - y_e))
-  const int comb[3][4] = {{5, 0, 4, 1}, {1, 2, 0, 3}, {3, 4, 2, 5}};
-  double ch[3];
-  for (int c = 0; c < 3; ++c) {
-    double v = 0.0;
-    for (int k = 0; k < np; ++k) {
-      if (P[k] == 0.0) continue;
-      v += P[k] * (at(comb[c][0], k) - at(comb[c][1], k) + at(comb[c][2], k + 1) - at(comb[c][3], k + 1));
+  double X = 0.0, Y = 0.0, Z = 0.0;  // links 0:12 1:21 2:23 3:32 4:31 5:13
+#pragma unroll
+  for (int l = 0; l < 6; ++l) {
+    const double* y = links + (size_t)l * n_ext + i;
+#pragma unroll
+    for (int m = 0; m <= MAXD; ++m) {
+      double v;
+      if (m == 0) {
+        v = y[dl.jb[0]];
+      } else {
+        const double a = y[dl.jb[m]], b = y[dl.jb[m] + 1];
+        v = a + (b - a) * dl.fr[m];
+      }
+      if (td_coef(0, l, m) != 0.0) X += td_coef(0, l, m) * v;
+      if (td_coef(1, l, m) != 0.0) Y += td_coef(1, l, m) * v;
+      if (td_coef(2, l, m) != 0.0) Z += td_coef(2, l, m) * v;
     }
-    ch[c] = v;
   }
-  double o[3];
-  if (orthogonal) {
-    o[0] = (ch[2] - ch[0]) / sqrt(2.0);
-    o[1] = (ch[0] - 2.0 * ch[1] + ch[2]) / sqrt(6.0);
-    o[2] = (ch[0] + ch[1] + ch[2]) / sqrt(3.0);
+  const size_t N = (size_t)n_out;
+  if (ORTH) {
+    out[i] = (Z - X) / sqrt(2.0);
+    out[N + i] = (X - 2.0 * Y + Z) / sqrt(6.0);
+    out[2 * N + i] = (X + Y + Z) / sqrt(3.0);
   } else {
-    o[0] = ch[0];
-    o[1] = ch[1];
-    o[2] = ch[2];
+    out[i] = X;
+    out[N + i] = Y;
+    out[2 * N + i] = Z;
   }
-  for (int c = 0; c < 3; ++c) out[(size_t)c * n_out + i] = o[c];
 }
 
 // ---- standalone Whittle lnL: -2 df sum_c sum_k |d - h|^2 / S (likelihood_ti.py:58-67) --------
@@ -1268,8 +1294,8 @@ extern "C" int pes_td_links(int orbit_kind, double arm_length_m, double rotation
   const int rc = fill_orbit(d, orbit_kind, arm_length_m, rotation_initial, revolution_initial, omega_rotation,
                             lambda_ref, beta_ref);
   if (rc != PES_OK || n == 0) return rc;
-  const TdWave w{d_hp, d_hc, wave_t0, wave_dt, wave_n};
-  hipLaunchKernelGGL(k_links_td, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream, d, w,
+  const TdWave w{d_hp, d_hc, wave_t0, 1.0 / wave_dt, wave_n};
+  hipLaunchKernelGGL(k_links_td, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, d, w,
                      extrinsic_of(lam, beta, psi, 0.0), t0, dt, n, d_out);
   HIP_TRY(hipGetLastError());
   return PES_OK;
@@ -1289,8 +1315,25 @@ extern "C" int pes_td_tdi(double arm_length_m, double dt, int tdi_generation, in
   if (n_out < 0) return fail(PES_ERR_INVALID, "td tdi: fewer link samples than the delays need");
   if (n_out > 0 && (!d_links || !d_out)) return fail(PES_ERR_INVALID, "td tdi: arrays must be non-NULL");
   if (n_out == 0) return PES_OK;
-  hipLaunchKernelGGL(k_tdi_td, dim3((n_out + 127) / 128), dim3(128), 0, (hipStream_t)stream, d_links, n_ext, n_out,
-                     added, (arm_length_m / C_SI) / dt, tdi_generation, orthogonal ? 1 : 0, d_out);
+  // delays m L / dt split once: the index stays inside [0, n_ext - 1] for every output sample
+  // (added - m L / dt > 0 up to rounding, clamped; the pair read at m >= 1 ends at i + added)
+  const int maxd = tdi_generation == 20 ? 7 : 3;
+  const double r = (arm_length_m / C_SI) / dt;
+  TdDelays dl;
+  std::memset(&dl, 0, sizeof(dl));
+  dl.jb[0] = added;
+  for (int m = 1; m <= maxd; ++m) {
+    const double c = (double)added - m * r;
+    const int jb = std::min(std::max((int)std::floor(c), 0), added - 1);
+    dl.jb[m] = jb;
+    dl.fr[m] = c - (double)jb;
+  }
+  const dim3 grid((n_out + 255) / 256), block(256);
+  hipStream_t st = (hipStream_t)stream;
+  if (maxd == 3 && orthogonal) hipLaunchKernelGGL((k_tdi_td<3, true>), grid, block, 0, st, d_links, n_ext, n_out, dl, d_out);
+  else if (maxd == 3) hipLaunchKernelGGL((k_tdi_td<3, false>), grid, block, 0, st, d_links, n_ext, n_out, dl, d_out);
+  else if (orthogonal) hipLaunchKernelGGL((k_tdi_td<7, true>), grid, block, 0, st, d_links, n_ext, n_out, dl, d_out);
+  else hipLaunchKernelGGL((k_tdi_td<7, false>), grid, block, 0, st, d_links, n_ext, n_out, dl, d_out);
   HIP_TRY(hipGetLastError());
   return PES_OK;
 }

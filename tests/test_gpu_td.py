@@ -38,6 +38,13 @@ def _chirp(n, dt, f0=2e-3, fdot=2e-9, seed=0):
     return amp * np.cos(phase), amp * np.sin(phase)
 
 
+def _tdi_tol(links):
+    """Rounding of a channel: a sum of up to 32 delayed link values (each interpolated from two
+    samples) taken in another order than the oracle's, so 64 ulps of the largest link; the
+    channels themselves are smaller than the links by ~ 4 pi f L at low frequency."""
+    return 64 * np.finfo(float).eps * float(np.max(np.abs(np.asarray(links.cpu() if hasattr(links, "cpu") else links))))
+
+
 def _td_links_gpu(orb, hp, hc, wt0, wdt, sky, t0, dt, n):
     from pespace_amd.core import td_links
     geo = dict(zip(("omega_rotation", "lambda_ref", "beta_ref"), orb[4:]))
@@ -71,7 +78,8 @@ def test_td_tdi_matches_oracle(gen, orthogonal):
     got = td_tdi(orb[1], dt, gen, orthogonal, links).cpu().numpy()
     ref = otd.tdi(links.cpu().numpy(), orb[1], dt, gen, orthogonal)
     assert got.shape == ref.shape == (3, n - otd.added_samples(orb[1], dt, gen))
-    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * np.max(np.abs(ref)))
+    np.testing.assert_allclose(got, ref, rtol=0, atol=_tdi_tol(links))
+    assert _tdi_tol(links) < 1e-10 * np.max(np.abs(ref))
 
 
 def test_td_links_and_tdi_one_year():
@@ -96,7 +104,7 @@ def test_td_links_and_tdi_one_year():
     aet = td_tdi(orb[1], dt, "1.5", True, links).cpu().numpy()
     ref = otd.tdi(got, orb[1], dt, "1.5", True)
     assert aet.shape == (3, n)
-    np.testing.assert_allclose(aet, ref, rtol=0, atol=1e-13 * np.max(np.abs(ref)))
+    np.testing.assert_allclose(aet, ref, rtol=0, atol=_tdi_tol(got))
 
 
 def test_td_host_api_matches_oracle():
@@ -118,10 +126,11 @@ def test_td_host_api_matches_oracle():
     ref_links = otd.links(odet.ORBITS["LISA_analytic"], hp, hc, tc + wt0, dt, *SKY, t_first, dt, n_ext)
     scale = np.max(np.abs(hp))
     np.testing.assert_allclose(np.stack(det.single_link_response_array), ref_links, rtol=0, atol=1e-12 * scale)
-    ref_aet = otd.tdi(ref_links, 2.5e9, dt, "1.5", True)
+    gpu_links = np.stack(det.single_link_response_array)
+    ref_aet = otd.tdi(gpu_links, 2.5e9, dt, "1.5", True)
     resp = det.tdi_response_numpy
     for ci, c in enumerate("AET"):
-        np.testing.assert_allclose(getattr(resp, c), ref_aet[ci], rtol=0, atol=1e-12 * scale)
+        np.testing.assert_allclose(getattr(resp, c), ref_aet[ci], rtol=0, atol=_tdi_tol(gpu_links))
     # the pure functions return the same without storing
     links = det.response_model.get_response_jit({"t0": wt0, "data": torch.as_tensor(wf["data"]).cuda()}, *SKY, tc)
     chans = comb.get_combination_jit(links)
