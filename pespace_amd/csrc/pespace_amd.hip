@@ -532,6 +532,60 @@ __host__ __device__ constexpr double td_coef(int c, int l, int m) {
   return (m % 2 == 0) ? p * ((ch[0] == l) - (ch[1] == l)) : p * ((ch[2] == l) - (ch[3] == l));
 }
 
+template <bool ORTH>
+__device__ inline void td_store(double* __restrict__ out, size_t N, int i, double X, double Y, double Z) {
+  if (ORTH) {
+    out[i] = (Z - X) / sqrt(2.0);
+    out[N + i] = (X - 2.0 * Y + Z) / sqrt(6.0);
+    out[2 * N + i] = (X + Y + Z) / sqrt(3.0);
+  } else {
+    out[i] = X;
+    out[N + i] = Y;
+    out[2 * N + i] = Z;
+  }
+}
+
+// The same with the links staged in LDS: a block of TD_BLOCK outputs reads the window of link
+// samples [i0 + jb_min, i0 + TD_BLOCK - 1 + added] of each link once, coalesced, and takes every
+// delayed value from LDS (the global path reads each sample 2 MAXD + 1 times through L1).  Used
+// when the window fits (added - jb_min <= TD_WIN_EXTRA: delays up to ~256 samples).
+constexpr int TD_BLOCK = 256, TD_WIN_EXTRA = 256;
+template <int MAXD, bool ORTH>
+__global__ __launch_bounds__(TD_BLOCK) void k_tdi_td_lds(const double* __restrict__ links, int n_ext, int n_out,
+                                                         TdDelays dl, double* __restrict__ out) {
+  extern __shared__ double win[];  // [6][W]
+  const int jb_min = dl.jb[MAXD];
+  const int W = TD_BLOCK + dl.jb[0] - jb_min + 1;
+  const int i0 = blockIdx.x * TD_BLOCK, tid = threadIdx.x;
+  const int g0 = i0 + jb_min;  // first link sample of the window
+  for (int l = 0; l < 6; ++l) {
+    const double* y = links + (size_t)l * n_ext;
+    for (int w = tid; w < W; w += TD_BLOCK) win[l * W + w] = (g0 + w < n_ext) ? y[g0 + w] : 0.0;
+  }
+  __syncthreads();
+  const int i = i0 + tid;
+  if (i >= n_out) return;
+  double X = 0.0, Y = 0.0, Z = 0.0;
+#pragma unroll
+  for (int l = 0; l < 6; ++l) {
+    const double* y = win + l * W + tid - jb_min;  // y[j] = link sample i + j
+#pragma unroll
+    for (int m = 0; m <= MAXD; ++m) {
+      double v;
+      if (m == 0) {
+        v = y[dl.jb[0]];
+      } else {
+        const double a = y[dl.jb[m]], b = y[dl.jb[m] + 1];
+        v = a + (b - a) * dl.fr[m];
+      }
+      if (td_coef(0, l, m) != 0.0) X += td_coef(0, l, m) * v;
+      if (td_coef(1, l, m) != 0.0) Y += td_coef(1, l, m) * v;
+      if (td_coef(2, l, m) != 0.0) Z += td_coef(2, l, m) * v;
+    }
+  }
+  td_store<ORTH>(out, (size_t)n_out, i, X, Y, Z);
+}
+
 template <int MAXD, bool ORTH>
 __global__ __launch_bounds__(256) void k_tdi_td(const double* __restrict__ links, int n_ext, int n_out, TdDelays dl,
                                                 double* __restrict__ out) {
@@ -555,16 +609,7 @@ __global__ __launch_bounds__(256) void k_tdi_td(const double* __restrict__ links
       if (td_coef(2, l, m) != 0.0) Z += td_coef(2, l, m) * v;
     }
   }
-  const size_t N = (size_t)n_out;
-  if (ORTH) {
-    out[i] = (Z - X) / sqrt(2.0);
-    out[N + i] = (X - 2.0 * Y + Z) / sqrt(6.0);
-    out[2 * N + i] = (X + Y + Z) / sqrt(3.0);
-  } else {
-    out[i] = X;
-    out[N + i] = Y;
-    out[2 * N + i] = Z;
-  }
+  td_store<ORTH>(out, (size_t)n_out, i, X, Y, Z);
 }
 
 // ---- standalone Whittle lnL: -2 df sum_c sum_k |d - h|^2 / S (likelihood_ti.py:58-67) --------
@@ -1328,12 +1373,24 @@ extern "C" int pes_td_tdi(double arm_length_m, double dt, int tdi_generation, in
     dl.jb[m] = jb;
     dl.fr[m] = c - (double)jb;
   }
-  const dim3 grid((n_out + 255) / 256), block(256);
   hipStream_t st = (hipStream_t)stream;
-  if (maxd == 3 && orthogonal) hipLaunchKernelGGL((k_tdi_td<3, true>), grid, block, 0, st, d_links, n_ext, n_out, dl, d_out);
-  else if (maxd == 3) hipLaunchKernelGGL((k_tdi_td<3, false>), grid, block, 0, st, d_links, n_ext, n_out, dl, d_out);
-  else if (orthogonal) hipLaunchKernelGGL((k_tdi_td<7, true>), grid, block, 0, st, d_links, n_ext, n_out, dl, d_out);
-  else hipLaunchKernelGGL((k_tdi_td<7, false>), grid, block, 0, st, d_links, n_ext, n_out, dl, d_out);
+  const int span = added - dl.jb[maxd];  // window beyond the block: jb[0] - jb_min
+  if (span <= TD_WIN_EXTRA) {
+    // LDS window of each link: TD_BLOCK + span + 1 samples (<= 24.6 KB for the six links);
+    // every read index tid - jb_min + jb[m] (+ 1) lies in [0, TD_BLOCK - 1 + span]
+    const dim3 grid((n_out + TD_BLOCK - 1) / TD_BLOCK), block(TD_BLOCK);
+    const size_t lds = 6 * (size_t)(TD_BLOCK + span + 1) * sizeof(double);
+    if (maxd == 3 && orthogonal) hipLaunchKernelGGL((k_tdi_td_lds<3, true>), grid, block, lds, st, d_links, n_ext, n_out, dl, d_out);
+    else if (maxd == 3) hipLaunchKernelGGL((k_tdi_td_lds<3, false>), grid, block, lds, st, d_links, n_ext, n_out, dl, d_out);
+    else if (orthogonal) hipLaunchKernelGGL((k_tdi_td_lds<7, true>), grid, block, lds, st, d_links, n_ext, n_out, dl, d_out);
+    else hipLaunchKernelGGL((k_tdi_td_lds<7, false>), grid, block, lds, st, d_links, n_ext, n_out, dl, d_out);
+  } else {
+    const dim3 grid((n_out + 255) / 256), block(256);
+    if (maxd == 3 && orthogonal) hipLaunchKernelGGL((k_tdi_td<3, true>), grid, block, 0, st, d_links, n_ext, n_out, dl, d_out);
+    else if (maxd == 3) hipLaunchKernelGGL((k_tdi_td<3, false>), grid, block, 0, st, d_links, n_ext, n_out, dl, d_out);
+    else if (orthogonal) hipLaunchKernelGGL((k_tdi_td<7, true>), grid, block, 0, st, d_links, n_ext, n_out, dl, d_out);
+    else hipLaunchKernelGGL((k_tdi_td<7, false>), grid, block, 0, st, d_links, n_ext, n_out, dl, d_out);
+  }
   HIP_TRY(hipGetLastError());
   return PES_OK;
 }

@@ -127,7 +127,7 @@ __device__ inline void constellation(const DetectorDev& d, double t, Vec3 n[3], 
       xd[i].z = d.rp * (-d.cbet * sk[i]);
     }
   }
-  double invL = 1.0 / d.L;
+  const double invL = d.invL;  // 1 / L, rounded on the host exactly as a device division would
   n[0] = {(xd[2].x - xd[1].x) * invL, (xd[2].y - xd[1].y) * invL, (xd[2].z - xd[1].z) * invL};
   n[1] = {(xd[0].x - xd[2].x) * invL, (xd[0].y - xd[2].y) * invL, (xd[0].z - xd[2].z) * invL};
   n[2] = {(xd[1].x - xd[0].x) * invL, (xd[1].y - xd[0].y) * invL, (xd[1].z - xd[0].z) * invL};

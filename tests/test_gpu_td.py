@@ -93,6 +93,26 @@ def test_td_tdi_matches_oracle(gen, orthogonal):
     assert _tdi_tol(links) < 1e-10 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("gen", ["1.5", "2.0"])
+@pytest.mark.parametrize("orthogonal", [True, False])
+@pytest.mark.parametrize("dt", [5.0, 0.1, 0.02])
+def test_td_tdi_both_kernels_random_links(gen, orthogonal, dt):
+    """Delays of a few samples (dt = 5 s: the LDS-window kernel), of up to ~250 samples (0.1 s,
+    TDI 1.5: the widest LDS window; 2.0: the global-memory kernel) and of thousands (0.02 s: the
+    global-memory kernel), on random link samples, including the last output sample."""
+    import torch
+    from pespace_amd.core import td_tdi
+    arm = 2.5e9
+    added = otd.added_samples(arm, dt, gen)
+    n_ext = added + 3001
+    links = np.random.default_rng(17).standard_normal((6, n_ext))
+    print(f"td tdi random gen={gen} orth={orthogonal} dt={dt} added={added}", flush=True)
+    got = td_tdi(arm, dt, gen, orthogonal, torch.as_tensor(links).cuda()).cpu().numpy()
+    ref = otd.tdi(links, arm, dt, gen, orthogonal)
+    assert got.shape == (3, 3001)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=_tdi_tol(links))
+
+
 def test_td_links_and_tdi_one_year():
     from pespace_amd.core import td_tdi
     orb = ORBITS[0]
