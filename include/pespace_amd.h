@@ -84,6 +84,16 @@ int pes_context_add_detector(pes_context* ctx, int orbit_kind, double arm_length
                              double df, const double* data_re, const double* data_im, const double* psd,
                              const int* channel_mask);
 
+/* Observation duty cycle of detector det_index (in order of addition; before finalize): the
+ * detector observes for on_duration seconds of every period, the windows starting at
+ * start_time + n period, with C1 smoothstep ramps of taper seconds inside each window (taper 0:
+ * a hard window; period 0: always observing, the default).  Each mode's response at a bin is
+ * weighted by the window at its time t = t_f + t_c (the stationary-phase time of that
+ * frequency), in the likelihood, its gradient and every response output.  The reference models
+ * no duty cycle; this serves the benchmark network's TianQin (3 months on, 3 months off). */
+int pes_context_set_duty_cycle(pes_context* ctx, int det_index, double period, double on_duration, double start_time,
+                               double taper);
+
 /* Upload detector data; must be called once after the last add_detector.  With
  * interpolation nodes (n_nodes > 1) every bin of every detector must lie inside
  * [f_node_min, f_node_max] (no extrapolation): otherwise PES_ERR_INVALID. */

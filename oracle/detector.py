@@ -198,12 +198,28 @@ def _sinc(x):
     return out
 
 
-def single_links(waveform, lam, beta, psi, tc, f, orbit):
+def duty_window(t, duty):
+    """Observation window of a duty-cycled detector (not in the reference; the benchmark
+    network's TianQin observes 3 months on, 3 months off).  duty = (period, on_duration,
+    start_time, taper): s = (t - start_time) mod period; inside [0, on_duration) the window
+    rises over the first taper seconds and falls over the last taper seconds by the C1
+    smoothstep S(x) = x^2 (3 - 2x) of the distance into the window over taper; 0 outside."""
+    period, on, t0, taper = (float(x) for x in duty)
+    t = np.asarray(t, dtype=np.float64)
+    u = (t - t0) / period
+    s = (u - np.floor(u)) * period
+    x = np.minimum(s, on - s) / taper if taper > 0 else np.ones_like(s)
+    x = np.clip(x, 0.0, 1.0)
+    return np.where(s < on, x * x * (3.0 - 2.0 * x), 0.0)
+
+
+def single_links(waveform, lam, beta, psi, tc, f, orbit, duty=None):
     """Marsat-Baker FD single-link response, summed over modes.
 
     waveform: {"plus","cross","tf"} or a mode-keyed dict of those (antenna.py:438-458)
     orbit:    (kind, arm_length[m], rot0, rev0) or, geocentric, (kind, arm_length, rot0, rev0,
               omega_rotation, lambda_ref, beta_ref)
+    duty:     None, or a duty cycle (duty_window): each mode weighted by the window at tf + tc
     Returns dict link -> complex[N].  antenna.py:429-460 (driver), 496-543 (links).
     """
     kind, arm, rot0, rev0 = orbit[:4]
@@ -220,6 +236,9 @@ def single_links(waveform, lam, beta, psi, tc, f, orbit):
         hp = wf["plus"] * cexp_tshift  # antenna.py:447-449
         hc = wf["cross"] * cexp_tshift
         tf = wf["tf"] + tc
+        if duty is not None:
+            w = duty_window(tf, duty)
+            hp, hc = hp * w, hc * w
         n1, n2, n3, x1, x2, x3 = constellation(kind, arm, rot0, rev0, tf, *extra)  # :496
 
         def nhn(n):  # antenna.py:499-512

@@ -29,7 +29,7 @@ class Detector:
     """Plain container for one detector's grid, data and PSD (oracle side)."""
 
     def __init__(self, f, data, psd, df, orbit, tdi_generation="1.5", channels=("A", "E", "T"),
-                 orthogonal=True):
+                 orthogonal=True, duty=None):
         self.f = np.asarray(f, dtype=np.float64)
         self.data = data
         self.psd = psd
@@ -38,6 +38,7 @@ class Detector:
         self.tdi_generation = tdi_generation
         self.channels = tuple(channels)
         self.orthogonal = orthogonal
+        self.duty = duty  # None or (period, on_duration, start_time, taper)
 
 
 def waveform(params, f, modes, f_ref, interp=None):
@@ -54,7 +55,8 @@ def response(params, d, modes=((2, 2),), f_ref=0.0, interp=None, wf=None):
     if wf is None:
         wf = waveform(params, d.f, modes, f_ref, interp)
     links = det.single_links(wf, params["ecliptic_longitude"], params["ecliptic_latitude"],
-                             params["polarization"], params["coalescence_time"], d.f, d.orbit)
+                             params["polarization"], params["coalescence_time"], d.f, d.orbit,
+                             getattr(d, "duty", None))
     return det.tdi_combination(links, d.f, d.orbit[1], d.tdi_generation, d.orthogonal)
 
 

@@ -53,12 +53,19 @@ class LikelihoodCore:
     # -- detectors ---------------------------------------------------------------------
     def add_detector(self, orbit_kind, arm_length, rotation_initial, revolution_initial, tdi_generation, orthogonal,
                      frequencies, data, psd, channel_mask, df=None, weight=None, omega_rotation=GEO_OMEGA_ROTATION,
-                     lambda_ref=GEO_LAMBDA_REF, beta_ref=GEO_BETA_REF):
+                     lambda_ref=GEO_LAMBDA_REF, beta_ref=GEO_BETA_REF, duty_cycle=None):
         """data: complex (3, N); psd: (3, N) (unused channels may hold ones).  omega_rotation,
         lambda_ref, beta_ref: rotation rate and reference source of a geocentric orbit
-        (orbit.py:77-85; TianQin's by default), unused for heliocentric ones."""
+        (orbit.py:77-85; TianQin's by default), unused for heliocentric ones.  duty_cycle:
+        None (always observing) or (period, on_duration, start_time, taper) in seconds: each
+        mode's response is weighted by the observation window at its time t_f + t_c (no
+        counterpart in the reference)."""
         if self._ctx is not None:
             raise PespaceError("detectors must be added before the first evaluation")
+        if duty_cycle is not None:
+            duty_cycle = tuple(float(x) for x in duty_cycle)
+            if len(duty_cycle) != 4:
+                raise ValueError("duty_cycle is (period, on_duration, start_time, taper)")
         f = np.ascontiguousarray(frequencies, dtype=np.float64)
         data = np.asarray(data, dtype=np.complex128)
         psd = np.ascontiguousarray(psd, dtype=np.float64)
@@ -70,7 +77,8 @@ class LikelihoodCore:
                                f=f, dre=np.ascontiguousarray(data.real), dim=np.ascontiguousarray(data.imag),
                                psd=psd, mask=np.asarray(channel_mask, dtype=np.int32),
                                df=0.0 if df is None else float(df),
-                               weight=None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)))
+                               weight=None if weight is None else np.ascontiguousarray(weight, dtype=np.float64),
+                               duty=duty_cycle))
         return len(self._dets) - 1
 
     def _build(self):
@@ -88,7 +96,7 @@ class LikelihoodCore:
         check(L.pes_context_create(ctypes.byref(h), self.device, len(self.modes), ids, self.reference_frequency,
                                    self.n_nodes, f_lo, f_hi))
         try:
-            for d in self._dets:
+            for i, d in enumerate(self._dets):
                 mask = np.ascontiguousarray(d["mask"], dtype=np.int32)
                 w = d["weight"]
                 check(L.pes_context_add_detector(
@@ -96,6 +104,8 @@ class LikelihoodCore:
                     d["orth"], d["f"].size, dptr(d["f"]),
                     dptr(w) if w is not None else None, d["df"], dptr(d["dre"]), dptr(d["dim"]), dptr(d["psd"]),
                     mask.ctypes.data_as(c_int_p)))
+                if d["duty"] is not None:
+                    check(L.pes_context_set_duty_cycle(h, i, *d["duty"]))
             check(L.pes_context_finalize(h))
         except Exception:
             L.pes_context_destroy(h)

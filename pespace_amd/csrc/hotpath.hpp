@@ -155,8 +155,15 @@ __device__ inline RowC load_row(const double* __restrict__ pc, const BinPre& b) 
 // link vector n_a = (xd_i - xd_j) / L is
 //   n_a.v = (sqrt3 / L) (al_a v.V - be_a v.U),      k.xd_a = al_a k.U + be_a k.V:
 // six scalar products per mode instead of nine, and two terms per projection.
+// KIND: bit 0 the orbit (0 heliocentric, 1 geocentric), bit 1 a duty-cycled detector.
+template <int KIND>
+constexpr bool kGeo = (KIND & 1) != 0;
+template <int KIND>
+constexpr bool kDuty = (KIND & 2) != 0;
+
 struct ModeG {
   double A, t;      // amplitude, t_f + t_c
+  double w, dw;     // duty window at t and dw/dt (duty-cycled detectors)
   Cx u, hm, hp, hc; // unit phasor, A u, F+ A u, 2 Fx A u
   Cx z0, z1, z01;   // pair phasors exp(i pi f k.xd_a) of nodes 0, 1, and their product
   double sa, ca;    // revolution angle
@@ -177,11 +184,12 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
   const double dP = P1 - P0;
   const double Psi = fma(b.hb1, dP, fma(b.hb2, dP0, fma(b.hb3, dP1, P0)));
   g.t = fma(b.he1, dP, fma(b.he2, dP0, fma(b.he3, dP1, rc.tc)));
+  if constexpr (kDuty<KIND>) duty_window(det, g.t, g.w, g.dw);
   sincos_tab_s(fma(kOmegaYear, g.t, det.rev0), &g.sa, &g.ca);
   const double sa = g.sa, ca = g.ca;
   g.kx0 = AU_SEC * fma(rc.k.x, ca, rc.k.y * sa);
   double kx_0, kx_1;
-  if constexpr (KIND == 0) {
+  if constexpr (!kGeo<KIND>) {
     const double sc = sa * ca;
     const V3 U = {sc, -fma(ca, ca, 1.0), -SQRT3 * sa};
     const V3 V = {-fma(sa, sa, 1.0), sc, -SQRT3 * ca};
@@ -247,7 +255,7 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
     sincos_tab_sum(rc.tc_r, rc.tc_k, fma(2.0 * b.pif, g.kx0, Psi), &sp, &cp);
   }
   g.u = {cp, -sp};
-  g.hm = scale(g.u, g.A);
+  g.hm = scale(g.u, kDuty<KIND> ? g.A * g.w : g.A);
   g.hp = mul(Cx{mf[MF_FPR], mf[MF_FPI]}, g.hm);
   g.hc = mul(Cx{mf[MF_FCR], mf[MF_FCI]}, g.hm);
 }
@@ -441,7 +449,7 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
   // angle
   double tgeo;
   double pUb = 0.0, pVb = 0.0, qUb = 0.0, qVb = 0.0, kUb = 0.0, kVb = 0.0;
-  if constexpr (KIND == 0) {
+  if constexpr (!kGeo<KIND>) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       pUb = fma(-npb[a], det.hca[a], pUb);
@@ -498,7 +506,13 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
   acc.kb.x = fma(kx0b * AU_SEC, g.ca, acc.kb.x);
   acc.kb.y = fma(kx0b * AU_SEC, g.sa, acc.kb.y);
   const double kx0_t = AU_SEC * kOmegaYear * fma(-rc.k.x, g.sa, rc.k.y * g.ca);
-  const double tb = fma(kx0b, kx0_t, tgeo);  // adjoint of t = t_f + t_c
+  const Cx Gu = mul(Gm, g.u);
+  double Ab = Gu.re;  // adjoint of the amplitude the response sees (A w with a duty window)
+  double tb = fma(kx0b, kx0_t, tgeo);  // adjoint of t = t_f + t_c
+  if constexpr (kDuty<KIND>) {  // h ~ A w(t): the window's slope moves t, A sees w
+    tb = fma(Ab * g.A, g.dw, tb);
+    Ab *= g.w;
+  }
   acc.gh_re = fma(Gm.re, g.hm.re, fma(-Gm.im, g.hm.im, acc.gh_re));  // += Re(Gm hm)
   acc.mphi = fma((double)kModeM[mid], phib, acc.mphi);
   {
@@ -508,8 +522,6 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
   }
   acc.tc = fma(2.0 * b.pif, phib, acc.tc + tb);
   // intrinsic parameters through the node tables: weights of the eight node entries
-  const Cx Gu = mul(Gm, g.u);
-  const double Ab = Gu.re;
   const double cA1 = Ab * b.hb1, cA0 = Ab - cA1, cdA0 = Ab * b.hb2, cdA1 = Ab * b.hb3;
   const double tP = tb * b.he1;
   const double cP1 = fma(phib, b.hb1, tP), cP0 = phib - cP1;
@@ -820,7 +832,9 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
 // Launch bounds: the gradient kernel runs two waves per SIMD (256 registers of the unified
 // file per lane), the forward-only kernel three (168 registers, some spilled; measured 9%
 // faster than two for lnL-only batches when three blocks per CU are launched).
-template <int PREC, bool GRAD>
+// DUTY: the network has a duty-cycled detector (instantiates the duty-window tile variants;
+// without, the kernel holds only the two orbit kinds' code).
+template <int PREC, bool GRAD, bool DUTY>
 __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* __restrict__ dets, int n_det, int n_tiles,
                                               const double* __restrict__ bin_f, const int* __restrict__ bin_j,
                                               const double* __restrict__ bin_s, const double* __restrict__ bin_rec,
@@ -875,16 +889,35 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
 #define PES_TILE_ARGS                                                                                         \
   det, tile, f0, p0, p1, bin_f, bin_j, bin_s, bin_rec, tile_dd, pconst, nodes, nodes_grad, NNS, du, \
       modes, lane, wave, &acc[0][0][0], j0, nj, stage_n, &stage_g[0], buf
-      if (nj <= NJ_STAGE) {
-        if (det.kind == 0)
-          tile_rows<PREC, GRAD, 0, true>(PES_TILE_ARGS);
-        else
-          tile_rows<PREC, GRAD, 1, true>(PES_TILE_ARGS);
+      if constexpr (!DUTY) {
+        if (nj <= NJ_STAGE) {
+          if (det.kind == 0)
+            tile_rows<PREC, GRAD, 0, true>(PES_TILE_ARGS);
+          else
+            tile_rows<PREC, GRAD, 1, true>(PES_TILE_ARGS);
+        } else {
+          if (det.kind == 0)
+            tile_rows<PREC, GRAD, 0, false>(PES_TILE_ARGS);
+          else
+            tile_rows<PREC, GRAD, 1, false>(PES_TILE_ARGS);
+        }
       } else {
-        if (det.kind == 0)
-          tile_rows<PREC, GRAD, 0, false>(PES_TILE_ARGS);
-        else
-          tile_rows<PREC, GRAD, 1, false>(PES_TILE_ARGS);
+        const int kc = det.kind | (det.duty ? 2 : 0);
+        if (nj <= NJ_STAGE) {
+          switch (kc) {
+            case 0: tile_rows<PREC, GRAD, 0, true>(PES_TILE_ARGS); break;
+            case 1: tile_rows<PREC, GRAD, 1, true>(PES_TILE_ARGS); break;
+            case 2: tile_rows<PREC, GRAD, 2, true>(PES_TILE_ARGS); break;
+            default: tile_rows<PREC, GRAD, 3, true>(PES_TILE_ARGS); break;
+          }
+        } else {
+          switch (kc) {
+            case 0: tile_rows<PREC, GRAD, 0, false>(PES_TILE_ARGS); break;
+            case 1: tile_rows<PREC, GRAD, 1, false>(PES_TILE_ARGS); break;
+            case 2: tile_rows<PREC, GRAD, 2, false>(PES_TILE_ARGS); break;
+            default: tile_rows<PREC, GRAD, 3, false>(PES_TILE_ARGS); break;
+          }
+        }
       }
 #undef PES_TILE_ARGS
     }
@@ -917,12 +950,13 @@ __global__ __launch_bounds__(128) void k_response_t(const DetectorDev* __restric
   TdiC tdc;
   tdi_consts(det, b, tdc);
   Cx o[3];
-  if (det.kind == 0)
-    response_row<PREC, 0>(det, pc, modes, nodes + ((size_t)p * modes.n * NNS + b.j) * 4, (size_t)NNS * 4, b, rc,
-                          tdc, o);
-  else
-    response_row<PREC, 1>(det, pc, modes, nodes + ((size_t)p * modes.n * NNS + b.j) * 4, (size_t)NNS * 4, b, rc,
-                          tdc, o);
+  const double* nb = nodes + ((size_t)p * modes.n * NNS + b.j) * 4;
+  switch (det.kind | (det.duty ? 2 : 0)) {
+    case 0: response_row<PREC, 0>(det, pc, modes, nb, (size_t)NNS * 4, b, rc, tdc, o); break;
+    case 1: response_row<PREC, 1>(det, pc, modes, nb, (size_t)NNS * 4, b, rc, tdc, o); break;
+    case 2: response_row<PREC, 2>(det, pc, modes, nb, (size_t)NNS * 4, b, rc, tdc, o); break;
+    default: response_row<PREC, 3>(det, pc, modes, nb, (size_t)NNS * 4, b, rc, tdc, o); break;
+  }
   for (int ch = 0; ch < 3; ++ch) {
     const size_t idx = (((size_t)p * 3 + ch) * det.n_bins + k) * 2;
     out[idx] = o[ch].re;
@@ -951,10 +985,12 @@ __global__ __launch_bounds__(128) void k_links(const DetectorDev* __restrict__ d
   const RowC rc = load_row(pc, b);
   LinksT L;
   const double* nb = nodes + ((size_t)p * modes.n * NNS + b.j) * 4;
-  if (det.kind == 0)
-    links_row<0, 0>(det, pc, modes, nb, (size_t)NNS * 4, b, rc, L);
-  else
-    links_row<0, 1>(det, pc, modes, nb, (size_t)NNS * 4, b, rc, L);
+  switch (det.kind | (det.duty ? 2 : 0)) {
+    case 0: links_row<0, 0>(det, pc, modes, nb, (size_t)NNS * 4, b, rc, L); break;
+    case 1: links_row<0, 1>(det, pc, modes, nb, (size_t)NNS * 4, b, rc, L); break;
+    case 2: links_row<0, 2>(det, pc, modes, nb, (size_t)NNS * 4, b, rc, L); break;
+    default: links_row<0, 3>(det, pc, modes, nb, (size_t)NNS * 4, b, rc, L); break;
+  }
   const Cx cf = {-b.xL * b.sxL, -b.xL * b.cxL};  // (-i xL) exp(-i xL)
   for (int i = 0; i < 6; ++i) {
     const Cx l = mul(cf, L.l[i]);

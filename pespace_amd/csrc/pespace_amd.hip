@@ -661,14 +661,17 @@ __global__ void k_whittle_final(const double* __restrict__ work, int nb, double 
 // scheduled with their own compiler options (three waves per SIMD against two for the
 // gradient kernel).  This launcher is that unit's only export.
 namespace pes {
-hipError_t launch_lnl_fwd(int precision, int grid, hipStream_t st, PES_LNL_PARAMS);
+hipError_t launch_lnl_fwd(int precision, bool duty, int grid, hipStream_t st, PES_LNL_PARAMS);
 }
 
 #ifdef PES_LNL_FWD_UNIT
-hipError_t pes::launch_lnl_fwd(int precision, int grid, hipStream_t st, PES_LNL_PARAMS) {
+hipError_t pes::launch_lnl_fwd(int precision, bool duty, int grid, hipStream_t st, PES_LNL_PARAMS) {
   // lnL alone is the FP64 forward in every precision mode (the modes differ in the adjoint only)
   (void)precision;
-  hipLaunchKernelGGL((k_lnl<0, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES);
+  if (duty)
+    hipLaunchKernelGGL((k_lnl<0, false, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES);
+  else
+    hipLaunchKernelGGL((k_lnl<0, false, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES);
   return hipGetLastError();
 }
 #else
@@ -681,6 +684,7 @@ struct pes_context {
   int n_nodes = 0;
   double f_node_min = 0.0, f_node_max = 0.0, u0 = 0.0, du = 1.0;
   bool finalized = false;
+  bool any_duty = false;  // a detector has a duty cycle (selects the duty-window kernel variants)
   struct HostDet {
     DetectorDev d;
     std::vector<double> f, w, dre, dim;  // w, dre, dim: [3][n]
@@ -813,6 +817,30 @@ extern "C" int pes_context_add_detector(pes_context* ctx, int orbit_kind, double
   return PES_OK;
 }
 
+extern "C" int pes_context_set_duty_cycle(pes_context* ctx, int det_index, double period, double on_duration,
+                                          double start_time, double taper) {
+  if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
+  if (ctx->finalized) return fail(PES_ERR_STATE, "context already finalized");
+  if (det_index < 0 || det_index >= (int)ctx->hdets.size()) return fail(PES_ERR_INVALID, "no such detector");
+  DetectorDev& d = ctx->hdets[det_index].d;
+  if (period == 0.0) {  // always observing
+    d.duty = 0;
+    return PES_OK;
+  }
+  if (!(period > 0.0) || !(on_duration > 0.0) || !(on_duration <= period) || !std::isfinite(start_time) ||
+      !(taper >= 0.0) || !(2.0 * taper <= on_duration) || !std::isfinite(period))
+    return fail(PES_ERR_INVALID,
+                "duty cycle: need 0 < on_duration <= period, a finite start time and 0 <= taper <= on_duration / 2");
+  d.duty = 1;
+  d.duty_period = period;
+  d.duty_on = on_duration;
+  d.duty_t0 = start_time;
+  d.duty_taper = taper;
+  d.duty_inv_period = 1.0 / period;
+  d.duty_inv_taper = taper > 0.0 ? 1.0 / taper : 0.0;
+  return PES_OK;
+}
+
 static int n_groups_of(const pes_context* ctx) {
   return (ctx->n_tiles + ctx->tiles_per_group - 1) / ctx->tiles_per_group;
 }
@@ -928,7 +956,11 @@ extern "C" int pes_context_finalize(pes_context* ctx) {
   }
   size_t nd = ctx->hdets.size();
   std::vector<DetectorDev> dd(nd);
-  for (size_t i = 0; i < nd; ++i) dd[i] = ctx->hdets[i].d;
+  ctx->any_duty = false;
+  for (size_t i = 0; i < nd; ++i) {
+    dd[i] = ctx->hdets[i].d;
+    ctx->any_duty = ctx->any_duty || dd[i].duty != 0;
+  }
   HIP_TRY(hipMalloc(&ctx->d_dets, nd * sizeof(DetectorDev)));
   HIP_TRY(hipMemcpy(ctx->d_dets, dd.data(), nd * sizeof(DetectorDev), hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&ctx->d_bin_f, NB * sizeof(double)));
@@ -1127,12 +1159,17 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
   ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_rec,                 \
       ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, sc.pconst, sc.nodes, sc.nodes_grad, NNS, ctx->du, \
       ctx->modes, P, rows, ctx->d_tile_jr, sc.work_counter, sc.partial
+  const bool duty = ctx->any_duty;
   if (!grad)
-    HIP_TRY(launch_lnl_fwd(precision, grid, st, PES_LNL_ARGS));
+    HIP_TRY(launch_lnl_fwd(precision, duty, grid, st, PES_LNL_ARGS));
+  else if (precision == 0 && !duty)
+    hipLaunchKernelGGL((k_lnl<0, true, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
   else if (precision == 0)
-    hipLaunchKernelGGL((k_lnl<0, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+    hipLaunchKernelGGL((k_lnl<0, true, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+  else if (!duty)
+    hipLaunchKernelGGL((k_lnl<1, true, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
   else
-    hipLaunchKernelGGL((k_lnl<1, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+    hipLaunchKernelGGL((k_lnl<1, true, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
 #undef PES_LNL_ARGS
   HIP_TRY(hipGetLastError());
   if (profile) {

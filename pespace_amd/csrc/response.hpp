@@ -39,7 +39,33 @@ struct DetectorDev {
   double c3L;
   double hsa[3], hca[3];
   double sgxs, sgxc, sgys, sgyc, sgzs;
+  // observation duty cycle (duty = 0: always observing).  The detector observes for duty_on
+  // seconds of every duty_period, the windows starting at duty_t0 + n duty_period, with smooth
+  // ramps of duty_taper seconds inside each window; the response of a mode at a bin is weighted
+  // by the window at its time t = t_f + t_c
+  int duty;
+  double duty_period, duty_on, duty_t0, duty_taper, duty_inv_period, duty_inv_taper;
 };
+
+// The duty window w(t) and dw/dt: s = (t - t0) mod period; w = S(s / taper) on the rising
+// ramp, 1 on the plateau, S((on - s) / taper) on the falling ramp, 0 while off, with the C1
+// smoothstep S(x) = x^2 (3 - 2 x) (taper 0: a hard window, dw/dt = 0)
+PES_HD void duty_window(const DetectorDev& d, double t, double& w, double& dw) {
+  const double u = (t - d.duty_t0) * d.duty_inv_period;
+  const double s = (u - floor(u)) * d.duty_period;
+  w = 0.0;
+  dw = 0.0;
+  if (s >= d.duty_on) return;
+  const double up = s, dn = d.duty_on - s;  // distances into the window from its two ends
+  const double x = d.duty_inv_taper > 0.0 ? fmin(up, dn) * d.duty_inv_taper : 1.0;  // 0: no ramps
+  if (x >= 1.0) {
+    w = 1.0;
+    return;
+  }
+  w = x * x * (3.0 - 2.0 * x);
+  const double g = 6.0 * x * (1.0 - x) * d.duty_inv_taper;
+  dw = up <= dn ? g : -g;
+}
 
 struct ModeList {  // the modes of a context (ids into kModeM etc.), passed to kernels by value
   int n;
@@ -155,6 +181,12 @@ struct Links {
 // hp, hc already include the t_c shift; t = tf + t_c.
 __device__ inline void add_mode_links(const DetectorDev& d, double f, double t, Cplx hp, Cplx hc, const Vec3& p,
                                       const Vec3& q, const Vec3& k, Links& L) {
+  if (d.duty) {  // the detector's duty window at the mode's time t = t_f + t_c
+    double w, dw;
+    duty_window(d, t, w, dw);
+    hp = {hp.re * w, hp.im * w};
+    hc = {hc.re * w, hc.im * w};
+  }
   Vec3 n[3], x[3];
   constellation(d, t, n, x);
   Cplx nhn[3];

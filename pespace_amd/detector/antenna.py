@@ -29,6 +29,12 @@ from .orbit import OrbitModelBase, available_orbit_models
 BACKENDS = ("hip",)
 LINK_NAMES = ("link12", "link21", "link23", "link32", "link31", "link13")  # antenna.py:24-31
 
+# An observation duty cycle (no counterpart in the reference): the detector observes for
+# on_duration seconds of every period, from start_time + n period, with smoothstep ramps of
+# taper seconds inside each window; each mode's response at a frequency is weighted by the
+# window at that frequency's time t_f + t_c.
+DutyCycle = namedtuple("DutyCycle", ["period", "on_duration", "start_time", "taper"], defaults=(0.0, 86400.0))
+
 
 class FDResponseModelMarset2018:
     """Marsat et al. 2018 (arXiv:1806.10734) FD single-link response, fused with TDI on the device."""
@@ -118,9 +124,12 @@ class TDResponseModelCornish2003:
 class InterferometerAntenna:
     """antenna.py:117-323 with backend='hip'."""
 
-    def __init__(self, name, tdi_data, orbit_model, response_model, tdi_combination, backend="hip"):
+    def __init__(self, name, tdi_data, orbit_model, response_model, tdi_combination, backend="hip", duty_cycle=None):
         if backend not in BACKENDS:
             raise ValueError(f"Unknown backend: {backend}")
+        if duty_cycle is not None and not isinstance(duty_cycle, DutyCycle):
+            duty_cycle = DutyCycle(*duty_cycle)
+        self.duty_cycle = duty_cycle
         if isinstance(orbit_model, str):
             models = available_orbit_models(backend)
             if orbit_model not in models:
@@ -226,4 +235,5 @@ def add_detector_to_core(core, det, data_channels):
     return core.add_detector(orb.kind, orb.arm_length, orb.rotation_initial, orb.revolution_initial,
                              det.tdi_combination.generation_code, det.tdi_combination.orthogonal, f, data, psd, mask,
                              df=td.data_info.delta_frequency if uniform else 0.0,
-                             weight=None if uniform else weight, **getattr(orb, "geo_params", {}))
+                             weight=None if uniform else weight, duty_cycle=getattr(det, "duty_cycle", None),
+                             **getattr(orb, "geo_params", {}))

@@ -46,7 +46,7 @@ def _noiseless_core(core, row):
         data = np.where(np.asarray(d["mask"])[:, None] != 0, resp, 0.0)
         nc.add_detector(d["kind"], d["arm"], d["rot0"], d["rev0"], d["gen"], d["orth"], d["f"], data, d["psd"],
                         d["mask"], df=d["df"], weight=d["weight"], omega_rotation=d["omega"],
-                        lambda_ref=d["lam"], beta_ref=d["bet"])
+                        lambda_ref=d["lam"], beta_ref=d["bet"], duty_cycle=d["duty"])
     return nc
 
 

@@ -56,7 +56,7 @@ def prior_rows(n, seed=3, tc_range=(0.0, 655360.0)):
 
 def make_detector(orbit="LISA_analytic", psd_model="LISA_SciRDv1", gen="1.5", T=655360.0, dt=5.0,
                   fmin=1e-4, fmax=0.05, seed=7, modes=((2, 2),), interp=(1e-4, 0.05, 1024), orthogonal=True,
-                  channels=("A", "E", "T"), inject=True):
+                  channels=("A", "E", "T"), inject=True, duty=None):
     f, mask, n_t, df, _ = odet.data_grid(T, dt, fmin, fmax)
     labels = ("A", "E", "T") if orthogonal else ("X", "Y", "Z")
     if not orthogonal and channels == ("A", "E", "T"):
@@ -64,12 +64,12 @@ def make_detector(orbit="LISA_analytic", psd_model="LISA_SciRDv1", gen="1.5", T=
     psd = odet.psd(psd_model, f, labels, False, gen)
     noise = odet.noise_realization(seed, labels, n_t, mask, psd, df)
     orb = odet.ORBITS[orbit] if isinstance(orbit, str) else tuple(orbit)
-    d0 = olk.Detector(f, None, psd, df, orb, gen, channels, orthogonal)
+    d0 = olk.Detector(f, None, psd, df, orb, gen, channels, orthogonal, duty)
     data = dict(noise)
     if inject:
         resp = olk.response(INJ, d0, modes, 0.0, interp)
         data = {c: noise[c] + resp[c] for c in labels}
-    return olk.Detector(f, data, psd, df, orb, gen, channels, orthogonal), labels
+    return olk.Detector(f, data, psd, df, orb, gen, channels, orthogonal, duty), labels
 
 
 def add_to_core(core, d, labels):
@@ -80,7 +80,7 @@ def add_to_core(core, d, labels):
     mask = [1 if c in d.channels else 0 for c in labels]
     gen = 15 if d.tdi_generation == "1.5" else 20
     return core.add_detector(ORBIT_KIND[kind], arm, rot0, rev0, gen, d.orthogonal, d.f, data, psd, mask, df=d.df,
-                             **geo)
+                             duty_cycle=getattr(d, "duty", None), **geo)
 
 
 YEAR_SI = 31557600.0
@@ -89,11 +89,17 @@ NETWORK = (("LISA", "LISA_analytic", "LISA_SciRDv1", 1e-4, 0.05),
            ("TianQin", "Tianqin_analytic", "Tianqin_GWSpace", 1e-3, 0.039))
 
 
+# TianQin's observation duty cycle in the benchmark network: three months on, three months off,
+# one-day ramps
+TIANQIN_DUTY = (YEAR_SI / 2, YEAR_SI / 4, 0.0, 86400.0)
+
+
 def network_core(modes=("22", "21", "33", "32", "44", "43"), T=YEAR_SI, dt=5.0, seed=7, n_nodes=1024,
-                 inj=None):
+                 inj=None, tianqin_duty=None):
     """LISA + Taiji + TianQin on rfft grids of duration T (ragged: 1,574,725 / 1,890,301 /
     1,199,189 bins for one year at dt = 5 s), seeded noise plus the injection, through the
-    product's host API (the data of the benchmark configurations).  Returns the core."""
+    product's host API (the data of the benchmark configurations), TianQin with the given duty
+    cycle (None: always observing).  Returns the core."""
     from pespace_amd.core import LikelihoodCore
     from pespace_amd.detector.antenna import (FDResponseModelMarset2018, InterferometerAntenna,
                                               add_detector_to_core)
@@ -109,7 +115,8 @@ def network_core(modes=("22", "21", "33", "32", "44", "43"), T=YEAR_SI, dt=5.0, 
         td.set_fd_data_from_zero(("A", "E", "T"), T, dt, 0.0, lo, hi)
         td.set_fd_noise_power_density_from_model(psd, tdi_generation="1.5")
         det = InterferometerAntenna(name, td, available_orbit_models()[orbit], FDResponseModelMarset2018(),
-                                    FDMichelsonConstantEqualArm("1.5", True))
+                                    FDMichelsonConstantEqualArm("1.5", True),
+                                    duty_cycle=tianqin_duty if name == "TianQin" else None)
         dets.append(det)
     wf = IMRPhenomHM(dets[0].tdi_data.frequency_samples, 0.0, modes=modes, n_nodes=n_nodes, f_node_min=fmin,
                      f_node_max=fmax)

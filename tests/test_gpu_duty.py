@@ -1,0 +1,150 @@
+"""Duty-cycled detectors (pes_context_set_duty_cycle) against the oracle.
+
+The benchmark network's TianQin observes three months on, three months off; the reference
+models no duty cycle, so the window is this build's (oracle/detector.py duty_window: each
+mode's response at a bin weighted by a C1 smoothstep window at t_f + t_c).  Checked here on
+short grids whose t_f + t_c spans several windows, so that bins fall on plateaus, ramps and
+off periods:
+
+* the TDI response, the links export, the container response and lnL against the oracle
+  (rtol 1e-10), heliocentric and geocentric orbits, the forward-only and gradient kernels;
+* the adjoint gradient against Richardson central differences of the HIP lnL (the window's
+  slope enters t_c and, through t_f, the intrinsic parameters);
+* the mixed mode's lnL equals the FP64 lnL (its forward is FP64) and its gradient agrees to 1e-3;
+* a three-detector network with only TianQin duty-cycled.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import detector as odet
+from oracle import likelihood as olk
+from oracle import phenom as oph
+from tests._setup import add_to_core, inj_row, make_detector, particle_cloud, prior_rows
+
+pytestmark = pytest.mark.gpu
+
+INTERP = (1e-4, 0.05, 1024)
+# windows across the t_f + t_c range of the test signal: on a linear grid most live bins lie
+# within ~500 s of t_c = 524,288 s (merger and ringdown), so the windows are that short here
+# (14-23% of the injection's live (bin, mode) off, 9-35% on a ramp); the early inspiral's bins
+# cross many windows
+DUTY_HELIO = (800.0, 500.0, 524000.0, 80.0)
+DUTY_GEO = (400.0, 250.0, 524100.0, 50.0)
+CASES = {
+    "lisa": dict(modes=oph.MODES, duty=DUTY_HELIO),
+    "lisa22-2.0-xyz": dict(modes=((2, 2),), gen="2.0", orthogonal=False, duty=DUTY_HELIO),
+    "tianqin": dict(modes=oph.MODES, orbit="Tianqin_analytic", psd_model="Tianqin_GWSpace", fmin=1e-3,
+                    duty=DUTY_GEO),
+}
+
+
+def _core(modes, dets):
+    from pespace_amd.core import LikelihoodCore
+    core = LikelihoodCore(modes=[f"{l}{m}" for l, m in modes], n_nodes=1024, f_node_min=1e-4, f_node_max=0.05)
+    for d, labels in dets:
+        add_to_core(core, d, labels)
+    return core
+
+
+def _relnorm(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _window_coverage(d, row, modes):
+    """Fractions of (bin, mode) at w = 0, 0 < w < 1 for this row."""
+    p = olk.as_dict(row)
+    wf = olk.waveform(p, d.f, modes, 0.0, INTERP)
+    w = np.concatenate([odet.duty_window(m["tf"] + p["coalescence_time"], d.duty) for m in wf.values()])
+    return np.mean(w == 0.0), np.mean((w > 0.0) & (w < 1.0))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_duty_response_links_and_lnl(name):
+    case = dict(CASES[name])
+    modes = case.pop("modes")
+    d, labels = make_detector(modes=modes, **case)
+    core = _core(modes, [(d, labels)])
+    rows = np.vstack([inj_row(), particle_cloud(3)])
+    off, ramp = _window_coverage(d, rows[0], modes)
+    print(f"{name}: (bin, mode) off {off:.3f}, on a ramp {ramp:.3f}", flush=True)
+    assert off > 0.05 and ramp > 0.01
+    resp = core.response(0, rows).cpu().numpy()
+    links = core.links(0, rows).cpu().numpy()
+    for p in range(rows.shape[0]):
+        prm = olk.as_dict(rows[p])
+        ref = olk.response(prm, d, modes, 0.0, INTERP)
+        for ci, c in enumerate(labels):
+            tol = 1e-10
+            if c == "T":
+                xyz = np.sqrt(np.sum(np.abs(ref[labels[0]]) ** 2 + np.abs(ref[labels[1]]) ** 2))
+                tol = max(tol, 64 * np.finfo(float).eps * xyz / np.linalg.norm(ref["T"]))
+            assert _relnorm(resp[p, ci], ref[c]) < tol, (p, c)
+        wf = olk.waveform(prm, d.f, modes, 0.0, INTERP)
+        ref_links = odet.single_links(wf, prm["ecliptic_longitude"], prm["ecliptic_latitude"], prm["polarization"],
+                                      prm["coalescence_time"], d.f, d.orbit, d.duty)
+        for li, nm in enumerate(("link12", "link21", "link23", "link32", "link31", "link13")):
+            assert _relnorm(links[p, li], ref_links[nm]) < 1e-10, (p, nm)
+    np.testing.assert_allclose(core.lnl(rows).cpu().numpy(), olk.lnl_batch(rows, [d], modes, 0.0, INTERP),
+                               rtol=1e-10)
+    lnl_g, _ = core.lnl(rows, grad=True)
+    np.testing.assert_allclose(lnl_g.cpu().numpy(), olk.lnl_batch(rows, [d], modes, 0.0, INTERP), rtol=1e-10)
+
+
+def test_duty_container_response():
+    modes = oph.MODES
+    d, labels = make_detector(modes=modes, duty=DUTY_HELIO)
+    core = _core(modes, [(d, labels)])
+    prm = olk.as_dict(inj_row())
+    wf = olk.waveform(prm, d.f, modes, 0.0, INTERP)
+    got = core.response_from_container(0, wf, prm["ecliptic_longitude"], prm["ecliptic_latitude"],
+                                       prm["polarization"], prm["coalescence_time"]).cpu().numpy()
+    ref = olk.response(prm, d, modes, 0.0, INTERP, wf=wf)
+    for ci, c in enumerate(labels[:2]):
+        assert _relnorm(got[ci], ref[c]) < 1e-10, c
+
+
+@pytest.mark.parametrize("name", ["lisa", "tianqin"])
+def test_duty_gradient_vs_fd(name):
+    from tests.test_gpu_grad import _fd_check_core
+    case = dict(CASES[name])
+    modes = case.pop("modes")
+    d, labels = make_detector(modes=modes, **case)
+    core = _core(modes, [(d, labels)])
+    _fd_check_core(core, np.vstack([inj_row(), particle_cloud(2, rel=1e-5)]))
+
+
+def test_duty_mixed_mode():
+    case = dict(CASES["tianqin"])
+    modes = case.pop("modes")
+    d, labels = make_detector(modes=modes, **case)
+    core = _core(modes, [(d, labels)])
+    rows = np.vstack([inj_row(), particle_cloud(3), prior_rows(4, seed=12)])
+    l64, g64 = (x.cpu().numpy() for x in core.lnl(rows, grad=True, precision="fp64"))
+    lmx, gmx = (x.cpu().numpy() for x in core.lnl(rows, grad=True, precision="mixed"))
+    ok = np.isfinite(l64)
+    np.testing.assert_allclose(lmx[ok], l64[ok], rtol=1e-13)
+    scale = np.maximum(np.abs(g64[ok]).max(axis=0), 1e-300)
+    assert np.max(np.abs(gmx[ok] - g64[ok]) / scale) < 1e-3
+
+
+def test_duty_network_only_tianqin():
+    modes = oph.MODES
+    dets = [make_detector(modes=modes, seed=7),
+            make_detector(modes=modes, orbit="Taiji_analytic", psd_model="Taiji_TDC", fmax=0.03, seed=8),
+            make_detector(modes=modes, orbit="Tianqin_analytic", psd_model="Tianqin_GWSpace", fmin=1e-3, seed=9,
+                          duty=DUTY_GEO)]
+    core = _core(modes, dets)
+    rows = np.vstack([inj_row(), particle_cloud(2)])
+    np.testing.assert_allclose(core.lnl(rows).cpu().numpy(),
+                               olk.lnl_batch(rows, [d for d, _ in dets], modes, 0.0, INTERP), rtol=1e-10)
+    # the same data with TianQin always observing: the model then has signal in the off
+    # periods, which the data lack, so lnL drops (and matches the oracle's without the window)
+    import copy
+    tq = copy.copy(dets[2][0])
+    tq.duty = None
+    plain = _core(modes, dets[:2] + [(tq, dets[2][1])])
+    l_plain = plain.lnl(rows).cpu().numpy()
+    np.testing.assert_allclose(l_plain, olk.lnl_batch(rows, [dets[0][0], dets[1][0], tq], modes, 0.0, INTERP),
+                               rtol=1e-10)
+    assert np.all(core.lnl(rows).cpu().numpy() - l_plain > 1.0)
