@@ -199,3 +199,51 @@ def test_lnl_of_noiseless_injection_is_zero():
     resp = olk.response(p, d0, ((2, 2),), 0.0)
     d = olk.Detector(f, resp, psd, df, orb)
     assert olk.lnl(p, [d]) == 0.0
+
+
+def test_duty_window_closed_forms():
+    """The observation window of a duty-cycled detector: plateau 1, off 0, smoothstep ramps
+    S(x) = x^2 (3 - 2x) (S(1/4) = 5/32, S(1/2) = 1/2), periodic, C1 at the ramp ends."""
+    period, on, t0, taper = 100.0, 60.0, 7.0, 10.0
+    duty = (period, on, t0, taper)
+    for k in (-3, 0, 5):
+        base = t0 + k * period
+        np.testing.assert_allclose(det.duty_window(base + np.array([10.0, 30.0, 50.0]), duty), 1.0)
+        np.testing.assert_array_equal(det.duty_window(base + np.array([60.5, 75.0, 99.9]), duty), 0.0)
+        np.testing.assert_allclose(det.duty_window(base + np.array([2.5, 5.0, 57.5, 55.0]), duty),
+                                   [5 / 32, 0.5, 5 / 32, 0.5], rtol=1e-12)
+    # slopes: zero at both ends of each ramp (C1), 1.5 / taper at mid-ramp
+    h = 1e-5
+    for tt, slope in ((t0 + 1e-3, 0.0), (t0 + 10.0, 0.0), (t0 + 5.0, 1.5 / taper), (t0 + 55.0, -1.5 / taper),
+                      (t0 + 60.0 - 1e-3, 0.0)):
+        d = (det.duty_window(tt + h, duty) - det.duty_window(tt - h, duty)) / (2 * h)
+        assert abs(d - slope) < 1e-3 / taper, (tt, d, slope)
+    # no ramps: a hard window
+    np.testing.assert_array_equal(det.duty_window(np.array([t0, t0 + 59.9, t0 + 60.0]), (period, on, t0, 0.0)),
+                                  [1.0, 1.0, 0.0])
+
+
+def test_duty_cycled_links_are_the_windowed_links():
+    """With a duty cycle each mode's links carry the window at its own t_f + t_c; a window that
+    is 1 everywhere (on = period, no ramps) changes nothing."""
+    rng = np.random.default_rng(2)
+    f = np.linspace(1e-3, 1e-2, 50)
+    wf = {m: {"plus": rng.standard_normal(50) + 1j * rng.standard_normal(50),
+              "cross": rng.standard_normal(50) + 1j * rng.standard_normal(50),
+              "tf": np.linspace(-3e5, 1e3, 50) * (i + 1)} for i, m in enumerate(("22", "33"))}
+    orb = det.ORBITS["LISA_analytic"]
+    args = (0.3, 0.2, 0.1, 1e5, f, orb)
+    plain = det.single_links(wf, *args)
+    np.testing.assert_array_equal(det.single_links(wf, *args, (1e6, 1e6, 0.0, 0.0))["link12"], plain["link12"])
+    duty = (2e5, 1.2e5, 3e4, 2e4)
+    got = det.single_links(wf, *args, duty)
+    ref = {n: 0.0 for n in plain}
+    for m, w in wf.items():
+        scaled = dict(w)
+        win = det.duty_window(w["tf"] + 1e5, duty)
+        scaled["plus"], scaled["cross"] = w["plus"] * win, w["cross"] * win
+        one = det.single_links({m: scaled}, *args)
+        for n in ref:
+            ref[n] = ref[n] + one[n]
+    for n in plain:
+        np.testing.assert_allclose(got[n], ref[n], rtol=1e-14, atol=0)
