@@ -25,12 +25,15 @@ from tests._setup import add_to_core, inj_row, make_detector, particle_cloud, pr
 pytestmark = pytest.mark.gpu
 
 INTERP = (1e-4, 0.05, 1024)
-# windows across the t_f + t_c range of the test signal: on a linear grid most live bins lie
-# within ~500 s of t_c = 524,288 s (merger and ringdown), so the windows are that short here
-# (14-23% of the injection's live (bin, mode) off, 9-35% on a ramp); the early inspiral's bins
-# cross many windows
-DUTY_HELIO = (800.0, 500.0, 524000.0, 80.0)
-DUTY_GEO = (400.0, 250.0, 524100.0, 50.0)
+# On a linear grid most live bins lie within ~500 s of t_c = 524,288 s (merger and ringdown),
+# so the test windows put one edge there, with a short ramp: LISA's window ends at 524,200 s
+# (an 80 s falling ramp; 57% of the injection's live (bin, mode) off, 8% on the ramp), TianQin's
+# starts at 524,100 s (a 50 s rising ramp; 31% off, 5% on it).  The windows' other edges lie
+# 5e7 s away, beyond every bin, so no bin with a large |t_f| sits on a ramp: there a window
+# that steep would turn the last-digit differences of t_f between the kernel and the oracle
+# (~1e-13 of 1e7 s) into visible ones.
+DUTY_HELIO = (1.0e8, 5.0e7, 524200.0 - 5.0e7, 80.0)
+DUTY_GEO = (1.0e8, 5.0e7, 524100.0, 50.0)
 CASES = {
     "lisa": dict(modes=oph.MODES, duty=DUTY_HELIO),
     "lisa22-2.0-xyz": dict(modes=((2, 2),), gen="2.0", orthogonal=False, duty=DUTY_HELIO),
