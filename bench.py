@@ -45,6 +45,10 @@ DETECTORS = {
     "Taiji": ("Taiji_analytic", "Taiji_TDC", 1e-4, 0.06),
     "TianQin": ("Tianqin_analytic", "Tianqin_GWSpace", 1e-3, 0.039),
 }
+# TianQin's observation duty cycle in the network workloads: three months on, three months off
+# (period half a year), windows from 1.5 months into the year (so that the canonical injection's
+# merger at 0.8 yr falls in a window), one-day ramps
+DUTY = {"TianQin": (YEAR_SI / 2, YEAR_SI / 4, YEAR_SI / 8, 86400.0)}
 HM = ("22", "21", "33", "32", "44", "43")
 WORKLOADS = {
     # 1-yr rfft grids, 1024 interpolation nodes, particle cloud of width 1e-3 around the injection
@@ -93,8 +97,9 @@ def build_network(workload, seed=7):
             f = np.geomspace(fmin, fmax, 2 ** 14)
             td.set_fd_data_on_grid(("A", "E", "T"), f, np.gradient(f))
         td.set_fd_noise_power_density_from_model(psd, tdi_generation="1.5")
+        duty = DUTY.get(name) if len(wl["detectors"]) > 1 else None
         det = InterferometerAntenna(name, td, available_orbit_models()[orbit], FDResponseModelMarset2018(),
-                                    FDMichelsonConstantEqualArm("1.5", True), backend="hip")
+                                    FDMichelsonConstantEqualArm("1.5", True), backend="hip", duty_cycle=duty)
         dets.append(det)
     fmin = min(DETECTORS[n][2] for n in wl["detectors"])
     fmax = max(DETECTORS[n][3] for n in wl["detectors"])
@@ -219,7 +224,8 @@ def cpu_baseline(dets, wf, rows, n_rows=2):
         if df is None:
             df = np.asarray(td.frequency_weights())
         odets.append(olk.Detector(f, data, psd, df, (kind, orb.arm_length, orb.rotation_initial,
-                                                       orb.revolution_initial, *geo), "1.5"))
+                                                       orb.revolution_initial, *geo), "1.5",
+                                  duty=tuple(d.duty_cycle) if d.duty_cycle is not None else None))
     net = cb.Network(odets)
     interp = (wf.f_node_min, wf.f_node_max, wf.n_nodes)
     ris = [cb.RowInputs(r, modes, wf.reference_frequency, interp, grad=True) for r in rows[:n_rows]]
@@ -414,6 +420,7 @@ def main():
         "data": "synthetic (seeded noise + injected signal, random particle cloud)",
         "config": {"workload": args.workload, "particles": P, "bins_per_detector": nbins, "modes": list(wf.modes),
                    "nodes": wf.n_nodes, "gradient": grad, "precision": head,
+                   "duty_cycle": {d.name: dict(d.duty_cycle._asdict()) for d in dets if d.duty_cycle is not None},
                    "schedule": tune if tune else dict(zip(("tiles_per_group", "blocks_per_cu"), core.schedule()))},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": traffic,

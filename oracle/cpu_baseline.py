@@ -50,7 +50,8 @@ def _p(a, t=c_dp):
 class Network:
     """Concatenated detector arrays.  dets: objects with f, data {c: complex}, psd {c: real},
     df (scalar or per-bin), orbit (kind, arm[m], rot0, rev0[, omega_rot, lambda_ref, beta_ref]),
-    tdi_generation ("1.5"/"2.0"), channels, orthogonal (the oracle's likelihood.Detector)."""
+    tdi_generation ("1.5"/"2.0"), channels, orthogonal and optionally duty (period, on_duration,
+    start_time, taper) (the oracle's likelihood.Detector)."""
 
     def __init__(self, dets):
         from .detector import GEO_BETA_REF, GEO_LAMBDA_REF, GEO_OMEGA_ROT
@@ -72,8 +73,10 @@ class Network:
                     self.dim[c, o:o + k] = np.imag(d.data[lab])
             kind, arm, rot0, rev0 = d.orbit[:4]
             geo = tuple(d.orbit[4:]) or (GEO_OMEGA_ROT, GEO_LAMBDA_REF, GEO_BETA_REF)
-            det_i += [0 if kind == "helio" else 1, 15 if d.tdi_generation == "1.5" else 20, int(d.orthogonal), 0]
-            det_d += [arm / C_SI, rot0, rev0, *geo]
+            duty = getattr(d, "duty", None)
+            det_i += [0 if kind == "helio" else 1, 15 if d.tdi_generation == "1.5" else 20, int(d.orthogonal),
+                      int(duty is not None)]
+            det_d += [arm / C_SI, rot0, rev0, *geo, *(duty if duty is not None else (0.0, 0.0, 0.0, 0.0))]
         self.det_i = np.array(det_i, dtype=np.int32)
         self.det_d = np.array(det_d, dtype=np.float64)
         self.n_det = len(dets)

@@ -164,7 +164,24 @@ inline T dot(const V3<T>& a, const V3<U>& b) { return a.x * b.x + a.y * b.y + a.
 struct Det {
   int kind, gen, orth, n, off;  // off: first bin in the concatenated arrays
   double L, rot0, rev0, omega_rot, slam, clam, sbet, cbet;
+  int duty;                                      // observation duty cycle (0: always observing)
+  double duty_period, duty_on, duty_t0, duty_taper;
 };
+
+// The duty window at time t (oracle/detector.py duty_window): s = (t - t0) mod period, the
+// smoothstep S(x) = x^2 (3 - 2x) of the distance into [0, on) over the taper, 0 outside; the
+// window's position s carries t's derivatives (the floor does not)
+template <class T>
+T duty_w(const Det& d, const T& t) {
+  const double u = (value(t) - d.duty_t0) / d.duty_period;
+  const T s = t - (d.duty_t0 + std::floor(u) * d.duty_period);
+  const double sv = value(s);
+  if (sv >= d.duty_on) return T(0.0);
+  if (d.duty_taper <= 0.0) return T(1.0);
+  const T x = (sv <= d.duty_on - sv ? s : T(d.duty_on) - s) * (1.0 / d.duty_taper);
+  if (value(x) >= 1.0) return T(1.0);
+  return x * x * (T(3.0) - x * 2.0);
+}
 
 struct Model {
   int n_modes, n_nodes;
@@ -290,9 +307,10 @@ T lnl_row(const std::vector<Det>& dets, const double* f_all, const double* w_all
         const T dPds = d00 * P0 + (d10 * M.du) * dP0 + d01 * P1 + (d11 * M.du) * dP1;
         const T tf = dPds * (1.0 / (2.0 * PI * M.du * f));
         const Cx<T> h = cmul(cexpi(T(0.0) - Psi), ex);  // e^{-i Psi} e^{-2 pi i f t_c}
-        const Cx<T> hA = {h.re * A, h.im * A};
-        const Cx<T> hp = cmul(Kp[m], hA), hc = cmul(Kc[m], hA);
         const T t = tf + tc;
+        const T Aw = d.duty ? A * duty_w(d, t) : A;
+        const Cx<T> hA = {h.re * Aw, h.im * Aw};
+        const Cx<T> hp = cmul(Kp[m], hA), hc = cmul(Kc[m], hA);
         V3<T> nv[3], xv[3];
         constellation(d, t, nv, xv);
         T nhn_re[3], nhn_im[3], kn[3];
@@ -368,8 +386,9 @@ extern "C" {
 // One row: lnL (and, if grad != NULL, its gradient over the 11 parameters) on the network.
 // Detector arrays are concatenated: f_all[NB], w_all / dre_all / dim_all [3][NB] (channel-major,
 // w = 2 df / S, 0 for unused channels); detector d owns bins [det_off[d], det_off[d] + det_n[d]).
-// det_i: [n_det][4] = kind, tdi generation (15, 20), orthogonal, (unused);
-// det_d: [n_det][6] = L [s], rot0, rev0, omega_rotation, lambda_ref, beta_ref.
+// det_i: [n_det][4] = kind, tdi generation (15, 20), orthogonal, duty cycle (0 / 1);
+// det_d: [n_det][10] = L [s], rot0, rev0, omega_rotation, lambda_ref, beta_ref, and the duty
+// cycle's period, on duration, start time, taper [s].
 // stride: evaluate every stride-th bin with its weight multiplied by stride (a bounded sample).
 int cpu_lnl_row(int n_det, const int* det_i, const double* det_d, const int* det_off, const int* det_n, int NB,
                 const double* f_all, const double* w_all, const double* dre_all, const double* dim_all, int n_modes,
@@ -385,14 +404,20 @@ int cpu_lnl_row(int n_det, const int* det_i, const double* det_d, const int* det
     d.orth = det_i[4 * i + 2];
     d.off = det_off[i];
     d.n = det_n[i];
-    d.L = det_d[6 * i];
-    d.rot0 = det_d[6 * i + 1];
-    d.rev0 = det_d[6 * i + 2];
-    d.omega_rot = det_d[6 * i + 3];
-    d.slam = std::sin(det_d[6 * i + 4]);
-    d.clam = std::cos(det_d[6 * i + 4]);
-    d.sbet = std::sin(det_d[6 * i + 5]);
-    d.cbet = std::cos(det_d[6 * i + 5]);
+    const double* dd = det_d + 10 * i;
+    d.L = dd[0];
+    d.rot0 = dd[1];
+    d.rev0 = dd[2];
+    d.omega_rot = dd[3];
+    d.slam = std::sin(dd[4]);
+    d.clam = std::cos(dd[4]);
+    d.sbet = std::sin(dd[5]);
+    d.cbet = std::cos(dd[5]);
+    d.duty = det_i[4 * i + 3];
+    d.duty_period = dd[6];
+    d.duty_on = dd[7];
+    d.duty_t0 = dd[8];
+    d.duty_taper = dd[9];
   }
   Model M{n_modes, n_nodes, u0, du, nodes, nodes_grad, kfac, kfac_d, mode_cut};
   if (grad) {

@@ -32,6 +32,10 @@ CASES = [
     dict(modes=oph.MODES, orbit=("geo", 1.7e8, 0.4, 1.1, 2 * np.pi / (5 * 86400.0), 0.7, 0.3),
          psd_model="Tianqin_GWSpace", fmin=1e-3),
     dict(modes=oph.MODES, channels=("A", "E")),
+    # observation duty cycles with an edge at the merger (on the grid's t_f + t_c range)
+    dict(modes=oph.MODES, duty=(1.0e8, 5.0e7, 524400.0 - 5.0e7, 2000.0)),
+    dict(modes=oph.MODES, orbit="Tianqin_analytic", psd_model="Tianqin_GWSpace", fmin=1e-3,
+         duty=(1.0e8, 5.0e7, 523000.0, 1500.0)),
 ]
 
 
@@ -59,9 +63,10 @@ def test_cpu_restatement_network_lnl(cb):
     assert abs(got - ref) < 1e-12 * abs(ref)
 
 
-def test_cpu_restatement_gradient_matches_richardson_differences(cb):
+@pytest.mark.parametrize("duty", [None, (1.0e8, 5.0e7, 524400.0 - 5.0e7, 2000.0)])
+def test_cpu_restatement_gradient_matches_richardson_differences(cb, duty):
     modes = oph.MODES
-    d, _ = make_detector(modes=modes, T=2 ** 13 * 5.0)
+    d, _ = make_detector(modes=modes, T=2 ** 13 * 5.0, duty=duty)
     net = cb.Network([d])
     row = particle_cloud(1, seed=11)[0]
     ri = cb.RowInputs(row, modes, 0.0, INTERP, grad=True)
@@ -84,4 +89,6 @@ def test_cpu_restatement_gradient_matches_richardson_differences(cb):
         fd[i] = (4 * d2 - d1) / 3
     scale = np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max())
     err = np.abs(g - fd) / scale
-    assert np.all(err < 1e-5), err
+    # the node tables' parameter derivatives are central differences (~1e-9 relative); under a
+    # duty window their t_f errors are weighed by the window's slope as well (q: ~1e-5)
+    assert np.all(err < (1e-5 if duty is None else 3e-5)), err
