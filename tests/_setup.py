@@ -90,8 +90,8 @@ NETWORK = (("LISA", "LISA_analytic", "LISA_SciRDv1", 1e-4, 0.05),
 
 
 # TianQin's observation duty cycle in the benchmark network: three months on, three months off,
-# one-day ramps
-TIANQIN_DUTY = (YEAR_SI / 2, YEAR_SI / 4, 0.0, 86400.0)
+# windows from 1.5 months into the year, one-day ramps
+TIANQIN_DUTY = (YEAR_SI / 2, YEAR_SI / 4, YEAR_SI / 8, 86400.0)
 
 
 def network_core(modes=("22", "21", "33", "32", "44", "43"), T=YEAR_SI, dt=5.0, seed=7, n_nodes=1024,

@@ -18,7 +18,8 @@ import pytest
 
 from oracle import likelihood as olk
 from oracle import phenom as oph
-from tests._setup import YEAR_SI, add_to_core, inj_row, make_detector, network_core, particle_cloud, prior_rows
+from tests._setup import (TIANQIN_DUTY, YEAR_SI, add_to_core, inj_row, make_detector, network_core, particle_cloud,
+                          prior_rows)
 
 pytestmark = pytest.mark.gpu
 
@@ -84,9 +85,10 @@ def test_short_grid_wide_prior_against_oracle(P):
 
 
 def test_network_one_year_uneven_prior_batches():
-    """The mixed-precision benchmark configuration at full size: LISA + Taiji + TianQin, 1-yr
-    grids (4.66e6 bins), six modes, batches of 256, 3001 and 8192 rows from the wide prior."""
-    core = network_core()
+    """The mixed-precision benchmark configuration at full size: LISA + Taiji + TianQin (its
+    3-month on/off duty cycle), 1-yr grids (4.66e6 bins), six modes, batches of 256, 3001 and
+    8192 rows from the wide prior."""
+    core = network_core(tianqin_duty=TIANQIN_DUTY)
     assert core.n_bins == [1574725, 1890301, 1199189]
     stats = []
     for P in (256, 3001, 8192):
