@@ -50,9 +50,21 @@ phase at every region boundary; Newtonian scaling A ~ f^(-7/6) at low f;
 exact mass scaling h(f; lam M) = lam^2 h(lam f; M), t(f; lam M) = lam t(lam f; M);
 1/dL scaling; (2,2) of the HM model is bitwise PhenomD; odd-m modes vanish
 for q = 1, chi1 = chi2; Psi_lm(f) = (m/2) Psi_22(2f/m) in the inspiral; the
-harmonic factors match closed-form spin-weighted harmonics.  The fit
-coefficients themselves cannot be checked against any executable in this
-container (tiwave, lalsuite and BBHx are absent): parity unpinned for them.
+harmonic factors match closed-form spin-weighted harmonics; with
+construction="published" the HM modes follow London et al. as published and
+agree with the C1 default below the inspiral joints.
+Known answers (tests/test_oracle_known_answers.py): the 1.5PN spin-orbit
+phase coefficient against Poisson & Will's beta, the 2PN spin-spin one against
+Arun et al.'s sigma, the spin terms of t(f) in the test-mass limit against the
+Kerr circular-orbit energy and the Tagoshi et al. flux (in full through 3PN, the
+part odd in the spin at 3.5PN), and the non-spinning inspiral amplitude through
+2.5PN against |H_22| sqrt(R) from Blanchet et al.
+Parity unpinned: the calibrated fit coefficients (Khan et al. table V, the
+final-spin and radiated-energy fits), the 3PN amplitude term and the spin
+terms of the amplitude, and the 3.5PN spin-squared phase term (it restates
+get_PN_d_phase, whose value differs from the test-mass limit by 65 pi chi^2 / 8
+in the flux): no executable reference for them is in this container (tiwave,
+lalsuite and BBHx are absent).
 """
 
 from __future__ import annotations

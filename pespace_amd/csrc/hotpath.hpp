@@ -163,7 +163,6 @@ constexpr bool kDuty = (KIND & 2) != 0;
 
 struct ModeG {
   double A, t;      // amplitude, t_f + t_c
-  double w, dw;     // duty window at t and dw/dt (duty-cycled detectors)
   Cx u, hm, hp, hc; // unit phasor, A u, F+ A u, 2 Fx A u
   Cx z0, z1, z01;   // pair phasors exp(i pi f k.xd_a) of nodes 0, 1, and their product
   double sa, ca;    // revolution angle
@@ -184,7 +183,11 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
   const double dP = P1 - P0;
   const double Psi = fma(b.hb1, dP, fma(b.hb2, dP0, fma(b.hb3, dP1, P0)));
   g.t = fma(b.he1, dP, fma(b.he2, dP0, fma(b.he3, dP1, rc.tc)));
-  if constexpr (kDuty<KIND>) duty_window(det, g.t, g.w, g.dw);
+  double w = 1.0;  // duty window at t (its slope is recomputed by the adjoint)
+  if constexpr (kDuty<KIND>) {
+    double dw;
+    duty_window(det, g.t, w, dw);
+  }
   sincos_tab_s(fma(kOmegaYear, g.t, det.rev0), &g.sa, &g.ca);
   const double sa = g.sa, ca = g.ca;
   g.kx0 = AU_SEC * fma(rc.k.x, ca, rc.k.y * sa);
@@ -255,7 +258,7 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
     sincos_tab_sum(rc.tc_r, rc.tc_k, fma(2.0 * b.pif, g.kx0, Psi), &sp, &cp);
   }
   g.u = {cp, -sp};
-  g.hm = scale(g.u, kDuty<KIND> ? g.A * g.w : g.A);
+  g.hm = scale(g.u, kDuty<KIND> ? g.A * w : g.A);
   g.hp = mul(Cx{mf[MF_FPR], mf[MF_FPI]}, g.hm);
   g.hc = mul(Cx{mf[MF_FCR], mf[MF_FCI]}, g.hm);
 }
@@ -509,9 +512,15 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
   const Cx Gu = mul(Gm, g.u);
   double Ab = Gu.re;  // adjoint of the amplitude the response sees (A w with a duty window)
   double tb = fma(kx0b, kx0_t, tgeo);  // adjoint of t = t_f + t_c
-  if constexpr (kDuty<KIND>) {  // h ~ A w(t): the window's slope moves t, A sees w
-    tb = fma(Ab * g.A, g.dw, tb);
-    Ab *= g.w;
+  if constexpr (kDuty<KIND>) {
+    // h ~ A w(t): the window's slope moves t, A sees w.  The window is recomputed from t here
+    // and A w is read off h = A w u (|u| = 1), so that only t stays live through the pair
+    // loop; w = 0 only where w' = 0 too (outside the window, or at a ramp's closed end)
+    double w, dw;
+    duty_window(det, g.t, w, dw);
+    const double Aw = fma(g.hm.re, g.u.re, g.hm.im * g.u.im);
+    tb = w > 0.0 ? fma(Ab * Aw, dw / w, tb) : tb;
+    Ab *= w;
   }
   acc.gh_re = fma(Gm.re, g.hm.re, fma(-Gm.im, g.hm.im, acc.gh_re));  // += Re(Gm hm)
   acc.mphi = fma((double)kModeM[mid], phib, acc.mphi);
