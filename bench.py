@@ -80,7 +80,7 @@ PEAK_FP64 = 78.6e12  # MI355X FP64 vector, FLOP/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lnl_traffic.json")
 
 
-def build_network(workload, seed=7):
+def build_network(workload, seed=7, duty=True):
     from pespace_amd.detector.antenna import FDResponseModelMarset2018, InterferometerAntenna
     from pespace_amd.detector.orbit import available_orbit_models
     from pespace_amd.detector.tdi import FDMichelsonConstantEqualArm, TDIChannelData
@@ -97,7 +97,7 @@ def build_network(workload, seed=7):
             f = np.geomspace(fmin, fmax, 2 ** 14)
             td.set_fd_data_on_grid(("A", "E", "T"), f, np.gradient(f))
         td.set_fd_noise_power_density_from_model(psd, tdi_generation="1.5")
-        duty = DUTY.get(name) if len(wl["detectors"]) > 1 else None
+        duty = DUTY.get(name) if duty and len(wl["detectors"]) > 1 else None
         det = InterferometerAntenna(name, td, available_orbit_models()[orbit], FDResponseModelMarset2018(),
                                     FDMichelsonConstantEqualArm("1.5", True), backend="hip", duty_cycle=duty)
         dets.append(det)
@@ -330,6 +330,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule of the fused kernel")
     ap.add_argument("--schedule", default=None, help="fixed schedule 'tiles_per_group,blocks_per_cu' (no autotune)")
     ap.add_argument("--no-td", action="store_true", help="skip the time-domain links/TDI timing")
+    ap.add_argument("--no-duty", action="store_true", help="network workloads without TianQin's duty cycle")
     ap.add_argument("--td-only", action="store_true", help="only time the time-domain links and TDI")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
@@ -350,7 +351,7 @@ def main():
         return
     grad = not args.no_grad
     t_setup = time.perf_counter()
-    dets, wf = build_network(args.workload)
+    dets, wf = build_network(args.workload, duty=not args.no_duty)
     core = LikelihoodCore(modes=wf.modes, reference_frequency=wf.reference_frequency, n_nodes=wf.n_nodes,
                           f_node_min=wf.f_node_min, f_node_max=wf.f_node_max)
     for d in dets:
