@@ -97,9 +97,9 @@ def build_network(workload, seed=7, duty=True):
             f = np.geomspace(fmin, fmax, 2 ** 14)
             td.set_fd_data_on_grid(("A", "E", "T"), f, np.gradient(f))
         td.set_fd_noise_power_density_from_model(psd, tdi_generation="1.5")
-        duty = DUTY.get(name) if duty and len(wl["detectors"]) > 1 else None
+        det_duty = DUTY.get(name) if duty and len(wl["detectors"]) > 1 else None
         det = InterferometerAntenna(name, td, available_orbit_models()[orbit], FDResponseModelMarset2018(),
-                                    FDMichelsonConstantEqualArm("1.5", True), backend="hip", duty_cycle=duty)
+                                    FDMichelsonConstantEqualArm("1.5", True), backend="hip", duty_cycle=det_duty)
         dets.append(det)
     fmin = min(DETECTORS[n][2] for n in wl["detectors"])
     fmax = max(DETECTORS[n][3] for n in wl["detectors"])
