@@ -852,7 +852,8 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
                                               const double* __restrict__ nodes,
                                               const double* __restrict__ nodes_grad, int NNS, double du,
                                               ModeList modes, int P, int rows, const int* __restrict__ tile_jr,
-                                              int* __restrict__ work_counter, double* __restrict__ partial) {
+                                              int g_begin, int g_end, int* __restrict__ work_counter,
+                                              double* __restrict__ partial) {
   constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
   constexpr int CH = chunk_rows<GRAD>();  // 4x the rows without the gradient: 1.3% faster
   __shared__ double acc[TILE / 64][CH][NOUT];
@@ -865,10 +866,11 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
   // there, 0.2% slower with the gradient
   const int lane = threadIdx.x & 63;
   const int wave = GRAD ? (int)(threadIdx.x >> 6) : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  // rows (1 <= rows <= CH) per work item, chosen by the host
+  // rows (1 <= rows <= CH) per work item, chosen by the host; this launch covers the tile
+  // groups [g_begin, g_end) (a network with a duty-cycled detector runs that detector's groups
+  // in the DUTY instantiation and the rest in the other, each launch with its own counter)
   const int n_chunks = (P + rows - 1) / rows;
-  const int n_groups = (n_tiles + tiles_per_group - 1) / tiles_per_group;
-  const int n_work = n_chunks * n_groups;
+  const int n_work = n_chunks * (g_end - g_begin);
   // work items are taken from a counter (zeroed by k_setup earlier on the same stream), so
   // blocks that draw cheap items (tiles above the mode cuts) take more of them; every block
   // leaves once the counter passes n_work
@@ -877,7 +879,7 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
     __syncthreads();
     const int w = next_work;
     if (w >= n_work) break;
-    const int g = w / n_chunks, c = w % n_chunks;
+    const int g = g_begin + w / n_chunks, c = w % n_chunks;
     const int p0 = c * rows, p1 = min(P, p0 + rows);
     for (int i = threadIdx.x; i < (TILE / 64) * CH * NOUT; i += TILE) (&acc[0][0][0])[i] = 0.0;
     double chunk_cut = 0.0;

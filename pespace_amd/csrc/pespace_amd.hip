@@ -67,7 +67,11 @@ __device__ inline Cplx mode_factor(int id) {
 __global__ __launch_bounds__(64) void k_setup(const double* __restrict__ params, int P, ModeList modes, double f_ref,
                         double* __restrict__ setup, double* __restrict__ pconst, int* __restrict__ work_counter) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) work_counter[0] = 0;  // work queue of the fused kernel that follows on this stream
+  if (p == 0) {  // work queues of the fused kernel launches that follow on this stream
+    work_counter[0] = 0;
+    work_counter[1] = 0;
+    work_counter[2] = 0;
+  }
   if (p >= P) return;
   const double* th = params + (size_t)p * PES_N_PARAMS;
   double Mc = th[0], q = th[1], c1 = th[2], c2 = th[3], dL = th[4], phi = th[5], iota = th[6];
@@ -649,10 +653,10 @@ __global__ void k_whittle_final(const double* __restrict__ work, int nb, double 
       const double *bin_rec, const double *tile_f0,                                                       \
       const double *tile_dd, int tiles_per_group, const double *pconst, const double *nodes,                 \
       const double *nodes_grad, int NNS, double du, ModeList modes, int P, int rows, const int *tile_jr,     \
-      int *work_counter, double *partial
+      int g_begin, int g_end, int *work_counter, double *partial
 #define PES_LNL_NAMES                                                                                     \
   dets, n_det, n_tiles, bin_f, bin_j, bin_s, bin_rec, tile_f0, tile_dd, tiles_per_group, pconst, \
-      nodes, nodes_grad, NNS, du, modes, P, rows, tile_jr, work_counter, partial
+      nodes, nodes_grad, NNS, du, modes, P, rows, tile_jr, g_begin, g_end, work_counter, partial
 
 }  // namespace
 
@@ -876,7 +880,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static size_t noalloc_layout(const pes_context* ctx, int P, bool grad, size_t off[6]) {
   const size_t NNS = (size_t)ctx->n_nodes + 1, rows = (size_t)P, nm = (size_t)ctx->modes.n;
   size_t o = 0;
-  off[0] = o; o += align256(sizeof(int));                                  // work counter
+  off[0] = o; o += align256(3 * sizeof(int));                              // work counters
   off[1] = o; o += align256(rows * N_SETUP * 5 * sizeof(double));          // setup
   off[2] = o; o += align256(rows * PC_STRIDE * sizeof(double));            // per-row constants
   off[3] = o; o += align256(rows * nm * NNS * 4 * sizeof(double));         // node tables
@@ -1008,7 +1012,7 @@ extern "C" int pes_context_finalize(pes_context* ctx) {
     HIP_TRY(hipMemcpy(ctx->d_tile_jr, tjr.data(), 2 * (size_t)nt * sizeof(int), hipMemcpyHostToDevice));
   }
   if (ctx->direct) ctx->d_node_f = ctx->d_bin_f;
-  HIP_TRY(hipMalloc(&ctx->d_work_counter, sizeof(int)));
+  HIP_TRY(hipMalloc(&ctx->d_work_counter, 3 * sizeof(int)));  // one per fused-kernel launch of a batch
   {  // the device address of the no-allocation entry point's scratch, once per device
     char* base = nullptr;
     const int rc = noalloc_base(ctx->device, &base);
@@ -1100,7 +1104,7 @@ static int check_batch_args(const pes_context* ctx, const double* d_params, int 
   return PES_OK;
 }
 
-// k_setup (which also zeroes the work counter) and k_nodes for P >= 1 rows
+// k_setup (which also zeroes the work counters) and k_nodes for P >= 1 rows
 static int launch_setup_nodes(const pes_context* ctx, const BatchScratch& sc, const double* d_params, int P, bool grad,
                               hipStream_t st) {
   hipLaunchKernelGGL(k_setup, dim3((P + 63) / 64), dim3(64), 0, st, d_params, P, ctx->modes, ctx->f_ref, sc.setup,
@@ -1145,9 +1149,26 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
   int grid = ctx->n_cu * ctx->blocks_per_cu;
   const int rows = rows_per_item(ctx, P, grad, grid);
   const int n_chunks = (P + rows - 1) / rows;
-  const int n_work = n_groups * n_chunks;
-  if (grid > n_work) grid = n_work;
   const int nout = grad ? 1 + PES_N_PARAMS : 1;
+  // tile-group ranges and the kernel variant for each: the groups holding a duty-cycled
+  // detector's tiles run the DUTY instantiation, the others the plain one (its register
+  // allocation is the one without the window code)
+  int ranges[3][3];  // g_begin, g_end, duty
+  int n_ranges = 0;
+  if (!ctx->any_duty) {
+    ranges[n_ranges][0] = 0, ranges[n_ranges][1] = n_groups, ranges[n_ranges][2] = 0, ++n_ranges;
+  } else {
+    int gd0 = n_groups, gd1 = 0;
+    for (const auto& h : ctx->hdets) {
+      if (!h.d.duty) continue;
+      const int t0 = h.d.bin_offset / TILE, t1 = t0 + h.d.n_tiles;
+      gd0 = std::min(gd0, t0 / ctx->tiles_per_group);
+      gd1 = std::max(gd1, (t1 + ctx->tiles_per_group - 1) / ctx->tiles_per_group);
+    }
+    if (gd0 > 0) ranges[n_ranges][0] = 0, ranges[n_ranges][1] = gd0, ranges[n_ranges][2] = 0, ++n_ranges;
+    ranges[n_ranges][0] = gd0, ranges[n_ranges][1] = gd1, ranges[n_ranges][2] = 1, ++n_ranges;
+    if (gd1 < n_groups) ranges[n_ranges][0] = gd1, ranges[n_ranges][1] = n_groups, ranges[n_ranges][2] = 0, ++n_ranges;
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (profile) {
     HIP_TRY(hipEventCreate(&e0));
@@ -1155,23 +1176,29 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
     HIP_TRY(hipEventRecord(e0, st));
   }
   const int n_det = (int)ctx->hdets.size();
+  for (int r = 0; r < n_ranges; ++r) {
+    const int g_begin = ranges[r][0], g_end = ranges[r][1];
+    const bool duty = ranges[r][2] != 0;
+    int* counter = sc.work_counter + r;
+    const int n_work = (g_end - g_begin) * n_chunks;
+    const int grid_r = std::min(grid, n_work);
 #define PES_LNL_ARGS                                                                                         \
   ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_rec,                 \
       ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, sc.pconst, sc.nodes, sc.nodes_grad, NNS, ctx->du, \
-      ctx->modes, P, rows, ctx->d_tile_jr, sc.work_counter, sc.partial
-  const bool duty = ctx->any_duty;
-  if (!grad)
-    HIP_TRY(launch_lnl_fwd(precision, duty, grid, st, PES_LNL_ARGS));
-  else if (precision == 0 && !duty)
-    hipLaunchKernelGGL((k_lnl<0, true, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
-  else if (precision == 0)
-    hipLaunchKernelGGL((k_lnl<0, true, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
-  else if (!duty)
-    hipLaunchKernelGGL((k_lnl<1, true, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
-  else
-    hipLaunchKernelGGL((k_lnl<1, true, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_ARGS);
+      ctx->modes, P, rows, ctx->d_tile_jr, g_begin, g_end, counter, sc.partial
+    if (!grad)
+      HIP_TRY(launch_lnl_fwd(precision, duty, grid_r, st, PES_LNL_ARGS));
+    else if (precision == 0 && !duty)
+      hipLaunchKernelGGL((k_lnl<0, true, false>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
+    else if (precision == 0)
+      hipLaunchKernelGGL((k_lnl<0, true, true>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
+    else if (!duty)
+      hipLaunchKernelGGL((k_lnl<1, true, false>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
+    else
+      hipLaunchKernelGGL((k_lnl<1, true, true>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
 #undef PES_LNL_ARGS
-  HIP_TRY(hipGetLastError());
+    HIP_TRY(hipGetLastError());
+  }
   if (profile) {
     HIP_TRY(hipEventRecord(e1, st));
     ctx->ev_start.push_back(e0);
