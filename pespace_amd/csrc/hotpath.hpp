@@ -513,13 +513,11 @@ __device__ inline void mode_adjoint(const DetectorDev& det, const double* __rest
   double Ab = Gu.re;  // adjoint of the amplitude the response sees (A w with a duty window)
   double tb = fma(kx0b, kx0_t, tgeo);  // adjoint of t = t_f + t_c
   if constexpr (kDuty<KIND>) {
-    // h ~ A w(t): the window's slope moves t, A sees w.  The window is recomputed from t here
-    // and A w is read off h = A w u (|u| = 1), so that only t stays live through the pair
-    // loop; w = 0 only where w' = 0 too (outside the window, or at a ramp's closed end)
+    // h ~ A w(t): the window's slope moves t, A sees w.  The window is recomputed from t here,
+    // so that only t and A stay live through the pair loop
     double w, dw;
     duty_window(det, g.t, w, dw);
-    const double Aw = fma(g.hm.re, g.u.re, g.hm.im * g.u.im);
-    tb = w > 0.0 ? fma(Ab * Aw, dw / w, tb) : tb;
+    tb = fma(Ab * g.A, dw, tb);
     Ab *= w;
   }
   acc.gh_re = fma(Gm.re, g.hm.re, fma(-Gm.im, g.hm.im, acc.gh_re));  // += Re(Gm hm)
@@ -841,8 +839,8 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
 // Launch bounds: the gradient kernel runs two waves per SIMD (256 registers of the unified
 // file per lane), the forward-only kernel three (168 registers, some spilled; measured 9%
 // faster than two for lnL-only batches when three blocks per CU are launched).
-// DUTY: the network has a duty-cycled detector (instantiates the duty-window tile variants;
-// without, the kernel holds only the two orbit kinds' code).
+// DUTY: the launch's tiles belong to duty-cycled detectors (the duty-window tile variants);
+// without, the kernel holds only the two orbit kinds' plain code.
 template <int PREC, bool GRAD, bool DUTY>
 __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* __restrict__ dets, int n_det, int n_tiles,
                                               const double* __restrict__ bin_f, const int* __restrict__ bin_j,
@@ -852,8 +850,8 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
                                               const double* __restrict__ nodes,
                                               const double* __restrict__ nodes_grad, int NNS, double du,
                                               ModeList modes, int P, int rows, const int* __restrict__ tile_jr,
-                                              int g_begin, int g_end, int* __restrict__ work_counter,
-                                              double* __restrict__ partial) {
+                                              int tile_begin, int tile_end, int group_offset,
+                                              int* __restrict__ work_counter, double* __restrict__ partial) {
   constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
   constexpr int CH = chunk_rows<GRAD>();  // 4x the rows without the gradient: 1.3% faster
   __shared__ double acc[TILE / 64][CH][NOUT];
@@ -866,11 +864,13 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
   // there, 0.2% slower with the gradient
   const int lane = threadIdx.x & 63;
   const int wave = GRAD ? (int)(threadIdx.x >> 6) : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  // rows (1 <= rows <= CH) per work item, chosen by the host; this launch covers the tile
-  // groups [g_begin, g_end) (a network with a duty-cycled detector runs that detector's groups
-  // in the DUTY instantiation and the rest in the other, each launch with its own counter)
+  // rows (1 <= rows <= CH) per work item, chosen by the host.  This launch covers the tiles
+  // [tile_begin, tile_end) in groups of tiles_per_group, whose partial sums go to groups
+  // group_offset + 0, 1, ...: a network with duty-cycled detectors runs their tiles in the DUTY
+  // instantiation and the others' in the plain one, one launch (and counter) per tile range
   const int n_chunks = (P + rows - 1) / rows;
-  const int n_work = n_chunks * (g_end - g_begin);
+  const int n_groups_r = (tile_end - tile_begin + tiles_per_group - 1) / tiles_per_group;
+  const int n_work = n_chunks * n_groups_r;
   // work items are taken from a counter (zeroed by k_setup earlier on the same stream), so
   // blocks that draw cheap items (tiles above the mode cuts) take more of them; every block
   // leaves once the counter passes n_work
@@ -879,14 +879,16 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
     __syncthreads();
     const int w = next_work;
     if (w >= n_work) break;
-    const int g = g_begin + w / n_chunks, c = w % n_chunks;
+    const int gl = w / n_chunks, c = w % n_chunks;
+    const int g = group_offset + gl;
     const int p0 = c * rows, p1 = min(P, p0 + rows);
     for (int i = threadIdx.x; i < (TILE / 64) * CH * NOUT; i += TILE) (&acc[0][0][0])[i] = 0.0;
     double chunk_cut = 0.0;
     for (int p = p0; p < p1; ++p) chunk_cut = fmax(chunk_cut, pconst[(size_t)p * PC_STRIDE + PC_FCUT]);
     __syncthreads();
-    const int t_end = min(n_tiles, (g + 1) * tiles_per_group);
-    for (int tile = g * tiles_per_group; tile < t_end; ++tile) {
+    const int t_first = tile_begin + gl * tiles_per_group;
+    const int t_end = min(tile_end, t_first + tiles_per_group);
+    for (int tile = t_first; tile < t_end; ++tile) {
       const double f0 = tile_f0[tile];
       if (f0 > chunk_cut) {  // no row of the chunk has a mode up here
         if (threadIdx.x == 0) {
@@ -900,35 +902,19 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
 #define PES_TILE_ARGS                                                                                         \
   det, tile, f0, p0, p1, bin_f, bin_j, bin_s, bin_rec, tile_dd, pconst, nodes, nodes_grad, NNS, du, \
       modes, lane, wave, &acc[0][0][0], j0, nj, stage_n, &stage_g[0], buf
-      if constexpr (!DUTY) {
-        if (nj <= NJ_STAGE) {
-          if (det.kind == 0)
-            tile_rows<PREC, GRAD, 0, true>(PES_TILE_ARGS);
-          else
-            tile_rows<PREC, GRAD, 1, true>(PES_TILE_ARGS);
-        } else {
-          if (det.kind == 0)
-            tile_rows<PREC, GRAD, 0, false>(PES_TILE_ARGS);
-          else
-            tile_rows<PREC, GRAD, 1, false>(PES_TILE_ARGS);
-        }
+      // the host's tile ranges give this launch only duty-cycled detectors (DUTY) or only
+      // detectors without (kind bit 1 = DUTY)
+      constexpr int KD = DUTY ? 2 : 0;
+      if (nj <= NJ_STAGE) {
+        if (det.kind == 0)
+          tile_rows<PREC, GRAD, 0 | KD, true>(PES_TILE_ARGS);
+        else
+          tile_rows<PREC, GRAD, 1 | KD, true>(PES_TILE_ARGS);
       } else {
-        const int kc = det.kind | (det.duty ? 2 : 0);
-        if (nj <= NJ_STAGE) {
-          switch (kc) {
-            case 0: tile_rows<PREC, GRAD, 0, true>(PES_TILE_ARGS); break;
-            case 1: tile_rows<PREC, GRAD, 1, true>(PES_TILE_ARGS); break;
-            case 2: tile_rows<PREC, GRAD, 2, true>(PES_TILE_ARGS); break;
-            default: tile_rows<PREC, GRAD, 3, true>(PES_TILE_ARGS); break;
-          }
-        } else {
-          switch (kc) {
-            case 0: tile_rows<PREC, GRAD, 0, false>(PES_TILE_ARGS); break;
-            case 1: tile_rows<PREC, GRAD, 1, false>(PES_TILE_ARGS); break;
-            case 2: tile_rows<PREC, GRAD, 2, false>(PES_TILE_ARGS); break;
-            default: tile_rows<PREC, GRAD, 3, false>(PES_TILE_ARGS); break;
-          }
-        }
+        if (det.kind == 0)
+          tile_rows<PREC, GRAD, 0 | KD, false>(PES_TILE_ARGS);
+        else
+          tile_rows<PREC, GRAD, 1 | KD, false>(PES_TILE_ARGS);
       }
 #undef PES_TILE_ARGS
     }

@@ -12,6 +12,7 @@ constexpr int BLOCKS_PER_CU = 4;     // ... and persistent blocks launched per C
 constexpr int MAX_MODES = 6;         // modes per context (ModeList)
 constexpr int NJ_STAGE = 30;         // node rows per tile that the fused kernel stages in LDS
 constexpr int BIN_REC = 10;          // doubles per bin of the fused kernel's weight and data record
+constexpr int MAX_TILE_RANGES = 16;  // fused-kernel launches per batch (runs of detectors by duty flag)
 
 // ---- per-row constant block ------------------------------------------------------------
 enum PConst {

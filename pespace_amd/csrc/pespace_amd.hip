@@ -67,11 +67,8 @@ __device__ inline Cplx mode_factor(int id) {
 __global__ __launch_bounds__(64) void k_setup(const double* __restrict__ params, int P, ModeList modes, double f_ref,
                         double* __restrict__ setup, double* __restrict__ pconst, int* __restrict__ work_counter) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) {  // work queues of the fused kernel launches that follow on this stream
-    work_counter[0] = 0;
-    work_counter[1] = 0;
-    work_counter[2] = 0;
-  }
+  if (p == 0)  // work queues of the fused kernel launches that follow on this stream
+    for (int i = 0; i < MAX_TILE_RANGES; ++i) work_counter[i] = 0;
   if (p >= P) return;
   const double* th = params + (size_t)p * PES_N_PARAMS;
   double Mc = th[0], q = th[1], c1 = th[2], c2 = th[3], dL = th[4], phi = th[5], iota = th[6];
@@ -653,10 +650,10 @@ __global__ void k_whittle_final(const double* __restrict__ work, int nb, double 
       const double *bin_rec, const double *tile_f0,                                                       \
       const double *tile_dd, int tiles_per_group, const double *pconst, const double *nodes,                 \
       const double *nodes_grad, int NNS, double du, ModeList modes, int P, int rows, const int *tile_jr,     \
-      int g_begin, int g_end, int *work_counter, double *partial
+      int tile_begin, int tile_end, int group_offset, int *work_counter, double *partial
 #define PES_LNL_NAMES                                                                                     \
   dets, n_det, n_tiles, bin_f, bin_j, bin_s, bin_rec, tile_f0, tile_dd, tiles_per_group, pconst, \
-      nodes, nodes_grad, NNS, du, modes, P, rows, tile_jr, g_begin, g_end, work_counter, partial
+      nodes, nodes_grad, NNS, du, modes, P, rows, tile_jr, tile_begin, tile_end, group_offset, work_counter, partial
 
 }  // namespace
 
@@ -845,8 +842,34 @@ extern "C" int pes_context_set_duty_cycle(pes_context* ctx, int det_index, doubl
   return PES_OK;
 }
 
+// The fused kernel's launches: maximal runs of consecutive detectors with the same duty flag,
+// each split into groups of tiles_per_group tiles (a group never straddles two runs), with the
+// partial sums of run r's groups at group_offset(r) + 0, 1, ...
+struct TileRange {
+  int tile_begin, tile_end, duty, group_offset;
+};
+static int tile_ranges(const pes_context* ctx, TileRange* out) {
+  int n = 0, groups = 0;
+  for (const auto& h : ctx->hdets) {
+    const int t0 = h.d.bin_offset / TILE, t1 = t0 + h.d.n_tiles, duty = h.d.duty ? 1 : 0;
+    if (n > 0 && out[n - 1].duty == duty && out[n - 1].tile_end == t0) {
+      out[n - 1].tile_end = t1;
+    } else {
+      out[n++] = {t0, t1, duty, 0};
+    }
+  }
+  for (int r = 0; r < n; ++r) {
+    out[r].group_offset = groups;
+    groups += (out[r].tile_end - out[r].tile_begin + ctx->tiles_per_group - 1) / ctx->tiles_per_group;
+  }
+  return n;
+}
+
 static int n_groups_of(const pes_context* ctx) {
-  return (ctx->n_tiles + ctx->tiles_per_group - 1) / ctx->tiles_per_group;
+  TileRange r[MAX_TILE_RANGES];
+  const int n = tile_ranges(ctx, r);
+  return r[n - 1].group_offset +
+         (r[n - 1].tile_end - r[n - 1].tile_begin + ctx->tiles_per_group - 1) / ctx->tiles_per_group;
 }
 
 // ---- the entry point that allocates nothing ---------------------------------------------------
@@ -880,7 +903,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static size_t noalloc_layout(const pes_context* ctx, int P, bool grad, size_t off[6]) {
   const size_t NNS = (size_t)ctx->n_nodes + 1, rows = (size_t)P, nm = (size_t)ctx->modes.n;
   size_t o = 0;
-  off[0] = o; o += align256(3 * sizeof(int));                              // work counters
+  off[0] = o; o += align256(MAX_TILE_RANGES * sizeof(int));                // work counters
   off[1] = o; o += align256(rows * N_SETUP * 5 * sizeof(double));          // setup
   off[2] = o; o += align256(rows * PC_STRIDE * sizeof(double));            // per-row constants
   off[3] = o; o += align256(rows * nm * NNS * 4 * sizeof(double));         // node tables
@@ -897,6 +920,14 @@ extern "C" int pes_context_finalize(pes_context* ctx) {
   if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
   if (ctx->finalized) return fail(PES_ERR_STATE, "context already finalized");
   if (ctx->hdets.empty()) return fail(PES_ERR_STATE, "no detector added");
+  {  // runs of consecutive detectors with the same duty flag: one fused-kernel launch each
+    int runs = 0;
+    for (size_t i = 0; i < ctx->hdets.size(); ++i)
+      runs += (i == 0 || (ctx->hdets[i].d.duty != 0) != (ctx->hdets[i - 1].d.duty != 0)) ? 1 : 0;
+    if (runs > MAX_TILE_RANGES)
+      return fail(PES_ERR_INVALID, "too many alternations of duty-cycled and continuous detectors (order the "
+                                   "detectors so that at most 16 runs of either remain)");
+  }
   HIP_TRY(hipSetDevice(ctx->device));
   long long total = 0;
   for (auto& h : ctx->hdets) total += (long long)(h.d.n_bins + TILE - 1) / TILE * TILE;
@@ -1012,7 +1043,7 @@ extern "C" int pes_context_finalize(pes_context* ctx) {
     HIP_TRY(hipMemcpy(ctx->d_tile_jr, tjr.data(), 2 * (size_t)nt * sizeof(int), hipMemcpyHostToDevice));
   }
   if (ctx->direct) ctx->d_node_f = ctx->d_bin_f;
-  HIP_TRY(hipMalloc(&ctx->d_work_counter, 3 * sizeof(int)));  // one per fused-kernel launch of a batch
+  HIP_TRY(hipMalloc(&ctx->d_work_counter, MAX_TILE_RANGES * sizeof(int)));  // one per fused-kernel launch
   {  // the device address of the no-allocation entry point's scratch, once per device
     char* base = nullptr;
     const int rc = noalloc_base(ctx->device, &base);
@@ -1150,25 +1181,11 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
   const int rows = rows_per_item(ctx, P, grad, grid);
   const int n_chunks = (P + rows - 1) / rows;
   const int nout = grad ? 1 + PES_N_PARAMS : 1;
-  // tile-group ranges and the kernel variant for each: the groups holding a duty-cycled
-  // detector's tiles run the DUTY instantiation, the others the plain one (its register
-  // allocation is the one without the window code)
-  int ranges[3][3];  // g_begin, g_end, duty
-  int n_ranges = 0;
-  if (!ctx->any_duty) {
-    ranges[n_ranges][0] = 0, ranges[n_ranges][1] = n_groups, ranges[n_ranges][2] = 0, ++n_ranges;
-  } else {
-    int gd0 = n_groups, gd1 = 0;
-    for (const auto& h : ctx->hdets) {
-      if (!h.d.duty) continue;
-      const int t0 = h.d.bin_offset / TILE, t1 = t0 + h.d.n_tiles;
-      gd0 = std::min(gd0, t0 / ctx->tiles_per_group);
-      gd1 = std::max(gd1, (t1 + ctx->tiles_per_group - 1) / ctx->tiles_per_group);
-    }
-    if (gd0 > 0) ranges[n_ranges][0] = 0, ranges[n_ranges][1] = gd0, ranges[n_ranges][2] = 0, ++n_ranges;
-    ranges[n_ranges][0] = gd0, ranges[n_ranges][1] = gd1, ranges[n_ranges][2] = 1, ++n_ranges;
-    if (gd1 < n_groups) ranges[n_ranges][0] = gd1, ranges[n_ranges][1] = n_groups, ranges[n_ranges][2] = 0, ++n_ranges;
-  }
+  // one launch per run of detectors with the same duty flag: the duty-cycled detectors' tiles
+  // run the DUTY instantiation, the others the plain one (whose register allocation is that of
+  // a kernel without the window code)
+  TileRange ranges[MAX_TILE_RANGES];
+  const int n_ranges = tile_ranges(ctx, ranges);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (profile) {
     HIP_TRY(hipEventCreate(&e0));
@@ -1177,15 +1194,16 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
   }
   const int n_det = (int)ctx->hdets.size();
   for (int r = 0; r < n_ranges; ++r) {
-    const int g_begin = ranges[r][0], g_end = ranges[r][1];
-    const bool duty = ranges[r][2] != 0;
+    const int tile_begin = ranges[r].tile_begin, tile_end = ranges[r].tile_end;
+    const int group_offset = ranges[r].group_offset;
+    const bool duty = ranges[r].duty != 0;
     int* counter = sc.work_counter + r;
-    const int n_work = (g_end - g_begin) * n_chunks;
+    const int n_work = (tile_end - tile_begin + ctx->tiles_per_group - 1) / ctx->tiles_per_group * n_chunks;
     const int grid_r = std::min(grid, n_work);
 #define PES_LNL_ARGS                                                                                         \
   ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_rec,                 \
       ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, sc.pconst, sc.nodes, sc.nodes_grad, NNS, ctx->du, \
-      ctx->modes, P, rows, ctx->d_tile_jr, g_begin, g_end, counter, sc.partial
+      ctx->modes, P, rows, ctx->d_tile_jr, tile_begin, tile_end, group_offset, counter, sc.partial
     if (!grad)
       HIP_TRY(launch_lnl_fwd(precision, duty, grid_r, st, PES_LNL_ARGS));
     else if (precision == 0 && !duty)
