@@ -25,6 +25,14 @@ particle still building, so one leapfrog step is one batched density-and-gradien
 of the whole cloud (finished particles are carried along and masked), as jax's vmap of the
 per-particle kernel does.  The Hessians come from central differences of the exact adjoint
 gradient in z (2 dim evaluations per particle in one batch).
+
+A level's random input (resampling offset, momenta, directions, leaf and merge uniforms) is
+drawn up front (``draw``).  With ``capturable=True`` the level is a fixed sequence of device
+operations: the metrics' eigen-decompositions by Jacobi rotations instead of the solver
+library, and trajectories of fixed length (every doubling and leaf executed, finished
+particles masked, no host synchronisation) -- the same transitions as the early-exiting
+default on the same random input, at 2^max_num_doublings - 1 evaluations per particle -- so
+that the whole level, likelihood launches included, can be captured in a HIP graph.
 """
 
 from __future__ import annotations
@@ -88,15 +96,60 @@ class Unconstrained:
 # SoftAbs metric
 # ------------------------------------------------------------------------------------------
 
-def softabs_inverse_metric(neg_hessian, alpha):
-    """inv(V diag(l / tanh(alpha l)) V^T) = V diag(tanh(alpha l) / l) V^T for a batch of
-    symmetric matrices (l -> 0: the finite limit alpha)."""
+def jacobi_eigh(h, sweeps=12):
+    """Eigenvalues and eigenvectors of a batch of small symmetric matrices [B, d, d] by cyclic
+    Jacobi rotations (Press et al., Numerical Recipes, sec. 11.1: t = sgn(theta) / (|theta| +
+    sqrt(theta^2 + 1)), theta = (a_qq - a_pp) / (2 a_pq)), a fixed number of sweeps over all
+    pairs (p, q) written in elementwise device operations: no solver library call and no host
+    synchronisation, so it can sit inside a captured graph.  Twelve sweeps take d = 11 to
+    working precision (Jacobi converges quadratically once the off-diagonal part is small).
+    Returns (lam [B, d], V [B, d, d]) with h = V diag(lam) V^T (unsorted)."""
+    A = 0.5 * (h + h.transpose(-1, -2))
+    d = A.shape[-1]
+    V = torch.eye(d, dtype=A.dtype, device=A.device).expand(A.shape).clone()
+    for _ in range(sweeps):
+        for p in range(d - 1):
+            for q in range(p + 1, d):
+                apq = A[:, p, q]
+                zero = apq == 0.0
+                theta = (A[:, q, q] - A[:, p, p]) / (2.0 * torch.where(zero, torch.ones_like(apq), apq))
+                t = torch.sign(theta) / (torch.abs(theta) + torch.sqrt(theta * theta + 1.0))
+                t = torch.where(zero | ~torch.isfinite(t), torch.zeros_like(t), t)
+                t = torch.where((theta == 0.0) & ~zero, torch.ones_like(t), t)  # sgn(0) = +1
+                c = 1.0 / torch.sqrt(t * t + 1.0)
+                sn = t * c
+                c_, s_ = c[:, None], sn[:, None]
+                colp, colq = A[:, :, p].clone(), A[:, :, q]
+                A[:, :, p] = c_ * colp - s_ * colq
+                A[:, :, q] = s_ * colp + c_ * colq
+                rowp, rowq = A[:, p, :].clone(), A[:, q, :]
+                A[:, p, :] = c_ * rowp - s_ * rowq
+                A[:, q, :] = s_ * rowp + c_ * rowq
+                vp, vq = V[:, :, p].clone(), V[:, :, q]
+                V[:, :, p] = c_ * vp - s_ * vq
+                V[:, :, q] = s_ * vp + c_ * vq
+    return torch.diagonal(A, dim1=-2, dim2=-1).clone(), V
+
+
+def softabs_factors(neg_hessian, alpha, capturable=False):
+    """SoftAbs metric of a batch of symmetric matrices: with eigenvalues l and vectors V,
+    M = V diag(l / tanh(alpha l)) V^T (l -> 0: the finite limit 1 / alpha).  Returns
+    (inv(M) = V diag(tanh(alpha l) / l) V^T, F = V diag(sqrt(l / tanh(alpha l)))), F F^T = M:
+    the kinetic energy's matrix and a factor that draws momenta from N(0, M).  capturable:
+    the eigen-decomposition by Jacobi rotations (jacobi_eigh) instead of the solver library."""
     h = 0.5 * (neg_hessian + neg_hessian.transpose(-1, -2))
-    lam, V = torch.linalg.eigh(h)
+    lam, V = jacobi_eigh(h) if capturable else torch.linalg.eigh(h)
     al = alpha * lam
     small = torch.abs(al) < 1e-8
     inv_soft = torch.where(small, torch.full_like(lam, alpha), torch.tanh(al) / torch.where(small, 1.0, lam))
-    return (V * inv_soft[..., None, :]) @ V.transpose(-1, -2)
+    inv_m = (V * inv_soft[..., None, :]) @ V.transpose(-1, -2)
+    return inv_m, V * torch.rsqrt(inv_soft)[..., None, :]
+
+
+def softabs_inverse_metric(neg_hessian, alpha):
+    """inv(V diag(l / tanh(alpha l)) V^T) = V diag(tanh(alpha l) / l) V^T for a batch of
+    symmetric matrices (l -> 0: the finite limit alpha)."""
+    return softabs_factors(neg_hessian, alpha)[0]
 
 
 def _bmv(A, v):
@@ -116,19 +169,37 @@ def _logaddexp(a, b):
 # NUTS transition (batched, lockstep)
 # ------------------------------------------------------------------------------------------
 
-def nuts_step(logp_grad, z, logp, grad, inv_mass, step_size, gen, max_num_doublings=7, divergence_threshold=100.0):
+def nuts_draw(P, d, max_num_doublings, gen, device, dtype=torch.float64):
+    """The random input of one transition, drawn up front: momentum normals [P, d], the
+    direction of each doubling [max, P], the uniform of every leaf [2^max - 1, P] (leaf n of
+    doubling depth is row 2^depth - 1 + n) and of every subtree merge [max, P]."""
+    return {"p": torch.randn((P, d), generator=gen, device=device, dtype=dtype),
+            "dir": torch.rand((max_num_doublings, P), generator=gen, device=device, dtype=dtype),
+            "leaf": torch.rand((2 ** max_num_doublings - 1, P), generator=gen, device=device, dtype=dtype),
+            "merge": torch.rand((max_num_doublings, P), generator=gen, device=device, dtype=dtype)}
+
+
+def nuts_step(logp_grad, z, logp, grad, inv_mass, step_size, gen, max_num_doublings=7, divergence_threshold=100.0,
+              mass_factor=None, rand=None, fixed_length=False):
     """One NUTS transition for every particle.
 
     logp_grad(z) -> (log density [P], gradient [P, d]); inv_mass [P, d, d] (the kinetic
-    energy's matrix A).  Returns (z, logp, grad, info) with info = dict(acceptance_rate,
-    num_integration_steps, is_divergent) per particle.
+    energy's matrix A); mass_factor [P, d, d]: F with F F^T = inv(A) (by default from a
+    Cholesky factorisation of the inverse); rand: the transition's random input (nuts_draw;
+    by default drawn from gen as the trajectory grows).  fixed_length: every doubling and every
+    leaf is executed, finished trajectories under masks, with no host synchronisation (the
+    same transition as the default's early exits, for a captured graph: 2^max - 1 likelihood
+    evaluations per particle).  Returns (z, logp, grad, info) with info =
+    dict(acceptance_rate, num_integration_steps, is_divergent) per particle.
     """
     P, d = z.shape
     dev, dt = z.device, z.dtype
-    mass = torch.linalg.inv(inv_mass)
-    mass = 0.5 * (mass + mass.transpose(-1, -2))
-    Lm = torch.linalg.cholesky(mass)
-    p0 = _bmv(Lm, torch.randn((P, d), generator=gen, device=dev, dtype=dt))
+    if mass_factor is None:
+        mass = torch.linalg.inv(inv_mass)
+        mass = 0.5 * (mass + mass.transpose(-1, -2))
+        mass_factor = torch.linalg.cholesky(mass)
+    normals = rand["p"] if rand is not None else torch.randn((P, d), generator=gen, device=dev, dtype=dt)
+    p0 = _bmv(mass_factor, normals)
     kin = lambda p: 0.5 * (p * _bmv(inv_mass, p)).sum(1)  # noqa: E731
     H0 = -logp + kin(p0)
 
@@ -146,9 +217,9 @@ def nuts_step(logp_grad, z, logp, grad, inv_mass, step_size, gen, max_num_doubli
     ck_s = torch.zeros_like(ck_p)
 
     for depth in range(max_num_doublings):
-        if bool(done.all()):
+        if not fixed_length and bool(done.all()):
             break
-        go_right = torch.rand(P, generator=gen, device=dev) < 0.5
+        go_right = (rand["dir"][depth] if rand is not None else torch.rand(P, generator=gen, device=dev)) < 0.5
         eps = torch.where(go_right, step_size, -step_size).to(dt)[:, None]
         q = torch.where(go_right[:, None], qr, ql)
         p = torch.where(go_right[:, None], pr, pl)
@@ -162,7 +233,7 @@ def nuts_step(logp_grad, z, logp, grad, inv_mass, step_size, gen, max_num_doubli
         last_q, last_p, last_g, last_lp = q, p, g, torch.zeros(P, dtype=dt, device=dev)
         for n in range(2 ** depth):
             act = building & ~s_turn & ~s_div
-            if not bool(act.any()):
+            if not fixed_length and not bool(act.any()):
                 break
             ph = p + 0.5 * eps * g
             qn = q + eps * _bmv(inv_mass, ph)
@@ -178,7 +249,8 @@ def nuts_step(logp_grad, z, logp, grad, inv_mass, step_size, gen, max_num_doubli
             acc_sum = acc_sum + torch.where(act, torch.clamp(torch.exp(-dH), max=1.0), torch.zeros_like(dH))
             # uniform progressive sampling inside the subtree
             new_s_log_w = torch.where(act, _logaddexp(s_log_w, lw_leaf), s_log_w)
-            u = torch.rand(P, generator=gen, device=dev, dtype=dt)
+            u = rand["leaf"][2 ** depth - 1 + n] if rand is not None else torch.rand(P, generator=gen, device=dev,
+                                                                                   dtype=dt)
             take = act & (torch.log(u) < lw_leaf - new_s_log_w)
             s_q = torch.where(take[:, None], qn, s_q)
             s_lp = torch.where(take, lpn, s_lp)
@@ -207,7 +279,7 @@ def nuts_step(logp_grad, z, logp, grad, inv_mass, step_size, gen, max_num_doubli
         # merge the subtree where it neither turned nor diverged
         ok = building & ~s_turn & ~s_div
         diverged = diverged | (building & s_div)
-        u = torch.rand(P, generator=gen, device=dev, dtype=dt)
+        u = rand["merge"][depth] if rand is not None else torch.rand(P, generator=gen, device=dev, dtype=dt)
         take = ok & (torch.log(u) < s_log_w - log_w)
         qprop = torch.where(take[:, None], s_q, qprop)
         lpprop = torch.where(take, s_lp, lpprop)
@@ -243,7 +315,7 @@ class TemperedSMCNUTS(TemperedSMC):
 
     def __init__(self, likelihood, prior_dict, num_particles=4096, target_ess=0.9, num_mcmc_steps=1,
                  step_size=0.2, softabs_alpha=0.3, max_num_doublings=7, divergence_threshold=100,
-                 hessian_step=1e-4, rkey_seed=0, device=None, names=None):
+                 hessian_step=1e-4, rkey_seed=0, device=None, names=None, capturable=False):
         super().__init__(likelihood, prior_dict, num_particles=num_particles, target_ess=target_ess,
                          num_mcmc_steps=num_mcmc_steps, rkey_seed=rkey_seed, device=device, names=names)
         if step_size <= 0.0:
@@ -255,6 +327,11 @@ class TemperedSMCNUTS(TemperedSMC):
         self.hessian_step = float(hessian_step)
         self.bij = Unconstrained(self.prior)
         self.n_grad_evals = 0
+        # capturable: a level is a fixed sequence of device operations (Jacobi eigen-
+        # decompositions for the metrics, fixed-length trajectories), so that it can be captured
+        # once in a HIP graph and replayed; each transition then costs 2^max_num_doublings - 1
+        # likelihood and gradient evaluations per particle
+        self.capturable = bool(capturable)
 
     # log prior (with the Jacobian) and log likelihood with their z-gradients
     def _prior_and_like(self, z):
@@ -291,6 +368,10 @@ class TemperedSMCNUTS(TemperedSMC):
     def inverse_mass_matrices(self, z, beta):
         """inv(SoftAbs(-H)) of the tempered log-posterior at every particle (central
         differences of the exact gradient, all 2 d shifted points in one batch)."""
+        return self._metrics(z, beta)[0]
+
+    def _metrics(self, z, beta):
+        """(inv(SoftAbs(-H)), a factor F of SoftAbs(-H) = F F^T) at every particle."""
         P, d = z.shape
         h = self.hessian_step
         shifts = torch.eye(d, dtype=z.dtype, device=z.device) * h
@@ -299,28 +380,34 @@ class TemperedSMCNUTS(TemperedSMC):
         gp, gm = g[:P * d].reshape(P, d, d), g[P * d:].reshape(P, d, d)
         H = (gp - gm) / (2.0 * h)  # H[p, k, :] = d grad / d z_k
         H = torch.nan_to_num(H, nan=0.0, posinf=0.0, neginf=0.0)
-        return softabs_inverse_metric(-H, self.softabs_alpha)
+        return softabs_factors(-H, self.softabs_alpha, capturable=self.capturable)
 
     def level(self, state, rand=None):
         z, logl, logprior, beta = state["x"], state["logl"], state["logprior"], state["beta"]
-        u = (rand or self.draw())["u"]
+        rand = rand if rand is not None else self.draw()
+        u = rand["u"]
         delta = tempering_increment(logl, beta, self.target_ess)
         beta_new = beta + delta
         lw = torch.nan_to_num(delta * logl)
         state["log_evidence"].add_(torch.logsumexp(lw, 0) - math.log(self.num_particles))
         idx = systematic_resample(lw, u)
         z = z[idx]
-        inv_mass = self.inverse_mass_matrices(z, beta_new)
+        inv_mass, mass_factor = self._metrics(z, beta_new)
         f = self._logpost_grad(beta_new)
         logp, grad = f(z)
         acc = torch.zeros((), dtype=torch.float64, device=self.device)
         self.last_info = []
-        for _ in range(self.num_mcmc_steps):
+        for k in range(self.num_mcmc_steps):
+            tr = {key: rand[key][k] for key in ("p", "dir", "leaf", "merge")}
             z, logp, grad, info = nuts_step(f, z, logp, grad, inv_mass, self.step_size, self.gen,
-                                            self.max_num_doublings, self.divergence_threshold)
+                                            self.max_num_doublings, self.divergence_threshold,
+                                            mass_factor=mass_factor, rand=tr, fixed_length=self.capturable)
             acc = acc + info["acceptance_rate"].mean()
             self.last_info.append(info)
-            self.n_like_evals += int(info["num_integration_steps"].sum())
+            if self.capturable:  # every leaf of every doubling is evaluated
+                self.n_like_evals += self.num_particles * (2 ** self.max_num_doublings - 1)
+            else:
+                self.n_like_evals += int(info["num_integration_steps"].sum())
         lp, ll, _, _ = self._prior_and_like(z)
         state["x"].copy_(z)
         state["logl"].copy_(ll)
@@ -331,9 +418,13 @@ class TemperedSMCNUTS(TemperedSMC):
         return state
 
     def draw(self):
-        """Random input of one level that the caller may fix: the resampling offset (the
-        momenta and the trajectory choices come from the sampler's generator)."""
-        return {"u": torch.rand((), generator=self.gen, device=self.device, dtype=torch.float64)}
+        """Random input of one level, drawn up front: the resampling offset and, per
+        transition (leading axis num_mcmc_steps), the momenta's normals and the trajectory's
+        uniforms (nuts_draw)."""
+        u = torch.rand((), generator=self.gen, device=self.device, dtype=torch.float64)
+        tr = [nuts_draw(self.num_particles, self.prior.dim, self.max_num_doublings, self.gen, self.device)
+              for _ in range(self.num_mcmc_steps)]
+        return {"u": u, **{key: torch.stack([t[key] for t in tr]) for key in ("p", "dir", "leaf", "merge")}}
 
     def _physical(self, z):
         return self.bij.to_x(z)

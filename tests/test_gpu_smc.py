@@ -136,3 +136,50 @@ def test_smc_with_nuts_mutation_on_hip_likelihood():
         assert 0.2 < float(state["accept"]) <= 1.0
         assert torch.all(torch.isfinite(state["logl"]))
     assert betas[1] > betas[0] > 0.0
+
+
+def test_nuts_level_hip_graph_replays_match_eager():
+    """A capturable NUTS level (Jacobi metrics, fixed-length trajectories; the metric batch of
+    2 d rows per particle and every leapfrog step on the fused lnL + gradient kernel, the
+    z-gradients through torch autograd) captured once and replayed: equal to the eager level
+    on the same random input, replay after replay."""
+    import torch
+
+    from pespace_amd.inference.nuts import TemperedSMCNUTS
+    lk = _likelihood()
+    s = TemperedSMCNUTS(lk, _prior(), num_particles=64, target_ess=0.9, num_mcmc_steps=1, max_num_doublings=3,
+                        rkey_seed=4, capturable=True)
+    state = s.init_state()
+    print("nuts capture: eager level", flush=True)
+    s.level(state, s.draw())  # sizes every buffer (the metric batch is the largest), moves beta off 0
+    lk.core.pin_scratch(True)
+    static_state = _clone(state)
+    static_rand = {k: v.clone() for k, v in s.draw().items()}
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):  # warm-up on the capture stream
+        s.level(_clone(state), static_rand)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    print("nuts capture: capturing", flush=True)
+    with torch.cuda.graph(graph):
+        s.level(static_state, static_rand)
+    for rep in range(3):
+        start = _clone(state)
+        r = s.draw()
+        for k in static_state:
+            static_state[k].copy_(start[k])
+        for k in static_rand:
+            static_rand[k].copy_(r[k])
+        graph.replay()
+        torch.cuda.synchronize()
+        eager = _clone(start)
+        s.level(eager, r)
+        torch.cuda.synchronize()
+        for k in ("x", "logl", "logprior", "beta", "log_evidence", "accept", "delta"):
+            assert torch.equal(static_state[k], eager[k]), (rep, k)
+        assert torch.all(torch.isfinite(eager["logl"]))
+        state = eager
+    lk.core.pin_scratch(False)

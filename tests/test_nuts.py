@@ -111,3 +111,60 @@ def test_smc_with_nuts_evidence_and_posterior():
     np.testing.assert_allclose(post.mean(0), mean, atol=0.03)
     np.testing.assert_allclose(np.cov(post.T), cov, rtol=0.15, atol=3e-3)
     assert all(lv["acceptance"] > 0.3 for lv in out["levels"])
+
+
+def test_jacobi_eigh_matches_the_solver_library():
+    rng = np.random.default_rng(9)
+    B, d = 6, 11
+    A = rng.normal(size=(B, d, d))
+    A = A + A.transpose(0, 2, 1)
+    A[0] = np.diag(np.arange(1.0, d + 1.0))                     # already diagonal
+    Q, _ = np.linalg.qr(rng.normal(size=(d, d)))
+    A[1] = (Q * np.array([5.0, 5.0, 5.0] + list(range(8)))) @ Q.T  # repeated eigenvalues
+    A[2] = (Q * np.logspace(-8, 8, d)) @ Q.T                    # wide dynamic range
+    At = torch.as_tensor(A)
+    lam, V = nuts.jacobi_eigh(At)
+    ref = np.linalg.eigvalsh(A)
+    for b in range(B):
+        np.testing.assert_allclose(np.sort(lam[b].numpy()), ref[b], rtol=1e-12, atol=1e-12 * np.abs(ref[b]).max())
+    np.testing.assert_allclose(((V * lam[:, None, :]) @ V.transpose(1, 2)).numpy(), A, rtol=0,
+                               atol=1e-12 * np.abs(A).max())
+    np.testing.assert_allclose((V.transpose(1, 2) @ V).numpy(), np.broadcast_to(np.eye(d), (B, d, d)), atol=1e-13)
+
+
+def test_softabs_factors_capturable_and_library_agree():
+    rng = np.random.default_rng(10)
+    d = 7
+    H = rng.normal(size=(4, d, d)) * 100.0
+    H = torch.as_tensor(H + H.transpose(0, 2, 1))
+    a1, f1 = nuts.softabs_factors(H, 0.3)
+    a2, f2 = nuts.softabs_factors(H, 0.3, capturable=True)
+    np.testing.assert_allclose(a2.numpy(), a1.numpy(), rtol=0, atol=1e-12 * float(a1.abs().max()))
+    M = torch.linalg.inv(a1)
+    for f in (f1, f2):  # F F^T = SoftAbs = inv(A)
+        np.testing.assert_allclose((f @ f.transpose(1, 2)).numpy(), M.numpy(), rtol=0, atol=1e-10 * float(M.abs().max()))
+
+
+def test_fixed_length_transition_equals_the_early_exit_one():
+    """On the same random input the fixed-length (capturable) transition, which runs every
+    doubling and leaf with finished trajectories masked, gives bit for bit the early-exiting
+    transition's result."""
+    rng = np.random.default_rng(11)
+    d, P = 4, 64
+    Q, _ = np.linalg.qr(rng.normal(size=(d, d)))
+    cov = (Q * np.array([1.0, 0.3, 0.05, 2.0])) @ Q.T
+    f = _gauss(np.zeros(d), cov)
+    gen = torch.Generator().manual_seed(4)
+    z = torch.as_tensor(rng.normal(size=(P, d)))
+    logp, grad = f(z)
+    inv_mass = torch.as_tensor(np.broadcast_to(cov, (P, d, d)).copy())
+    _, F = nuts.softabs_factors(-torch.linalg.inv(inv_mass), 1e6)  # ~ |H|: F F^T = inv(cov)
+    inv_mass_s = torch.linalg.inv(F @ F.transpose(1, 2))
+    rand = nuts.nuts_draw(P, d, 6, gen, "cpu")
+    a = nuts.nuts_step(f, z, logp, grad, inv_mass_s, 0.4, gen, 6, mass_factor=F, rand=rand)
+    b = nuts.nuts_step(f, z, logp, grad, inv_mass_s, 0.4, gen, 6, mass_factor=F, rand=rand, fixed_length=True)
+    for x, y in zip(a[:3], b[:3]):
+        assert torch.equal(x, y)
+    for k in a[3]:
+        assert torch.equal(a[3][k], b[3][k]), k
+    assert int(a[3]["num_integration_steps"].max()) < 2 ** 6 - 1  # some trajectories ended early
