@@ -328,7 +328,8 @@ def main():
     ap.add_argument("--no-prior-sweep", action="store_true",
                     help="skip the mixed vs FP64 sweep over uneven wide-prior batches (256, 3001, 8192 rows)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule of the fused kernel")
-    ap.add_argument("--schedule", default=None, help="fixed schedule 'tiles_per_group,blocks_per_cu' (no autotune)")
+    ap.add_argument("--schedule", default=None,
+                    help="fixed schedule 'tiles_per_group,blocks_per_cu[,rows_per_item]' (no autotune)")
     ap.add_argument("--no-td", action="store_true", help="skip the time-domain links/TDI timing")
     ap.add_argument("--no-duty", action="store_true", help="network workloads without TianQin's duty cycle")
     ap.add_argument("--td-only", action="store_true", help="only time the time-domain links and TDI")
@@ -366,17 +367,21 @@ def main():
 
     tune = None
     if args.schedule:
-        t, b = (int(v) for v in args.schedule.split(","))
-        core.set_schedule(t, b)
-        tune = {"tiles_per_group": t, "blocks_per_cu": b, "fixed": True}
+        vals = [int(v) for v in args.schedule.split(",")]
+        t, b, r = vals[0], vals[1], (vals[2] if len(vals) > 2 else 0)
+        core.set_schedule(t, b, r)
+        tune = {"tiles_per_group": t, "blocks_per_cu": b, "rows_per_item": r or "chunk", "fixed": True}
     elif not args.no_autotune:
         # schedule chosen by the autotuner on a 512-row sample of the batch (untimed)
         t0 = time.perf_counter()
+        # rows per work item: 0 = the kernel's chunk (40 with the gradient, 160 without), or fixed
+        rows_c = (0, 20, 10) if grad else (0, 80, 40)
         best, times = core.autotune(params[:min(512, args.particles)], grad=grad,
                                     precision="fp64" if args.precision == "all" else args.precision,
-                                    candidates=[(t, b) for t in (8, 16, 32, 64) for b in (2, 3, 4)], reps=1)
-        tune = {"tiles_per_group": best[0], "blocks_per_cu": best[1],
-                "candidates_ms": {f"{t}x{b}": round(ms, 2) for (t, b), ms in times.items()}}
+                                    candidates=[(t, b, r) for t in (8, 16, 32) for b in (2, 3, 4) for r in rows_c],
+                                    reps=1)
+        tune = {"tiles_per_group": best[0], "blocks_per_cu": best[1], "rows_per_item": best[2] or "chunk",
+                "candidates_ms": {f"{t}x{b}x{r or 'chunk'}": round(ms, 2) for (t, b, r), ms in times.items()}}
         print(f"[bench] autotune {time.perf_counter() - t0:.1f}s: best {best}; "
               + ", ".join(f"{k}: {v}" for k, v in tune["candidates_ms"].items()), file=sys.stderr, flush=True)
 

@@ -223,6 +223,14 @@ int pes_whittle_lnl(const double* d_data, const double* d_resp, const double* d_
 int pes_context_set_schedule(pes_context* ctx, int tiles_per_group, int blocks_per_cu);
 int pes_context_get_schedule(const pes_context* ctx, int* tiles_per_group, int* blocks_per_cu);
 
+/* Parameter rows per work item of the fused kernel (the rows a block evaluates over one tile
+ * group): 0 (the default) picks them per batch (the kernel's chunk, 40 with the gradient and 160
+ * without, fewer when the grid has too few tile groups); 1..160 fixes them (capped at the
+ * kernel's chunk).  A tunable of the project's autotuner, beside the tile-group size and the
+ * blocks per CU; no counterpart in the reference. */
+int pes_context_set_rows_per_item(pes_context* ctx, int rows);
+int pes_context_get_rows_per_item(const pes_context* ctx, int* rows);
+
 /* Timing of the fused lnL kernel: when enabled, pes_lnl_batch records a HIP event
  * pair around the fused kernel on its stream.  pes_profile_read waits for the
  * recorded pairs, returns their count and summed duration (ms) and clears them.

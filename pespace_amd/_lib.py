@@ -64,6 +64,8 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
       ctypes.c_void_p]),
     ("pes_lnl_batch_noalloc_bytes", ctypes.c_size_t, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("pes_context_set_rows_per_item", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("pes_context_get_rows_per_item", ctypes.c_int, [ctypes.c_void_p, c_int_p]),
     ("pes_context_set_duty_cycle", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     ("pes_td_links", ctypes.c_int,

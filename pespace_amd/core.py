@@ -183,25 +183,38 @@ class LikelihoodCore:
         check(lib().pes_context_pin_scratch(self._ctx, int(bool(pinned))))
 
     # -- schedule of the fused kernel ------------------------------------------------------
-    def set_schedule(self, tiles_per_group, blocks_per_cu):
-        """Work items of ``tiles_per_group`` 256-bin tiles (x 40 rows, 160 without gradient),
-        ``blocks_per_cu`` persistent blocks per CU."""
+    def set_schedule(self, tiles_per_group, blocks_per_cu, rows_per_item=None):
+        """Work items of ``tiles_per_group`` 256-bin tiles x ``rows_per_item`` parameter rows
+        (0: chosen per batch, 40 with the gradient and 160 without unless the grid is small;
+        None: unchanged), ``blocks_per_cu`` persistent blocks per CU."""
         self._build()
         check(lib().pes_context_set_schedule(self._ctx, int(tiles_per_group), int(blocks_per_cu)))
+        if rows_per_item is not None:
+            check(lib().pes_context_set_rows_per_item(self._ctx, int(rows_per_item)))
 
     def schedule(self):
+        """(tiles_per_group, blocks_per_cu)."""
         self._build()
         t, b = ctypes.c_int(), ctypes.c_int()
         check(lib().pes_context_get_schedule(self._ctx, ctypes.byref(t), ctypes.byref(b)))
         return t.value, b.value
 
+    def rows_per_item(self):
+        """The fixed rows per work item, or 0 when they are chosen per batch."""
+        self._build()
+        r = ctypes.c_int()
+        check(lib().pes_context_get_rows_per_item(self._ctx, ctypes.byref(r)))
+        return r.value
+
     def autotune(self, params, grad=True, precision="fp64", candidates=None, reps=2, warmup=1):
         """Time the fused kernel on ``params`` for each candidate (tiles_per_group, blocks_per_cu)
-        with HIP events, keep the fastest, and return it with every candidate's mean time (ms).
+        or (tiles_per_group, blocks_per_cu, rows_per_item) with HIP events, keep the fastest,
+        and return it with every candidate's mean time (ms).
 
         The default candidates cover work items of 8-64 tiles and 2-8 persistent blocks per CU
         (the gradient kernel holds two blocks of four waves per CU, the lnL-only kernel three;
-        blocks beyond those wait and find the work queue drained).
+        blocks beyond those wait and find the work queue drained), with the rows per item chosen
+        per batch.
         """
         self._build()
         p = self._params(params)
@@ -210,8 +223,9 @@ class LikelihoodCore:
         lnl = torch.empty(P, dtype=torch.float64, device=p.device)
         g = torch.empty(P, N_PARAMS, dtype=torch.float64, device=p.device) if grad else None
         times = {}
-        for t, b in cands:
-            self.set_schedule(t, b)
+        for cand in cands:
+            t, b = cand[0], cand[1]
+            self.set_schedule(t, b, cand[2] if len(cand) > 2 else 0)
             for _ in range(warmup):
                 self.lnl(p, grad=grad, precision=precision, out=lnl, grad_out=g)
             self.set_profiling(True)
@@ -219,9 +233,9 @@ class LikelihoodCore:
                 self.lnl(p, grad=grad, precision=precision, out=lnl, grad_out=g)
             n, ms = self.read_profile()
             self.set_profiling(False)
-            times[(t, b)] = ms / max(n, 1)
+            times[tuple(cand)] = ms / max(n, 1)
         best = min(times, key=times.get)
-        self.set_schedule(*best)
+        self.set_schedule(best[0], best[1], best[2] if len(best) > 2 else 0)
         return best, times
 
     def set_profiling(self, enable=True):

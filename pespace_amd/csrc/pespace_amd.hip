@@ -709,6 +709,7 @@ struct pes_context {
   // schedule of the fused kernel (pes_context_set_schedule)
   int tiles_per_group = TILES_PER_GROUP;
   int blocks_per_cu = BLOCKS_PER_CU;
+  int rows_override = 0;  // parameter rows per work item (0: chosen per batch)
   double *d_setup = nullptr, *d_pconst = nullptr, *d_nodes = nullptr, *d_nodes_grad = nullptr,
          *d_partial = nullptr;
   int* d_work_counter = nullptr;
@@ -1167,6 +1168,7 @@ static int prepare(pes_context* ctx, const double* d_params, int P, bool grad, h
 // tile groups to give every persistent block about four work items (small grids, e.g. 16k bins)
 static int rows_per_item(const pes_context* ctx, int P, bool grad, int grid) {
   const int max_rows = grad ? chunk_rows<true>() : chunk_rows<false>();
+  if (ctx->rows_override > 0) return std::min(ctx->rows_override, max_rows);
   const long long fit = (long long)P * n_groups_of(ctx) / (4LL * grid);
   return (int)std::max(8LL, std::min((long long)max_rows, fit));
 }
@@ -1511,6 +1513,20 @@ extern "C" int pes_context_set_schedule(pes_context* ctx, int tiles_per_group, i
     return fail(PES_ERR_INVALID, "schedule: tiles_per_group in [1, 4096], blocks_per_cu in [1, 64]");
   ctx->tiles_per_group = tiles_per_group;
   ctx->blocks_per_cu = blocks_per_cu;
+  return PES_OK;
+}
+
+extern "C" int pes_context_set_rows_per_item(pes_context* ctx, int rows) {
+  if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
+  if (rows < 0 || rows > chunk_rows<false>())
+    return fail(PES_ERR_INVALID, "rows per work item: 0 (automatic) or 1 .. 160");
+  ctx->rows_override = rows;
+  return PES_OK;
+}
+
+extern "C" int pes_context_get_rows_per_item(const pes_context* ctx, int* rows) {
+  if (!ctx || !rows) return fail(PES_ERR_INVALID, "NULL argument");
+  *rows = ctx->rows_override;
   return PES_OK;
 }
 

@@ -166,9 +166,17 @@ def test_schedules_agree_and_autotune_picks_one():
         l, g = (x.cpu().numpy() for x in core.lnl(rows, grad=True))
         np.testing.assert_allclose(l, ref_l, rtol=1e-12)
         assert np.max(np.abs(g - ref_g) / scale) < 1e-11
-    best, times = core.autotune(rows, grad=True, candidates=[(8, 2), (32, 4)], reps=1, warmup=0)
+    # rows per work item: any value gives the same results (the partial sums are per row)
+    for r in (1, 7, 40, 160):
+        core.set_schedule(8, 2, r)
+        assert core.rows_per_item() == r
+        l, g = (x.cpu().numpy() for x in core.lnl(rows, grad=True))
+        np.testing.assert_allclose(l, ref_l, rtol=1e-12)
+        assert np.max(np.abs(g - ref_g) / scale) < 1e-11
+        np.testing.assert_allclose(core.lnl(rows).cpu().numpy(), ref_l, rtol=1e-12)
+    best, times = core.autotune(rows, grad=True, candidates=[(8, 2), (32, 4), (8, 2, 10)], reps=1, warmup=0)
     assert best in times and times[best] == min(times.values())
-    assert core.schedule() == best
+    assert core.schedule() == best[:2] and core.rows_per_item() == (best[2] if len(best) > 2 else 0)
     torch.cuda.synchronize()
 
 

@@ -69,6 +69,8 @@ def test_c_abi_argument_validation():
     assert L.pes_context_reserve(None, 8, 1) == 1
     assert L.pes_context_pin_scratch(None, 1) == 1
     assert L.pes_context_set_duty_cycle(None, 0, 1.0, 0.5, 0.0, 0.1) == 1
+    assert L.pes_context_set_rows_per_item(None, 8) == 1
+    assert L.pes_context_get_rows_per_item(None, None) == 1
     # time domain: validation precedes any device work
     td_args = [0, 2.5e9, 0.0, 0.0, 0.0, 0.0, 0.0, None, None, 0.0, 1.0]
     assert L.pes_td_links(*td_args, 1, 0.1, 0.2, 0.3, 0.0, 1.0, 4, None, None) == 1  # one waveform sample
