@@ -546,45 +546,56 @@ __device__ inline void td_store(double* __restrict__ out, size_t N, int i, doubl
   }
 }
 
-// The same with the links staged in LDS: a block of TD_BLOCK outputs reads the window of link
-// samples [i0 + jb_min, i0 + TD_BLOCK - 1 + added] of each link once, coalesced, and takes every
-// delayed value from LDS (the global path reads each sample 2 MAXD + 1 times through L1).  Used
-// when the window fits (added - jb_min <= TD_WIN_EXTRA: delays up to ~256 samples).
+// The same with the links staged in LDS: a block of TD_BLOCK x TD_OPT outputs reads the window
+// of link samples [i0 + jb_min, i0 + TD_BLOCK TD_OPT - 1 + added] of each link once, coalesced,
+// and takes every delayed value from LDS (the global path reads each sample 2 MAXD + 1 times
+// through L1); each thread computes TD_OPT outputs TD_BLOCK apart, so that more window loads are
+// in flight per thread before the barrier.  Used when the window fits (added - jb_min <=
+// TD_WIN_EXTRA: delays up to ~256 samples).
+#ifndef TD_OPT
+#define TD_OPT 2
+#endif
 constexpr int TD_BLOCK = 256, TD_WIN_EXTRA = 256;
 template <int MAXD, bool ORTH>
 __global__ __launch_bounds__(TD_BLOCK) void k_tdi_td_lds(const double* __restrict__ links, int n_ext, int n_out,
                                                          TdDelays dl, double* __restrict__ out) {
   extern __shared__ double win[];  // [6][W]
   const int jb_min = dl.jb[MAXD];
-  const int W = TD_BLOCK + dl.jb[0] - jb_min + 1;
-  const int i0 = blockIdx.x * TD_BLOCK, tid = threadIdx.x;
+  const int W = TD_BLOCK * TD_OPT + dl.jb[0] - jb_min + 1;
+  const int i0 = blockIdx.x * (TD_BLOCK * TD_OPT), tid = threadIdx.x;
   const int g0 = i0 + jb_min;  // first link sample of the window
+#pragma unroll
   for (int l = 0; l < 6; ++l) {
     const double* y = links + (size_t)l * n_ext;
+#pragma unroll 2
     for (int w = tid; w < W; w += TD_BLOCK) win[l * W + w] = (g0 + w < n_ext) ? y[g0 + w] : 0.0;
   }
   __syncthreads();
-  const int i = i0 + tid;
-  if (i >= n_out) return;
-  double X = 0.0, Y = 0.0, Z = 0.0;
 #pragma unroll
-  for (int l = 0; l < 6; ++l) {
-    const double* y = win + l * W + tid - jb_min;  // y[j] = link sample i + j
+  for (int o = 0; o < TD_OPT; ++o) {
+    const int t = o * TD_BLOCK + tid;  // this output's place in the block
+    const int i = i0 + t;
+    if (i >= n_out) return;
+    double X = 0.0, Y = 0.0, Z = 0.0;
 #pragma unroll
-    for (int m = 0; m <= MAXD; ++m) {
-      double v;
-      if (m == 0) {
-        v = y[dl.jb[0]];
-      } else {
-        const double a = y[dl.jb[m]], b = y[dl.jb[m] + 1];
-        v = a + (b - a) * dl.fr[m];
+    for (int l = 0; l < 6; ++l) {
+      const double* y = win + l * W + t - jb_min;  // y[j] = link sample i + j
+#pragma unroll
+      for (int m = 0; m <= MAXD; ++m) {
+        double v;
+        if (m == 0) {
+          v = y[dl.jb[0]];
+        } else {
+          const double a = y[dl.jb[m]], b = y[dl.jb[m] + 1];
+          v = a + (b - a) * dl.fr[m];
+        }
+        if (td_coef(0, l, m) != 0.0) X += td_coef(0, l, m) * v;
+        if (td_coef(1, l, m) != 0.0) Y += td_coef(1, l, m) * v;
+        if (td_coef(2, l, m) != 0.0) Z += td_coef(2, l, m) * v;
       }
-      if (td_coef(0, l, m) != 0.0) X += td_coef(0, l, m) * v;
-      if (td_coef(1, l, m) != 0.0) Y += td_coef(1, l, m) * v;
-      if (td_coef(2, l, m) != 0.0) Z += td_coef(2, l, m) * v;
     }
+    td_store<ORTH>(out, (size_t)n_out, i, X, Y, Z);
   }
-  td_store<ORTH>(out, (size_t)n_out, i, X, Y, Z);
 }
 
 template <int MAXD, bool ORTH>
@@ -1460,10 +1471,11 @@ extern "C" int pes_td_tdi(double arm_length_m, double dt, int tdi_generation, in
   hipStream_t st = (hipStream_t)stream;
   const int span = added - dl.jb[maxd];  // window beyond the block: jb[0] - jb_min
   if (span <= TD_WIN_EXTRA) {
-    // LDS window of each link: TD_BLOCK + span + 1 samples (<= 24.6 KB for the six links);
-    // every read index tid - jb_min + jb[m] (+ 1) lies in [0, TD_BLOCK - 1 + span]
-    const dim3 grid((n_out + TD_BLOCK - 1) / TD_BLOCK), block(TD_BLOCK);
-    const size_t lds = 6 * (size_t)(TD_BLOCK + span + 1) * sizeof(double);
+    // LDS window of each link: TD_BLOCK TD_OPT + span + 1 samples (<= 36.9 KB for the six links
+    // at TD_OPT = 2); every read index t - jb_min + jb[m] (+ 1), t < TD_BLOCK TD_OPT, lies in
+    // [0, TD_BLOCK TD_OPT - 1 + span]
+    const dim3 grid((n_out + TD_BLOCK * TD_OPT - 1) / (TD_BLOCK * TD_OPT)), block(TD_BLOCK);
+    const size_t lds = 6 * (size_t)(TD_BLOCK * TD_OPT + span + 1) * sizeof(double);
     if (maxd == 3 && orthogonal) hipLaunchKernelGGL((k_tdi_td_lds<3, true>), grid, block, lds, st, d_links, n_ext, n_out, dl, d_out);
     else if (maxd == 3) hipLaunchKernelGGL((k_tdi_td_lds<3, false>), grid, block, lds, st, d_links, n_ext, n_out, dl, d_out);
     else if (orthogonal) hipLaunchKernelGGL((k_tdi_td_lds<7, true>), grid, block, lds, st, d_links, n_ext, n_out, dl, d_out);
