@@ -239,8 +239,13 @@ int pes_context_set_profiling(pes_context* ctx, int enable);
 int pes_profile_read(pes_context* ctx, int* n_launches, double* total_ms);
 
 /* Work of the last profiled pes_lnl_batch: the number of (bin, mode, row) triples with the bin
- * at or below the mode's cut (where the fused kernel evaluates a mode) and of (bin, row) pairs
- * at or below the row's largest cut; synchronises the device.  No counterpart in the reference. */
+ * at or below the mode's cut and, for a duty-cycled detector, a nonzero observation window at
+ * the mode's t_f + t_c (where the fused kernel needs to evaluate a mode; it skips the tiles of
+ * 256 bins where the window is zero for every mode), and of (bin, row) pairs at or below the row's
+ * largest cut.  Counted when called, from the context's scratch, which must still hold that
+ * batch: PES_ERR_STATE once another batch or output call has set up rows through the context
+ * (a HIP graph replay also rewrites the scratch without being seen here).  Synchronises the
+ * device.  No counterpart in the reference. */
 int pes_profile_work(pes_context* ctx, double* bin_mode_rows, double* bin_rows);
 
 /* Number of bins of detector det (after finalize). */

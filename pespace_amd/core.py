@@ -251,7 +251,9 @@ class LikelihoodCore:
 
     def profile_work(self):
         """(bin_mode_rows, bin_rows) of the last profiled evaluation: the triples on which a mode
-        is evaluated and the (bin, row) pairs at or below the row's largest mode cut."""
+        needs evaluating (bin at or below the mode's cut and, for a duty-cycled detector, inside
+        an observation window) and the (bin, row) pairs at or below the row's largest mode cut.
+        Counted from the context's scratch: call it before the next evaluation on this core."""
         bm = ctypes.c_double()
         br = ctypes.c_double()
         check(lib().pes_profile_work(self._ctx, ctypes.byref(bm), ctypes.byref(br)))

@@ -174,6 +174,12 @@ struct ModeG {
 
 constexpr double kOmegaYear = 2.0 * PI / YEAR_SI;
 
+// t = t_f + t_c of a mode at this lane's bin, from the mode's node entries n0 (Psi, dPsi/du of
+// nodes j, j + 1 at n0[2], n0[3], n0[6], n0[7]) and the Hermite weights of t_f
+__device__ inline double mode_time(const double* __restrict__ n0, const BinPre& b, double tc) {
+  return fma(b.he1, n0[6] - n0[2], fma(b.he2, n0[3], fma(b.he3, n0[7], tc)));
+}
+
 template <int PREC, int KIND>
 __device__ inline void mode_geom(const DetectorDev& det, const double* __restrict__ mf,
                                  const double* __restrict__ n0, const BinPre& b, const RowC& rc, ModeG& g) {
@@ -182,7 +188,7 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
   g.A = fma(b.hb1, A1 - A0, fma(b.hb2, dA0, fma(b.hb3, dA1, A0)));
   const double dP = P1 - P0;
   const double Psi = fma(b.hb1, dP, fma(b.hb2, dP0, fma(b.hb3, dP1, P0)));
-  g.t = fma(b.he1, dP, fma(b.he2, dP0, fma(b.he3, dP1, rc.tc)));
+  g.t = mode_time(n0, b, rc.tc);
   double w = 1.0;  // duty window at t (its slope is recomputed by the adjoint)
   if constexpr (kDuty<KIND>) {
     double dw;
@@ -672,22 +678,36 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
                                  const double* __restrict__ tile_dd, const double* __restrict__ pconst,
                                  const double* __restrict__ nodes, const double* __restrict__ nodes_grad, int NNS,
                                  double du, const ModeList& modes, int lane, int wave, double* acc, int j0,
-                                 int nj, double* stage_n, double* stage_g, int& buf) {
+                                 int nj, double* stage_n, double* stage_g, int& buf,
+                                 const unsigned long long* __restrict__ off_bits, int n_tiles) {
   constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
   const int bi = tile * TILE + threadIdx.x;
+  // a duty-cycled detector's tile whose window is zero for every mode of row p throughout
+  // (k_duty_off): the response is zero there, as above the row's cuts.  The tile's flag is read
+  // only for rows that have such tiles (PC_OFFANY).  p is uniform: scalar loads
+  auto tile_off = [&](int p) -> bool {
+    if constexpr (kDuty<KIND>) {
+      const int pu = __builtin_amdgcn_readfirstlane(p);
+      if (pconst[(size_t)pu * PC_STRIDE + PC_OFFANY] == 0.0) return false;
+      const size_t w = (size_t)pu * ((n_tiles + 63) >> 6) + (tile >> 6);
+      return ((off_bits[w] >> (tile & 63)) & 1ull) != 0;
+    } else {
+      return false;
+    }
+  };
   BinPre b;
   bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
   constexpr int SN = MAX_MODES * NJ_STAGE * 4, SG = MAX_MODES * NJ_STAGE * 16;  // doubles per buffer
   if constexpr (STAGED) {  // copy for the first evaluated row
     int q = p0;
-    while (q < p1 && !row_live(pconst, q, f0)) ++q;
+    while (q < p1 && !(row_live(pconst, q, f0) && !tile_off(q))) ++q;
     if (q < p1)
       stage_nodes_dma<GRAD>(nodes, nodes_grad, (size_t)q * modes.n * NNS + j0, NNS, modes.n, nj, stage_n + buf * SN,
                             stage_g + buf * SG, lane, wave);
   }
   for (int p = p0; p < p1; ++p) {
     const double* pc = pconst + (size_t)p * PC_STRIDE;
-    if (f0 > pc[PC_FCUT]) {
+    if (f0 > pc[PC_FCUT] || tile_off(p)) {
       if (threadIdx.x == 0) acc[(p - p0) * NOUT] -= tile_dd[tile];
       continue;
     }
@@ -718,7 +738,7 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
       size_t nstride, gstride;
       if constexpr (STAGED) {
         int qn = p + 1;  // next evaluated row
-        while (qn < p1 && !row_live(pconst, qn, f0)) ++qn;
+        while (qn < p1 && !(row_live(pconst, qn, f0) && !tile_off(qn))) ++qn;
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's copies for row p are in LDS
         __syncthreads();                     // ... and every wave's; all waves are past row p - 1
         if constexpr (GRAD) load_rec();
@@ -851,7 +871,8 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
                                               const double* __restrict__ nodes_grad, int NNS, double du,
                                               ModeList modes, int P, int rows, const int* __restrict__ tile_jr,
                                               int tile_begin, int tile_end, int group_offset,
-                                              int* __restrict__ work_counter, double* __restrict__ partial) {
+                                              int* __restrict__ work_counter, double* __restrict__ partial,
+                                              const unsigned long long* __restrict__ off_bits) {
   constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
   constexpr int CH = chunk_rows<GRAD>();  // 4x the rows without the gradient: 1.3% faster
   __shared__ double acc[TILE / 64][CH][NOUT];
@@ -901,7 +922,7 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
       const int j0 = tile_jr[2 * tile], nj = tile_jr[2 * tile + 1];
 #define PES_TILE_ARGS                                                                                         \
   det, tile, f0, p0, p1, bin_f, bin_j, bin_s, bin_rec, tile_dd, pconst, nodes, nodes_grad, NNS, du, \
-      modes, lane, wave, &acc[0][0][0], j0, nj, stage_n, &stage_g[0], buf
+      modes, lane, wave, &acc[0][0][0], j0, nj, stage_n, &stage_g[0], buf, off_bits, n_tiles
       // the host's tile ranges give this launch only duty-cycled detectors (DUTY) or only
       // detectors without (kind bit 1 = DUTY)
       constexpr int KD = DUTY ? 2 : 0;

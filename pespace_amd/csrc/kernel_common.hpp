@@ -22,6 +22,7 @@ enum PConst {
   PC_MODE = 32,  // per mode: Fp re,im, 2 Fc re,im, cut_hz, dFp/diota re,im, 2 dFc/diota re,im
   PC_MODE_STRIDE = 9,
   PC_FCUT = 32 + 6 * 9,  // largest mode cut [Hz]: the response is zero above it
+  PC_OFFANY,             // 1: a tile of a duty-cycled detector lies in an off period for this row
   PC_STRIDE = 32 + 6 * 9 + 2
 };
 enum { MF_FPR = 0, MF_FPI, MF_FCR, MF_FCI, MF_CUT, MF_DFPR, MF_DFPI, MF_DFCR, MF_DFCI };

@@ -95,6 +95,7 @@ __global__ __launch_bounds__(64) void k_setup(const double* __restrict__ params,
   pc[PC_DL] = dL;
   pc[PC_INVDL] = 1.0 / dL;
   pc[PC_VALID] = valid ? 1.0 : 0.0;
+  pc[PC_OFFANY] = 0.0;  // set by k_duty_off
   double sl, cl, sb, cb, sp, cp;
   sincos(lam, &sl, &cl);
   sincos(bet, &sb, &cb);
@@ -306,6 +307,103 @@ __global__ __launch_bounds__(64) void k_live(const DetectorDev* __restrict__ det
   }
   live[2 * (size_t)p] = bm;
   live[2 * (size_t)p + 1] = br;
+}
+
+// profiler only: per row, the (bin, mode) of one duty-cycled detector, below the mode's cut,
+// whose observation window is zero at t = t_f + t_c (they need no work; the fused kernel skips
+// the tiles where that holds for every mode, k_duty_off), subtracted from k_live's count.  One lane per bin, rows over the
+// grid's y dimension; the counts are whole numbers, so the atomic sums are exact.
+__global__ __launch_bounds__(256) void k_live_off(const DetectorDev* __restrict__ dets, int det_i,
+                                                  const double* __restrict__ bin_f, const int* __restrict__ bin_j,
+                                                  const double* __restrict__ bin_s, const double* __restrict__ pconst,
+                                                  const double* __restrict__ nodes, int NNS, double du, ModeList modes,
+                                                  int P, double* __restrict__ live) {
+  const DetectorDev& det = dets[det_i];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int p = blockIdx.y; p < P; p += gridDim.y) {
+    const double* pc = pconst + (size_t)p * PC_STRIDE;
+    double off = 0.0;
+    if (k < det.n_bins && pc[PC_VALID] != 0.0) {
+      const int bi = det.bin_offset + k;
+      BinPre b;
+      bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
+      for (int mi = 0; mi < modes.n; ++mi) {
+        if (b.f > pc[PC_MODE + mi * PC_MODE_STRIDE + MF_CUT]) continue;
+        const double* n0 = nodes + (((size_t)p * modes.n + mi) * NNS + b.j) * 4;
+        double w, dw;
+        duty_window(det, mode_time(n0, b, pc[PC_TC]), w, dw);
+        if (w == 0.0) off += 1.0;
+      }
+    }
+    off = wave_sum(off);  // every lane of the block reaches this
+    if ((threadIdx.x & 63) == 0 && off != 0.0) atomicAdd(&live[2 * (size_t)p], -off);
+  }
+}
+
+// True when t = t_f + t_c over node interval j of one mode (node entries n: Psi, dPsi/du of
+// nodes j, j + 1 at n[2], n[3], n[6], n[7]; frequencies fa < fb) lies inside one off period of
+// the detector's window.  The fused kernel's t at a bin of the interval is N(s) / (2 pi du f),
+// N(s) = 6 s (1 - s) (Psi1 - Psi0) + du [(1 - s)(1 - 3s) Psi0' + s (3s - 2) Psi1'], a quadratic
+// in s with N(0) = du Psi0', N(1) = du Psi1': bounded by its extremes over [0, 1] times the
+// range of 1 / (2 pi du f).  The margin (1 ms plus 1e-9 of |t|) covers the kernel's rounding of
+// t, so no bin of a flagged interval can see a nonzero window.  NaN entries give false.
+__device__ inline bool interval_off(const DetectorDev& det, const double* __restrict__ n, double fa, double fb,
+                                    double du, double tc) {
+  const double dP = n[6] - n[2], c = du * n[3], d1 = du * n[7];
+  const double a = 3.0 * (c + d1) - 6.0 * dP, bq = 6.0 * dP - 4.0 * c - 2.0 * d1;
+  double lo = fmin(c, d1), hi = fmax(c, d1);
+  if (a != 0.0) {
+    const double sv = -bq / (2.0 * a);
+    if (sv > 0.0 && sv < 1.0) {
+      const double v = c - bq * bq / (4.0 * a);
+      lo = fmin(lo, v);
+      hi = fmax(hi, v);
+    }
+  }
+  const double ga = 1.0 / (2.0 * PI * du * fa), gb = 1.0 / (2.0 * PI * du * fb);
+  const double tl = tc + fmin(lo * ga, lo * gb), th = tc + fmax(hi * ga, hi * gb);
+  const double m = 1e-3 + 1e-9 * (fabs(tl) + fabs(th));
+  const double start = det.duty_t0 + floor((tl - m - det.duty_t0) * det.duty_inv_period) * det.duty_period;
+  return tl - m - start >= det.duty_on && th + m - start < det.duty_period;
+}
+
+// Per (tile, row): bit 1 when the tile belongs to a duty-cycled detector and every mode's t over
+// the tile's node intervals lies inside an off period of its window (modes whose cut lies below
+// the tile's first frequency aside), so that the response is zero over the whole tile: the fused
+// kernel then treats it as it treats a tile above the row's cuts (lnL adds -sum w |d|^2 of the
+// tile, the gradient nothing).  One wave per 64 tiles writes the row's word with a vote (no
+// clearing needed) and marks a row with any such tile (PC_OFFANY, cleared by k_setup), so that
+// the kernel reads the flags of those rows only; rows over the grid's y dimension.  enable = 0
+// (direct evaluation at every bin, no node intervals) writes zeros.
+__global__ __launch_bounds__(64) void k_duty_off(const DetectorDev* __restrict__ dets, int n_det, int n_tiles,
+                                                 const double* __restrict__ tile_f0, const int* __restrict__ tile_jr,
+                                                 double* __restrict__ pconst, const double* __restrict__ nodes,
+                                                 int NNS, double u0, double du, ModeList modes, int P, int words,
+                                                 int enable, unsigned long long* __restrict__ off_bits) {
+  const int tile = blockIdx.x * 64 + threadIdx.x;
+  for (int p = blockIdx.y; p < P; p += gridDim.y) {
+    const double* pc = pconst + (size_t)p * PC_STRIDE;
+    bool off = false;
+    if (enable && tile < n_tiles && pc[PC_VALID] != 0.0) {
+      const DetectorDev& det = dets[tile_detector(dets, n_det, tile)];
+      if (det.duty) {
+        off = true;
+        const double f0 = tile_f0[tile], tc = pc[PC_TC];
+        const int j0 = tile_jr[2 * tile], j1 = j0 + tile_jr[2 * tile + 1];
+        for (int mi = 0; mi < modes.n && off; ++mi) {
+          if (f0 > pc[PC_MODE + mi * PC_MODE_STRIDE + MF_CUT]) continue;
+          const double* nm = nodes + ((size_t)p * modes.n + mi) * NNS * 4;
+          for (int j = j0; j < j1 && off; ++j)
+            off = interval_off(det, nm + (size_t)j * 4, exp(u0 + du * j), exp(u0 + du * (j + 1)), du, tc);
+        }
+      }
+    }
+    const unsigned long long m = __ballot(off);
+    if (threadIdx.x == 0) {
+      off_bits[(size_t)p * words + blockIdx.x] = m;
+      if (m != 0ull) pconst[(size_t)p * PC_STRIDE + PC_OFFANY] = 1.0;  // every writer writes 1
+    }
+  }
 }
 
 // ---- response / waveform output kernels (parity checks) -------------------------------------
@@ -661,10 +759,12 @@ __global__ void k_whittle_final(const double* __restrict__ work, int nb, double 
       const double *bin_rec, const double *tile_f0,                                                       \
       const double *tile_dd, int tiles_per_group, const double *pconst, const double *nodes,                 \
       const double *nodes_grad, int NNS, double du, ModeList modes, int P, int rows, const int *tile_jr,     \
-      int tile_begin, int tile_end, int group_offset, int *work_counter, double *partial
+      int tile_begin, int tile_end, int group_offset, int *work_counter, double *partial,                    \
+      const unsigned long long *off_bits
 #define PES_LNL_NAMES                                                                                     \
   dets, n_det, n_tiles, bin_f, bin_j, bin_s, bin_rec, tile_f0, tile_dd, tiles_per_group, pconst, \
-      nodes, nodes_grad, NNS, du, modes, P, rows, tile_jr, tile_begin, tile_end, group_offset, work_counter, partial
+      nodes, nodes_grad, NNS, du, modes, P, rows, tile_jr, tile_begin, tile_end, group_offset, work_counter, partial, \
+      off_bits
 
 }  // namespace
 
@@ -724,12 +824,18 @@ struct pes_context {
   double *d_setup = nullptr, *d_pconst = nullptr, *d_nodes = nullptr, *d_nodes_grad = nullptr,
          *d_partial = nullptr;
   int* d_work_counter = nullptr;
+  unsigned long long* d_off_bits = nullptr;  // [cap_P][off_words] duty off-tile flags (duty contexts)
   int n_cu = 256;
   // optional timing of the fused kernel (HIP events on the launch stream)
   bool profiling = false;
   std::vector<hipEvent_t> ev_start, ev_stop;
-  double* d_live = nullptr;  // [cap_live][2] live (bin, mode) and bin counts of the last profiled launch
-  int cap_live = 0, n_live = 0;
+  double* d_live = nullptr;  // [cap_live][2] per-row work counts, filled by pes_profile_work
+  int cap_live = 0;
+  // the context scratch is refilled by every batch set up through it (scratch_seq counts them);
+  // a profiled pes_lnl_batch records its row count and that batch's number, so that
+  // pes_profile_work can count its work from the scratch while it still holds the batch
+  unsigned long long scratch_seq = 0, live_seq = 0;
+  int live_P = 0;
 };
 
 extern "C" const char* pes_last_error(void) { return g_err.c_str(); }
@@ -877,6 +983,10 @@ static int tile_ranges(const pes_context* ctx, TileRange* out) {
   return n;
 }
 
+// 64-bit words per row of the duty off-tile flags (one bit per tile of the context; the fused
+// kernel forms the same count from n_tiles)
+static int off_words_of(const pes_context* ctx) { return (ctx->n_tiles + 63) / 64; }
+
 static int n_groups_of(const pes_context* ctx) {
   TileRange r[MAX_TILE_RANGES];
   const int n = tile_ranges(ctx, r);
@@ -912,7 +1022,7 @@ static int noalloc_base(int device, char** out) {
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // byte offsets of the scratch buffers of a batch of P rows inside the array, and their total
-static size_t noalloc_layout(const pes_context* ctx, int P, bool grad, size_t off[6]) {
+static size_t noalloc_layout(const pes_context* ctx, int P, bool grad, size_t off[7]) {
   const size_t NNS = (size_t)ctx->n_nodes + 1, rows = (size_t)P, nm = (size_t)ctx->modes.n;
   size_t o = 0;
   off[0] = o; o += align256(MAX_TILE_RANGES * sizeof(int));                // work counters
@@ -921,6 +1031,7 @@ static size_t noalloc_layout(const pes_context* ctx, int P, bool grad, size_t of
   off[3] = o; o += align256(rows * nm * NNS * 4 * sizeof(double));         // node tables
   off[4] = o; if (grad) o += align256(rows * nm * NNS * 16 * sizeof(double));  // their gradients
   off[5] = o; o += align256((size_t)n_groups_of(ctx) * (grad ? 1 + PES_N_PARAMS : 1) * rows * sizeof(double));
+  off[6] = o; if (ctx->any_duty) o += align256(rows * off_words_of(ctx) * sizeof(unsigned long long));  // off tiles
   return o;
 }
 
@@ -1081,6 +1192,7 @@ extern "C" void pes_context_destroy(pes_context* ctx) {
   free_dev(ctx->d_pconst);
   free_dev(ctx->d_nodes);
   free_dev(ctx->d_nodes_grad);
+  free_dev(ctx->d_off_bits);
   free_dev(ctx->d_partial);
   free_dev(ctx->d_live);
   for (auto e : ctx->ev_start) (void)hipEventDestroy(e);
@@ -1114,16 +1226,21 @@ static int ensure_scratch(pes_context* ctx, int P, bool grad) {
   if (P <= ctx->cap_P && (!grad || ctx->cap_grad)) return PES_OK;
   int cap = P > ctx->cap_P ? P : ctx->cap_P;
   bool g = grad || ctx->cap_grad;
+  ++ctx->scratch_seq;  // the rows and node tables of earlier batches are gone
   free_dev(ctx->d_setup);
   free_dev(ctx->d_pconst);
   free_dev(ctx->d_nodes);
   free_dev(ctx->d_nodes_grad);
+  free_dev(ctx->d_off_bits);
   ctx->d_setup = ctx->d_pconst = ctx->d_nodes = ctx->d_nodes_grad = nullptr;
+  ctx->d_off_bits = nullptr;
   size_t NNS = (size_t)ctx->n_nodes + 1;
   HIP_TRY(hipMalloc(&ctx->d_setup, (size_t)cap * N_SETUP * 5 * sizeof(double)));
   HIP_TRY(hipMalloc(&ctx->d_pconst, (size_t)cap * PC_STRIDE * sizeof(double)));
   HIP_TRY(hipMalloc(&ctx->d_nodes, (size_t)cap * ctx->modes.n * NNS * 4 * sizeof(double)));
   if (g) HIP_TRY(hipMalloc(&ctx->d_nodes_grad, (size_t)cap * ctx->modes.n * NNS * 16 * sizeof(double)));
+  if (ctx->any_duty)
+    HIP_TRY(hipMalloc(&ctx->d_off_bits, (size_t)cap * off_words_of(ctx) * sizeof(unsigned long long)));
   ctx->cap_P = cap;
   ctx->cap_grad = g;
   return PES_OK;
@@ -1133,10 +1250,12 @@ static int ensure_scratch(pes_context* ctx, int P, bool grad) {
 struct BatchScratch {
   double *setup, *pconst, *nodes, *nodes_grad, *partial;
   int* work_counter;
+  unsigned long long* off_bits;  // duty contexts only
 };
 
 static BatchScratch context_scratch(const pes_context* ctx) {
-  return {ctx->d_setup, ctx->d_pconst, ctx->d_nodes, ctx->d_nodes_grad, ctx->d_partial, ctx->d_work_counter};
+  return {ctx->d_setup, ctx->d_pconst,       ctx->d_nodes,     ctx->d_nodes_grad,
+          ctx->d_partial, ctx->d_work_counter, ctx->d_off_bits};
 }
 
 static int check_batch_args(const pes_context* ctx, const double* d_params, int P) {
@@ -1172,6 +1291,7 @@ static int prepare(pes_context* ctx, const double* d_params, int P, bool grad, h
   rc = ensure_scratch(ctx, P > 0 ? P : 1, grad);
   if (rc != PES_OK) return rc;
   if (P == 0) return PES_OK;
+  ++ctx->scratch_seq;
   return launch_setup_nodes(ctx, context_scratch(ctx), d_params, P, grad, st);
 }
 
@@ -1199,6 +1319,13 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
   // a kernel without the window code)
   TileRange ranges[MAX_TILE_RANGES];
   const int n_ranges = tile_ranges(ctx, ranges);
+  const int off_words = off_words_of(ctx);
+  if (ctx->any_duty) {  // tiles that a duty cycle's off periods cover for every mode of a row
+    hipLaunchKernelGGL(k_duty_off, dim3(off_words, std::min(P, 65535)), dim3(64), 0, st, ctx->d_dets,
+                       (int)ctx->hdets.size(), ctx->n_tiles, ctx->d_tile_f0, ctx->d_tile_jr, sc.pconst, sc.nodes, NNS,
+                       ctx->u0, ctx->du, ctx->modes, P, off_words, ctx->direct ? 0 : 1, sc.off_bits);
+    HIP_TRY(hipGetLastError());
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (profile) {
     HIP_TRY(hipEventCreate(&e0));
@@ -1216,7 +1343,8 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
 #define PES_LNL_ARGS                                                                                         \
   ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_rec,                 \
       ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, sc.pconst, sc.nodes, sc.nodes_grad, NNS, ctx->du, \
-      ctx->modes, P, rows, ctx->d_tile_jr, tile_begin, tile_end, group_offset, counter, sc.partial
+      ctx->modes, P, rows, ctx->d_tile_jr, tile_begin, tile_end, group_offset, counter, sc.partial,       \
+      sc.off_bits
     if (!grad)
       HIP_TRY(launch_lnl_fwd(precision, duty, grid_r, st, PES_LNL_ARGS));
     else if (precision == 0 && !duty)
@@ -1234,16 +1362,8 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
     HIP_TRY(hipEventRecord(e1, st));
     ctx->ev_start.push_back(e0);
     ctx->ev_stop.push_back(e1);
-    if (P > ctx->cap_live) {
-      free_dev(ctx->d_live);
-      ctx->d_live = nullptr;
-      HIP_TRY(hipMalloc(&ctx->d_live, (size_t)P * 2 * sizeof(double)));
-      ctx->cap_live = P;
-    }
-    hipLaunchKernelGGL(k_live, dim3((P + 63) / 64), dim3(64), 0, st, ctx->d_dets, n_det, ctx->d_bin_f, sc.pconst,
-                       ctx->modes, P, ctx->d_live);
-    HIP_TRY(hipGetLastError());
-    ctx->n_live = P;
+    ctx->live_P = P;  // the work is counted by pes_profile_work, outside the caller's timed steps
+    ctx->live_seq = ctx->scratch_seq;
   }
   hipLaunchKernelGGL(k_reduce, dim3((P + 63) / 64, nout), dim3(64 * REDUCE_WAVES), 0, st, sc.partial, n_groups, P,
                      nout, sc.pconst, d_lnl, d_grad);
@@ -1253,7 +1373,7 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
 
 extern "C" size_t pes_lnl_batch_noalloc_bytes(const pes_context* ctx, int P, int grad) {
   if (!ctx || !ctx->finalized || P < 1) return 0;
-  size_t off[6];
+  size_t off[7];
   return noalloc_layout(ctx, P, grad != 0, off);
 }
 
@@ -1265,7 +1385,7 @@ extern "C" int pes_lnl_batch_noalloc(pes_context* ctx, const double* d_params, i
   int rc = check_batch_args(ctx, d_params, P);
   if (rc != PES_OK || P == 0) return rc;
   const bool grad = d_grad != nullptr;
-  size_t off[6];
+  size_t off[7];
   const size_t need = noalloc_layout(ctx, P, grad, off);
   if (need > (size_t)PES_NOALLOC_SCRATCH_BYTES) {
     char msg[256];
@@ -1286,6 +1406,7 @@ extern "C" int pes_lnl_batch_noalloc(pes_context* ctx, const double* d_params, i
   sc.nodes = reinterpret_cast<double*>(base + off[3]);
   sc.nodes_grad = grad ? reinterpret_cast<double*>(base + off[4]) : nullptr;
   sc.partial = reinterpret_cast<double*>(base + off[5]);
+  sc.off_bits = ctx->any_duty ? reinterpret_cast<unsigned long long*>(base + off[6]) : nullptr;
   hipStream_t st = (hipStream_t)stream;
   rc = launch_setup_nodes(ctx, sc, d_params, P, grad, st);
   if (rc != PES_OK) return rc;
@@ -1578,12 +1699,36 @@ extern "C" int pes_profile_work(pes_context* ctx, double* bin_mode_rows, double*
   if (!ctx || !bin_mode_rows || !bin_rows) return fail(PES_ERR_INVALID, "NULL argument");
   *bin_mode_rows = 0.0;
   *bin_rows = 0.0;
-  if (ctx->n_live == 0) return PES_OK;
+  const int P = ctx->live_P;
+  if (P == 0) return PES_OK;
+  if (ctx->live_seq != ctx->scratch_seq)
+    return fail(PES_ERR_STATE,
+                "pes_profile_work: a later evaluation has reused the scratch of the profiled batch (count its "
+                "work before the next batch or output call on the context)");
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipDeviceSynchronize());
-  std::vector<double> h((size_t)ctx->n_live * 2);
+  if (P > ctx->cap_live) {
+    free_dev(ctx->d_live);
+    ctx->d_live = nullptr;
+    ctx->cap_live = 0;
+    HIP_TRY(hipMalloc(&ctx->d_live, (size_t)P * 2 * sizeof(double)));
+    ctx->cap_live = P;
+  }
+  const BatchScratch sc = context_scratch(ctx);
+  const int n_det = (int)ctx->hdets.size();
+  hipLaunchKernelGGL(k_live, dim3((P + 63) / 64), dim3(64), 0, nullptr, ctx->d_dets, n_det, ctx->d_bin_f, sc.pconst,
+                     ctx->modes, P, ctx->d_live);
+  HIP_TRY(hipGetLastError());
+  for (int d = 0; d < n_det; ++d) {
+    if (!ctx->hdets[d].d.duty) continue;
+    const dim3 g_off((ctx->hdets[d].d.n_bins + 255) / 256, std::min(P, 65535));
+    hipLaunchKernelGGL(k_live_off, g_off, dim3(256), 0, nullptr, ctx->d_dets, d, ctx->d_bin_f, ctx->d_bin_j,
+                       ctx->d_bin_s, sc.pconst, sc.nodes, ctx->n_nodes + 1, ctx->du, ctx->modes, P, ctx->d_live);
+    HIP_TRY(hipGetLastError());
+  }
+  std::vector<double> h((size_t)P * 2);
   HIP_TRY(hipMemcpy(h.data(), ctx->d_live, h.size() * sizeof(double), hipMemcpyDeviceToHost));
-  for (int p = 0; p < ctx->n_live; ++p) {
+  for (int p = 0; p < P; ++p) {
     *bin_mode_rows += h[2 * (size_t)p];
     *bin_rows += h[2 * (size_t)p + 1];
   }

@@ -11,7 +11,11 @@ off periods:
 * the adjoint gradient against Richardson central differences of the HIP lnL (the window's
   slope enters t_c and, through t_f, the intrinsic parameters);
 * the mixed mode's lnL equals the FP64 lnL (its forward is FP64) and its gradient agrees to 1e-3;
-* a three-detector network with only TianQin duty-cycled.
+* a three-detector network with only TianQin duty-cycled;
+* rows whose signal lies wholly in an off period, whose modes the kernel skips wave by wave:
+  lnL is the data-only value (bitwise the same for every such row), the gradient exactly zero,
+  and the work count (pes_profile_work) leaves out every (bin, mode) with a zero window,
+  checked against the oracle's t_f row by row.
 """
 
 import numpy as np
@@ -152,3 +156,61 @@ def test_duty_network_only_tianqin():
     np.testing.assert_allclose(l_plain, olk.lnl_batch(rows, [dets[0][0], dets[1][0], tq], modes, 0.0, INTERP),
                                rtol=1e-10)
     assert np.all(core.lnl(rows).cpu().numpy() - l_plain > 1.0)
+
+
+def _needed_bin_modes(d, row, modes):
+    """(bin, mode) below the mode's cut, and those of them inside an observation window (the
+    oracle's t_f + t_c): the work count without and with the duty cycle."""
+    p = olk.as_dict(row)
+    hm = oph.PhenomHM(p["chirp_mass"], p["mass_ratio"], p["chi_1"], p["chi_2"], 0.0, modes)
+    wf = olk.waveform(p, d.f, modes, 0.0, INTERP)
+    n_all = n_on = 0
+    for md in modes:
+        keep = d.f * hm.Msec <= hm.mode_cut(md)
+        w = odet.duty_window(wf[f"{md[0]}{md[1]}"]["tf"] + p["coalescence_time"], d.duty)
+        n_all += int(keep.sum())
+        n_on += int((keep & (w != 0.0)).sum())
+    return n_all, n_on
+
+
+def test_duty_off_period_rows_skip_exactly():
+    from pespace_amd._lib import PespaceError
+    modes = oph.MODES
+    case = dict(CASES["tianqin"])
+    case.pop("modes")
+    d, labels = make_detector(modes=modes, **case)
+    core = _core(modes, [(d, labels)])
+    # the window opens at 523,000 s: with t_c = 300,000 or 400,000 s every bin's t_f + t_c (at most
+    # ~3e4 s before the merger on this grid) lies in the off period before it
+    off = np.vstack([inj_row(coalescence_time=300000.0),
+                     particle_cloud(2, seed=5, center=inj_row(coalescence_time=400000.0))])
+    rows = np.vstack([off, inj_row(), particle_cloud(2)])
+    print(f"off-period rows: {len(off)} of {len(rows)}", flush=True)
+    ref = olk.lnl_batch(rows, [d], modes, 0.0, INTERP)
+    for prec in ("fp64", "mixed"):
+        lnl, g = (x.cpu().numpy() for x in core.lnl(rows, grad=True, precision=prec))
+        np.testing.assert_allclose(lnl, ref, rtol=1e-10)
+        assert np.all(lnl[:len(off)] == lnl[0]) and np.all(g[:len(off)] == 0.0), prec
+        assert np.all(np.abs(g[len(off):]).max(axis=1) > 0.0)
+        np.testing.assert_array_equal(core.lnl(rows, precision=prec).cpu().numpy(), lnl)
+    core.set_profiling(True)
+    try:
+        for i, row in enumerate(rows):
+            core.lnl(row[None], grad=True)
+            core.read_profile()
+            bm, br = core.profile_work()
+            n_all, n_on = _needed_bin_modes(d, row, modes)
+            assert bm == n_on, (i, bm, n_on, n_all)
+            assert br > 0
+            if i < len(off):
+                assert n_on == 0 and n_all > 0
+            else:
+                assert 0 < n_on < n_all  # the injection's window starts near its merger
+        # the count is taken from the scratch when asked: another call in between refills it
+        core.lnl(rows[:1], grad=True)
+        core.read_profile()
+        core.response(0, rows[1:2])
+        with pytest.raises(PespaceError, match="reused the scratch"):
+            core.profile_work()
+    finally:
+        core.set_profiling(False)

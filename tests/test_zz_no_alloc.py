@@ -3,8 +3,9 @@
 Both run the same kernels on the same rows; the no-allocation entry point takes its scratch from
 a fixed device array instead of the context's buffers, so results are bitwise equal.  Two calls
 in a row, the second with fewer rows (the array keeps the first call's contents), with and
-without the gradient, in both precision modes; a batch beyond the array is rejected before
-anything is launched.
+without the gradient, in both precision modes, and on a duty-cycled detector with rows in its
+off period (the off-tile flags live in the array too); a batch beyond the array is rejected
+before anything is launched.
 """
 
 import ctypes
@@ -18,10 +19,16 @@ from tests._setup import add_to_core, inj_row, make_detector, particle_cloud
 pytestmark = pytest.mark.gpu
 
 
-def _core():
+# TianQin with a duty cycle whose window opens at 523,000 s (1500 s ramp): rows with t_c well
+# before it lie in an off period, whose tiles the kernel skips (flags in the scratch too)
+TIANQIN_DUTY = dict(orbit="Tianqin_analytic", psd_model="Tianqin_GWSpace", fmin=1e-3,
+                    duty=(1.0e8, 5.0e7, 523000.0, 1500.0))
+
+
+def _core(**det):
     from pespace_amd.core import LikelihoodCore
     modes = oph.MODES
-    d, labels = make_detector(modes=modes)
+    d, labels = make_detector(modes=modes, **det)
     core = LikelihoodCore(modes=[f"{l}{m}" for l, m in modes], n_nodes=1024, f_node_min=1e-4, f_node_max=0.05)
     add_to_core(core, d, labels)
     core._build()
@@ -44,10 +51,13 @@ def _noalloc(core, rows, grad, precision):
     return lnl.cpu().numpy(), (g.cpu().numpy() if grad else None)
 
 
+@pytest.mark.parametrize("duty", [False, True])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_noalloc_matches_pes_lnl_batch_on_two_calls(precision):
-    core = _core()
+def test_noalloc_matches_pes_lnl_batch_on_two_calls(precision, duty):
+    core = _core(**(TIANQIN_DUTY if duty else {}))
     rows = np.vstack([inj_row(), particle_cloud(40, seed=9)])
+    if duty:  # every third row in the off period before the window
+        rows[::3, 10] = 300000.0 + 1000.0 * np.arange(len(rows[::3]))
     for P in (41, 13):  # the second call is smaller: the array keeps the first call's contents
         for grad in (True, False):
             ref = core.lnl(rows[:P], grad=grad, precision=precision)
