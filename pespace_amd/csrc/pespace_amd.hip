@@ -550,54 +550,123 @@ struct TdWave {  // h+, hx sampled at t0 + m dt, m = 0 .. n - 1; zero outside
   double t0, inv_dt;
   int n;
 };
-// h+, hx at the fractional sample x = (t - t0) / dt: linear between samples, zero outside
-__device__ inline void td_wave_at(const TdWave& w, double x, double& hp, double& hc) {
-  if (!(x >= 0.0) || x > (double)(w.n - 1)) {  // outside the samples (or NaN): no signal
-    hp = hc = 0.0;
-    return;
-  }
-  const int j = min((int)floor(x), w.n - 2);  // x = n - 1 exactly reads the last pair at fraction 1
-  const double fr = x - (double)j;
-  const double p0 = w.hp[j], p1 = w.hp[j + 1], c0 = w.hc[j], c1 = w.hc[j + 1];
-  hp = p0 + (p1 - p0) * fr;
-  hc = c0 + (c1 - c0) * fr;
+// h+, hx at the fractional sample x = (t - t0) / dt: linear between samples, zero outside (and
+// for NaN).  In two steps, so that a thread forms all its positions before it loads, and its
+// twelve loads are all issued before the first is waited for: td_pos clamps the position into
+// the samples (the loads need no branch) and notes whether it was inside; td_lerp interpolates
+// the loaded pair, or gives zero outside.  x = n - 1 exactly reads the last pair at fraction 1.
+struct TdPos {
+  int j;
+  double fr;
+  bool inside;
+};
+__device__ inline TdPos td_pos(const TdWave& w, double x) {
+  const double xc = fmin(fmax(x, 0.0), (double)(w.n - 1));  // NaN -> 0
+  const int j = min((int)xc, w.n - 2);                      // xc >= 0: truncation is floor
+  return {j, xc - (double)j, x >= 0.0 && x <= (double)(w.n - 1)};
+}
+__device__ inline double td_lerp(const TdPos& q, double v0, double v1) {
+  return q.inside ? v0 + (v1 - v0) * q.fr : 0.0;
 }
 
 // y_rs(t) = n.[H(t - L - k.x_s) - H(t - k.x_r)].n / (2 (1 - k.n_sr)) at t_i = t0 + i dt, with the
 // constellation at t_i; out [6][n] in the order 12, 21, 23, 32, 31, 13 (link_rs: sent from s).
-// One thread per sample: the orbit (one sincos), six waveform positions in sample units
-// (x = (t - t_w0) / dt - (L + k.x_a) / dt), twelve interpolations and three divisions
-// (1 / (1 +- k.n_c) from 1 / (1 - (k.n_c)^2)).  Neighbouring threads read neighbouring samples.
+// One thread per sample.  The geometry takes the closed forms of the fused kernel's mode_geom
+// (node offsets xd_i = al_i U + be_i V; for link vector n_a = (xd_a+2 - xd_a+1) / L every
+// projection is n_a.v = (sqrt3 / L)(al_a v.V - be_a v.U)): one sincos of the revolution angle
+// (and, KIND 1, of node 1's rotation angle), six scalar products, no library sincos and no
+// division (the three denominators by rcp_d).  The waveform positions are formed in sample units
+// (x = (t - t_w0) / dt - (L + k.x_a) / dt).  Neighbouring threads read neighbouring samples.
+template <int KIND>
 __global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Extrinsic ex, double t0, double dt, int n,
                                                   double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double t = t0 + dt * (double)i;
-  Vec3 nv[3], xv[3];
-  constellation(det, t, nv, xv);
+  double sa, ca;
+  sincos_d(fma(kOmegaYear, t, det.rev0), &sa, &ca);
+  const double kx0 = AU_SEC * fma(ex.k.x, ca, ex.k.y * sa);  // k.x0, x0 = AU (cos, sin, 0)
+  double np[3], nq[3], kn[3], kx[3];                         // n_a.p, n_a.q, n_a.k; k.x_i
+  if constexpr (KIND == 0) {
+    const double sc = sa * ca;
+    const V3 U = {sc, -fma(ca, ca, 1.0), -SQRT3 * sa};
+    const V3 V = {-fma(sa, sa, 1.0), sc, -SQRT3 * ca};
+    const V3 p = {ex.p.x, ex.p.y, ex.p.z}, q = {ex.q.x, ex.q.y, ex.q.z}, k = {ex.k.x, ex.k.y, ex.k.z};
+    const double pU = dot3(p, U), pV = dot3(p, V), qU = dot3(q, U), qV = dot3(q, V);
+    const double kU = dot3(k, U), kV = dot3(k, V);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      np[a] = fma(det.hsa[a], pV, -det.hca[a] * pU);
+      nq[a] = fma(det.hsa[a], qV, -det.hca[a] * qU);
+      kn[a] = fma(det.hsa[a], kV, -det.hca[a] * kU);
+      kx[a] = kx0 + fma(det.hsk[a], kU, det.hck[a] * kV);
+    }
+  } else {
+    double s1, c1;
+    sincos_d(fma(det.omega_rot, t, det.rot0), &s1, &c1);
+    const double cr = -0.5, sr = 0.5 * SQRT3;  // rotation by 2 pi / 3
+    double al[3], be[3];
+    al[0] = s1;
+    be[0] = c1;
+    al[1] = fma(s1, cr, c1 * sr);
+    be[1] = fma(c1, cr, -s1 * sr);
+    al[2] = fma(al[1], cr, be[1] * sr);
+    be[2] = fma(be[1], cr, -al[1] * sr);
+    const double pU = fma(ex.p.x, det.sgxs, fma(ex.p.y, det.sgys, ex.p.z * det.sgzs));
+    const double pV = fma(ex.p.x, det.sgxc, ex.p.y * det.sgyc);
+    const double qU = fma(ex.q.x, det.sgxs, fma(ex.q.y, det.sgys, ex.q.z * det.sgzs));
+    const double qV = fma(ex.q.x, det.sgxc, ex.q.y * det.sgyc);
+    const double kU = fma(ex.k.x, det.gxs, fma(ex.k.y, det.gys, ex.k.z * det.gzs));
+    const double kV = fma(ex.k.x, det.gxc, ex.k.y * det.gyc);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      np[a] = fma(al[a], pV, -be[a] * pU);
+      nq[a] = fma(al[a], qV, -be[a] * qU);
+      kn[a] = det.c3L * fma(al[a], kV, -be[a] * kU);
+      kx[a] = kx0 + fma(al[a], kU, be[a] * kV);
+    }
+  }
   const double base = (t - w.t0) * w.inv_dt, Ls = det.L * w.inv_dt;
-  double Pp[3], Pc[3], kn[3], sp[3], sc[3], rp[3], rc[3];  // sent (t - L - k.x_a), received (t - k.x_a)
+  TdPos qs[3], qr[3];  // waveform positions sent (t - L - k.x_a) and received (t - k.x_a)
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const double np = dot(nv[a], ex.p), nq = dot(nv[a], ex.q);
-    Pp[a] = np * np - nq * nq;  // n.e+.n
-    Pc[a] = 2.0 * np * nq;      // n.ex.n
-    kn[a] = dot(ex.k, nv[a]);
-    const double xr = base - dot(ex.k, xv[a]) * w.inv_dt;
-    td_wave_at(w, xr - Ls, sp[a], sc[a]);
-    td_wave_at(w, xr, rp[a], rc[a]);
+    const double xr = fma(-kx[a], w.inv_dt, base);
+    qs[a] = td_pos(w, xr - Ls);
+    qr[a] = td_pos(w, xr);
+  }
+  double hs[3][4], hr[3][4];  // h+ at j, j + 1 and hx at j, j + 1
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    hs[a][0] = w.hp[qs[a].j];
+    hs[a][1] = w.hp[qs[a].j + 1];
+    hs[a][2] = w.hc[qs[a].j];
+    hs[a][3] = w.hc[qs[a].j + 1];
+    hr[a][0] = w.hp[qr[a].j];
+    hr[a][1] = w.hp[qr[a].j + 1];
+    hr[a][2] = w.hc[qr[a].j];
+    hr[a][3] = w.hc[qr[a].j + 1];
+  }
+  double Pp[3], Pc[3], sp[3], sc[3], rp[3], rc[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    Pp[a] = fma(np[a], np[a], -nq[a] * nq[a]);  // n.e+.n
+    Pc[a] = 2.0 * np[a] * nq[a];                // n.ex.n
+    sp[a] = td_lerp(qs[a], hs[a][0], hs[a][1]);
+    sc[a] = td_lerp(qs[a], hs[a][2], hs[a][3]);
+    rp[a] = td_lerp(qr[a], hr[a][0], hr[a][1]);
+    rc[a] = td_lerp(qr[a], hr[a][2], hr[a][3]);
   }
   // link (r, s) lies along n_c with c the third node; 1 - k.n_sr = 1 + k.n_c when s -> r runs
   // against n_c (links 12, 23, 31) and 1 - k.n_c along it (21, 32, 13)
   double up[3], dn[3];  // 0.5 / (1 + k.n_c), 0.5 / (1 - k.n_c)
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const double h = 0.5 / fma(-kn[c], kn[c], 1.0);
+    const double h = 0.5 * rcp_d(fma(-kn[c], kn[c], 1.0));
     up[c] = (1.0 - kn[c]) * h;
     dn[c] = (1.0 + kn[c]) * h;
   }
   auto link = [&](int r, int s, int c, double f) {
-    return f * (Pp[c] * (sp[s] - rp[r]) + Pc[c] * (sc[s] - rc[r]));
+    return f * fma(Pp[c], sp[s] - rp[r], Pc[c] * (sc[s] - rc[r]));
   };
   const size_t N = (size_t)n;
   out[i] = link(0, 1, 2, up[2]);          // 12
@@ -662,11 +731,26 @@ __global__ __launch_bounds__(TD_BLOCK) void k_tdi_td_lds(const double* __restric
   const int W = TD_BLOCK * TD_OPT + dl.jb[0] - jb_min + 1;
   const int i0 = blockIdx.x * (TD_BLOCK * TD_OPT), tid = threadIdx.x;
   const int g0 = i0 + jb_min;  // first link sample of the window
+  // this thread's window samples (at most KW per link): every load is issued before the first is
+  // waited for (a load-then-store loop waits for each load in turn)
+  constexpr int KW = (TD_BLOCK * TD_OPT + TD_WIN_EXTRA + 1 + TD_BLOCK - 1) / TD_BLOCK;
+  double v[6][KW];
 #pragma unroll
   for (int l = 0; l < 6; ++l) {
     const double* y = links + (size_t)l * n_ext;
-#pragma unroll 2
-    for (int w = tid; w < W; w += TD_BLOCK) win[l * W + w] = (g0 + w < n_ext) ? y[g0 + w] : 0.0;
+#pragma unroll
+    for (int k = 0; k < KW; ++k) {
+      const int w = tid + k * TD_BLOCK;
+      v[l][k] = (w < W && g0 + w < n_ext) ? y[g0 + w] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < 6; ++l) {
+#pragma unroll
+    for (int k = 0; k < KW; ++k) {
+      const int w = tid + k * TD_BLOCK;
+      if (w < W) win[l * W + w] = v[l][k];
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -1556,8 +1640,13 @@ extern "C" int pes_td_links(int orbit_kind, double arm_length_m, double rotation
                             lambda_ref, beta_ref);
   if (rc != PES_OK || n == 0) return rc;
   const TdWave w{d_hp, d_hc, wave_t0, 1.0 / wave_dt, wave_n};
-  hipLaunchKernelGGL(k_links_td, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, d, w,
-                     extrinsic_of(lam, beta, psi, 0.0), t0, dt, n, d_out);
+  const Extrinsic ex = extrinsic_of(lam, beta, psi, 0.0);
+  if (d.kind == 0)
+    hipLaunchKernelGGL(k_links_td<0>, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, d, w, ex, t0, dt, n,
+                       d_out);
+  else
+    hipLaunchKernelGGL(k_links_td<1>, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, d, w, ex, t0, dt, n,
+                       d_out);
   HIP_TRY(hipGetLastError());
   return PES_OK;
 }
