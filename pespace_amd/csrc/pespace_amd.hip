@@ -577,11 +577,49 @@ __device__ inline double td_lerp(const TdPos& q, double v0, double v1) {
 // (and, KIND 1, of node 1's rotation angle), six scalar products, no library sincos and no
 // division (the three denominators by rcp_d).  The waveform positions are formed in sample units
 // (x = (t - t_w0) / dt - (L + k.x_a) / dt).  Neighbouring threads read neighbouring samples.
-template <int KIND>
+//
+// STAGED: the block first copies the stretch of h+, hx its samples can reach into LDS (all loads
+// issued before the first wait), and the twelve interpolation reads per sample come from LDS
+// (the gathers through L1, 192 B per sample, cost more than the kernel's stores).  A node lies
+// within |x_a| <= AU + 2 L of the barycentre, so every position of a block whose samples have
+// base positions in [b_lo, b_hi] lies in [b_lo - R - L/dt_w, b_hi + R], R = (AU + 2 L) / dt_w
+// (td_links_window); the host stages windows of at most 256 KW samples (KW = 2 or 6: 5 s
+// sampling needs ~470, 1 s ~1300), wider ones read global memory (KW = 0).
+constexpr int TD_LINKS_WMAX = 1536;
+__host__ __device__ inline void td_links_window(const DetectorDev& det, const TdWave& w, double t0, double dt,
+                                                int i0, int& lo, int& len) {
+  const double b_lo = (t0 + dt * (double)i0 - w.t0) * w.inv_dt;
+  const double b_hi = (t0 + dt * (double)(i0 + 255) - w.t0) * w.inv_dt;
+  const double R = (AU_SEC + 2.0 * det.L) * w.inv_dt, Ls = det.L * w.inv_dt;
+  lo = (int)floor(b_lo - R - Ls) - 2;
+  len = (int)floor(b_hi + R) + 4 - lo;
+}
+
+template <int KIND, int KW>
 __global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Extrinsic ex, double t0, double dt, int n,
                                                   double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  constexpr bool STAGED = KW > 0;
+  constexpr int WMAX = 256 * KW;
+  __shared__ double win_p[STAGED ? WMAX : 1], win_c[STAGED ? WMAX : 1];
+  // STAGED: the window loads are issued first and land while the geometry is computed
+  double vp[STAGED ? KW : 1], vc[STAGED ? KW : 1];
+  int wlo = 0, wlen = 0;
+  if constexpr (STAGED) {
+    td_links_window(det, w, t0, dt, blockIdx.x * blockDim.x, wlo, wlen);
+    wlen = min(max(wlen, 2), WMAX);  // the host checked the length; LDS reads stay inside
+#pragma unroll
+    for (int k = 0; k < KW; ++k) {
+      const int q = threadIdx.x + 256 * k, s = wlo + q;
+      const bool ok = q < wlen && s >= 0 && s < w.n;
+      vp[k] = ok ? w.hp[s] : 0.0;
+      vc[k] = ok ? w.hc[s] : 0.0;
+    }
+  } else if (i >= n) {
+    return;
+  }
+  // (a thread past the last sample computes a geometry it never stores: it takes part in the
+  // block's barrier)
   const double t = t0 + dt * (double)i;
   double sa, ca;
   sincos_d(fma(kOmegaYear, t, det.rev0), &sa, &ca);
@@ -634,17 +672,42 @@ __global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Ext
     qs[a] = td_pos(w, xr - Ls);
     qr[a] = td_pos(w, xr);
   }
+  if constexpr (STAGED) {
+#pragma unroll
+    for (int k = 0; k < KW; ++k) {
+      const int q = threadIdx.x + 256 * k;
+      if (q < wlen) {
+        win_p[q] = vp[k];
+        win_c[q] = vc[k];
+      }
+    }
+    __syncthreads();
+    if (i >= n) return;
+  }
   double hs[3][4], hr[3][4];  // h+ at j, j + 1 and hx at j, j + 1
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    hs[a][0] = w.hp[qs[a].j];
-    hs[a][1] = w.hp[qs[a].j + 1];
-    hs[a][2] = w.hc[qs[a].j];
-    hs[a][3] = w.hc[qs[a].j + 1];
-    hr[a][0] = w.hp[qr[a].j];
-    hr[a][1] = w.hp[qr[a].j + 1];
-    hr[a][2] = w.hc[qr[a].j];
-    hr[a][3] = w.hc[qr[a].j + 1];
+    if constexpr (STAGED) {
+      // inside positions lie in the window; outside ones (result zero) are kept in the buffer
+      const int js = min(max(qs[a].j - wlo, 0), wlen - 2), jr = min(max(qr[a].j - wlo, 0), wlen - 2);
+      hs[a][0] = win_p[js];
+      hs[a][1] = win_p[js + 1];
+      hs[a][2] = win_c[js];
+      hs[a][3] = win_c[js + 1];
+      hr[a][0] = win_p[jr];
+      hr[a][1] = win_p[jr + 1];
+      hr[a][2] = win_c[jr];
+      hr[a][3] = win_c[jr + 1];
+    } else {
+      hs[a][0] = w.hp[qs[a].j];
+      hs[a][1] = w.hp[qs[a].j + 1];
+      hs[a][2] = w.hc[qs[a].j];
+      hs[a][3] = w.hc[qs[a].j + 1];
+      hr[a][0] = w.hp[qr[a].j];
+      hr[a][1] = w.hp[qr[a].j + 1];
+      hr[a][2] = w.hc[qr[a].j];
+      hr[a][3] = w.hc[qr[a].j + 1];
+    }
   }
   double Pp[3], Pc[3], sp[3], sc[3], rp[3], rc[3];
 #pragma unroll
@@ -1641,12 +1704,19 @@ extern "C" int pes_td_links(int orbit_kind, double arm_length_m, double rotation
   if (rc != PES_OK || n == 0) return rc;
   const TdWave w{d_hp, d_hc, wave_t0, 1.0 / wave_dt, wave_n};
   const Extrinsic ex = extrinsic_of(lam, beta, psi, 0.0);
-  if (d.kind == 0)
-    hipLaunchKernelGGL(k_links_td<0>, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, d, w, ex, t0, dt, n,
-                       d_out);
-  else
-    hipLaunchKernelGGL(k_links_td<1>, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, d, w, ex, t0, dt, n,
-                       d_out);
+  // every block's waveform window has length floor(b_hi + R) - floor(b_lo - R - L / dt_w) + 6
+  // (td_links_window), at most 255 dt / dt_w + 2 R + L / dt_w + 8: stage it in LDS when that fits
+  const double span = 255.0 * dt / wave_dt + 2.0 * (AU_SEC + 3.0 * d.L) / wave_dt + 8.0;  // >= every block's
+  const int kw = span <= 512.0 ? 2 : (span <= (double)TD_LINKS_WMAX ? 6 : 0);
+  const dim3 grid((n + 255) / 256), block(256);
+  hipStream_t st = (hipStream_t)stream;
+#define PES_TDL(K, W) hipLaunchKernelGGL((k_links_td<K, W>), grid, block, 0, st, d, w, ex, t0, dt, n, d_out)
+  if (d.kind == 0) {
+    if (kw == 2) PES_TDL(0, 2); else if (kw == 6) PES_TDL(0, 6); else PES_TDL(0, 0);
+  } else {
+    if (kw == 2) PES_TDL(1, 2); else if (kw == 6) PES_TDL(1, 6); else PES_TDL(1, 0);
+  }
+#undef PES_TDL
   HIP_TRY(hipGetLastError());
   return PES_OK;
 }

@@ -2,7 +2,8 @@
 
 * pes_td_links over three orbits (LISA, TianQin, a geocentric orbit with its own rotation and
   reference source) for a chirping waveform that starts and ends inside the window (so the
-  zero-outside rule is exercised on both sides), 2^14 samples at 5 s;
+  zero-outside rule is exercised on both sides), 2^14 samples at 5 s; and for waveforms sampled
+  at 2.5 s and 0.5 s (the LDS-window and the global-memory kernel);
 * pes_td_tdi of those links for TDI 1.5 / 2.0, A/E/T and X/Y/Z;
 * a year at 5 s (6,311,521 samples) through both kernels, compared with the oracle on every
   sample;
@@ -75,6 +76,25 @@ def test_td_links_match_oracle(orbit):
     assert tol < 1e-7 * np.max(np.abs(got))
     assert np.all(got[:, :1700] == 0.0) and np.all(got[:, -2000:] == 0.0)
     assert np.max(np.abs(got)) > 1e-3 * scale
+
+
+@pytest.mark.parametrize("orbit", [0, 1])
+@pytest.mark.parametrize("wave_dt", [2.5, 0.5])
+def test_td_links_finer_waveform_sampling(orbit, wave_dt):
+    """A waveform sampled more finely than the output: at 2.5 s each block's reach into the
+    waveform (its 256 samples plus the light travel time across the orbit) still fits the LDS
+    window; at 0.5 s it does not, and the kernel reads the waveform from global memory."""
+    orb = ORBITS[orbit]
+    n, dt = 4000, 5.0
+    m = int(12000 * 5.0 / wave_dt)
+    hp, hc = _chirp(m, wave_dt)
+    wt0, t0 = 3.0e6 + 9000.0, 3.0e6
+    print(f"td links orbit {orb[:2]} wave_dt={wave_dt}", flush=True)
+    got = _td_links_gpu(orb, hp, hc, wt0, wave_dt, SKY, t0, dt, n).cpu().numpy()
+    ref = otd.links(orb, hp, hc, wt0, wave_dt, *SKY, t0, dt, n)
+    tol = _time_tol(t0 + n * dt, 2e-3 + 2e-9 * m * wave_dt, np.max(np.abs(hp)))
+    np.testing.assert_allclose(got, ref, rtol=0, atol=tol)
+    assert np.all(got[:, :1700] == 0.0) and np.max(np.abs(got)) > 1e-3 * np.max(np.abs(hp))
 
 
 @pytest.mark.parametrize("gen", ["1.5", "2.0"])
