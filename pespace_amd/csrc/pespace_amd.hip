@@ -595,10 +595,19 @@ __host__ __device__ inline void td_links_window(const DetectorDev& det, const Td
   len = (int)floor(b_hi + R) + 4 - lo;
 }
 
+// Workgroups are dealt to the 8 XCDs in turn (workgroup b to XCD b mod 8), each with its own L2:
+// xcd_block gives XCD x a contiguous run of the logical blocks instead (a bijection of
+// [0, nb)), so that neighbouring blocks, whose waveform windows overlap, share one L2.
+__device__ inline int xcd_block(int b, int nb) {
+  const int x = b & 7, q = nb >> 3, r = nb & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 template <int KIND, int KW>
 __global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Extrinsic ex, double t0, double dt, int n,
                                                   double* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lb = KW > 0 ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int i = lb * blockDim.x + threadIdx.x;
   constexpr bool STAGED = KW > 0;
   constexpr int WMAX = 256 * KW;
   __shared__ double win_p[STAGED ? WMAX : 1], win_c[STAGED ? WMAX : 1];
@@ -606,7 +615,7 @@ __global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Ext
   double vp[STAGED ? KW : 1], vc[STAGED ? KW : 1];
   int wlo = 0, wlen = 0;
   if constexpr (STAGED) {
-    td_links_window(det, w, t0, dt, blockIdx.x * blockDim.x, wlo, wlen);
+    td_links_window(det, w, t0, dt, lb * blockDim.x, wlo, wlen);
     wlen = min(max(wlen, 2), WMAX);  // the host checked the length; LDS reads stay inside
 #pragma unroll
     for (int k = 0; k < KW; ++k) {
