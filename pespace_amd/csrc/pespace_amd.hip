@@ -340,33 +340,6 @@ __global__ __launch_bounds__(256) void k_live_off(const DetectorDev* __restrict_
   }
 }
 
-// True when t = t_f + t_c over node interval j of one mode (node entries n: Psi, dPsi/du of
-// nodes j, j + 1 at n[2], n[3], n[6], n[7]; frequencies fa < fb) lies inside one off period of
-// the detector's window.  The fused kernel's t at a bin of the interval is N(s) / (2 pi du f),
-// N(s) = 6 s (1 - s) (Psi1 - Psi0) + du [(1 - s)(1 - 3s) Psi0' + s (3s - 2) Psi1'], a quadratic
-// in s with N(0) = du Psi0', N(1) = du Psi1': bounded by its extremes over [0, 1] times the
-// range of 1 / (2 pi du f).  The margin (1 ms plus 1e-9 of |t|) covers the kernel's rounding of
-// t, so no bin of a flagged interval can see a nonzero window.  NaN entries give false.
-__device__ inline bool interval_off(const DetectorDev& det, const double* __restrict__ n, double fa, double fb,
-                                    double du, double tc) {
-  const double dP = n[6] - n[2], c = du * n[3], d1 = du * n[7];
-  const double a = 3.0 * (c + d1) - 6.0 * dP, bq = 6.0 * dP - 4.0 * c - 2.0 * d1;
-  double lo = fmin(c, d1), hi = fmax(c, d1);
-  if (a != 0.0) {
-    const double sv = -bq / (2.0 * a);
-    if (sv > 0.0 && sv < 1.0) {
-      const double v = c - bq * bq / (4.0 * a);
-      lo = fmin(lo, v);
-      hi = fmax(hi, v);
-    }
-  }
-  const double ga = 1.0 / (2.0 * PI * du * fa), gb = 1.0 / (2.0 * PI * du * fb);
-  const double tl = tc + fmin(lo * ga, lo * gb), th = tc + fmax(hi * ga, hi * gb);
-  const double m = 1e-3 + 1e-9 * (fabs(tl) + fabs(th));
-  const double start = det.duty_t0 + floor((tl - m - det.duty_t0) * det.duty_inv_period) * det.duty_period;
-  return tl - m - start >= det.duty_on && th + m - start < det.duty_period;
-}
-
 // Per (tile, row): bit 1 when the tile belongs to a duty-cycled detector and every mode's t over
 // the tile's node intervals lies inside an off period of its window (modes whose cut lies below
 // the tile's first frequency aside), so that the response is zero over the whole tile: the fused

@@ -8,6 +8,9 @@ memory access and arithmetic operation checked) and -O3 without.  Both must give
 A/E/T to 1e-10 on every bin, including bins where all three link pairs take the reciprocal branch
 of the sinc (|pi f L (1 +- k.n_a)| >= 0.01 for every pair), the case a spilling build of the
 response kernel once returned wrong values for on the GPU.
+
+interval_off, the bound by which k_duty_off marks tiles that a duty cycle's off period covers, is
+checked against brute force the same way (tests/host_kernel/host_duty_off.cpp).
 """
 
 import os
@@ -160,3 +163,26 @@ def test_response_row_under_sanitizers_matches_oracle(binaries, case):
                 g = got[name][p, ci]
                 assert np.linalg.norm(g - r) <= tol * np.linalg.norm(r), (name, p, c)
                 assert np.linalg.norm((g - r)[mask]) <= tol * np.linalg.norm(r[mask]), (name, p, c, "reciprocal bins")
+
+
+def test_duty_off_interval_bound_is_sound(tmp_path):
+    """interval_off (k_duty_off's test that a node interval's t_f + t_c lies in an off period of
+    the window) against brute force over 60,000 random intervals near the edges of three windows
+    (host build under ASan/UBSan): every marked interval sampled at 4001 points with the fused
+    kernel's own t and window has w = 0; a weakened bound (endpoints only, no margin) fails the
+    same sampling on some intervals, so the sampling can tell."""
+    if CXX is None:
+        pytest.skip("no clang++")
+    if not os.path.exists(os.path.join(CSRC, "trig_table.inc")):
+        from pespace_amd import build as pbuild
+        pbuild.gen_trig_inc()
+        pbuild.gen_qnm_inc()
+    exe = str(tmp_path / "host_duty_off")
+    src = os.path.join(ROOT, "tests", "host_kernel", "host_duty_off.cpp")
+    subprocess.run([CXX, "-std=c++17", "-g", "-I", STUB, "-I", CSRC, "-Wno-unknown-attributes", "-O1",
+                    "-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer",
+                    "-o", exe, src], check=True, capture_output=True, text=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    counts = dict(zip(r.stdout.split()[0::2], map(int, r.stdout.split()[1::2])))
+    assert counts["marked"] > 500 and counts["weak_misses"] > 0, counts
