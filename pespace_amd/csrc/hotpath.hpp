@@ -180,15 +180,14 @@ __device__ inline double mode_time(const double* __restrict__ n0, const BinPre& 
   return fma(b.he1, n0[6] - n0[2], fma(b.he2, n0[3], fma(b.he3, n0[7], tc)));
 }
 
-// True when t = t_f + t_c over node interval j of one mode (node entries n: Psi, dPsi/du of
-// nodes j, j + 1 at n[2], n[3], n[6], n[7]; frequencies fa < fb) lies inside one off period of
-// the detector's window.  The fused kernel's t at a bin of the interval is N(s) / (2 pi du f),
-// N(s) = 6 s (1 - s) (Psi1 - Psi0) + du [(1 - s)(1 - 3s) Psi0' + s (3s - 2) Psi1'], a quadratic
-// in s with N(0) = du Psi0', N(1) = du Psi1': bounded by its extremes over [0, 1] times the
-// range of 1 / (2 pi du f).  The margin (1 ms plus 1e-9 of |t|) covers the kernel's rounding of
-// t, so no bin of a flagged interval can see a nonzero window.  NaN entries give false.
-__device__ inline bool interval_off(const DetectorDev& det, const double* __restrict__ n, double fa, double fb,
-                                    double du, double tc) {
+// Bounds of t = t_f + t_c over node interval j of one mode (node entries n: Psi, dPsi/du of
+// nodes j, j + 1 at n[2], n[3], n[6], n[7]; frequencies fa < fb).  The fused kernel's t at a bin
+// of the interval is N(s) / (2 pi du f), N(s) = 6 s (1 - s) (Psi1 - Psi0) + du [(1 - s)(1 - 3s)
+// Psi0' + s (3s - 2) Psi1'], a quadratic in s with N(0) = du Psi0', N(1) = du Psi1': bounded by
+// its extremes over [0, 1] times the range of 1 / (2 pi du f).  The bounds are widened by 1 ms
+// plus 1e-9 of |t|, which covers the kernel's rounding of t.  NaN entries give NaN bounds.
+__device__ inline void interval_t_range(const double* __restrict__ n, double fa, double fb, double du, double tc,
+                                        double& tl, double& th) {
   const double dP = n[6] - n[2], c = du * n[3], d1 = du * n[7];
   const double a = 3.0 * (c + d1) - 6.0 * dP, bq = 6.0 * dP - 4.0 * c - 2.0 * d1;
   double lo = fmin(c, d1), hi = fmax(c, d1);
@@ -201,10 +200,31 @@ __device__ inline bool interval_off(const DetectorDev& det, const double* __rest
     }
   }
   const double ga = 1.0 / (2.0 * PI * du * fa), gb = 1.0 / (2.0 * PI * du * fb);
-  const double tl = tc + fmin(lo * ga, lo * gb), th = tc + fmax(hi * ga, hi * gb);
+  tl = tc + fmin(lo * ga, lo * gb);
+  th = tc + fmax(hi * ga, hi * gb);
   const double m = 1e-3 + 1e-9 * (fabs(tl) + fabs(th));
-  const double start = det.duty_t0 + floor((tl - m - det.duty_t0) * det.duty_inv_period) * det.duty_period;
-  return tl - m - start >= det.duty_on && th + m - start < det.duty_period;
+  tl -= m;
+  th += m;
+}
+
+// The interval's t lies inside one off period of the detector's window (w = 0, no bin of it can
+// see a nonzero window); false for NaN entries
+__device__ inline bool interval_off(const DetectorDev& det, const double* __restrict__ n, double fa, double fb,
+                                    double du, double tc) {
+  double tl, th;
+  interval_t_range(n, fa, fb, du, tc, tl, th);
+  const double start = det.duty_t0 + floor((tl - det.duty_t0) * det.duty_inv_period) * det.duty_period;
+  return tl - start >= det.duty_on && th - start < det.duty_period;
+}
+
+// The interval's t lies on one plateau of the window, clear of its ramps (w = 1 and w' = 0 at
+// every bin of it, so the window changes nothing); false for NaN entries
+__device__ inline bool interval_on(const DetectorDev& det, const double* __restrict__ n, double fa, double fb,
+                                   double du, double tc) {
+  double tl, th;
+  interval_t_range(n, fa, fb, du, tc, tl, th);
+  const double start = det.duty_t0 + floor((tl - det.duty_t0) * det.duty_inv_period) * det.duty_period;
+  return tl - start >= det.duty_taper && th - start <= det.duty_on - det.duty_taper;
 }
 
 template <int PREC, int KIND>
@@ -887,7 +907,9 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
 // file per lane), the forward-only kernel three (168 registers, some spilled; measured 9%
 // faster than two for lnL-only batches when three blocks per CU are launched).
 // DUTY: the launch's tiles belong to duty-cycled detectors (the duty-window tile variants);
-// without, the kernel holds only the two orbit kinds' plain code.
+// without, the kernel holds only the two orbit kinds' plain code.  items (duty-cycled ranges
+// only; null otherwise): the work items of the range that the plain instantiation takes (1) and
+// those the duty one takes (0).
 template <int PREC, bool GRAD, bool DUTY>
 __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* __restrict__ dets, int n_det, int n_tiles,
                                               const double* __restrict__ bin_f, const int* __restrict__ bin_j,
@@ -899,7 +921,8 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
                                               ModeList modes, int P, int rows, const int* __restrict__ tile_jr,
                                               int tile_begin, int tile_end, int group_offset,
                                               int* __restrict__ work_counter, double* __restrict__ partial,
-                                              const unsigned long long* __restrict__ off_bits) {
+                                              const unsigned long long* __restrict__ off_bits,
+                                              const unsigned char* __restrict__ items) {
   constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
   constexpr int CH = chunk_rows<GRAD>();  // 4x the rows without the gradient: 1.3% faster
   __shared__ double acc[TILE / 64][CH][NOUT];
@@ -929,6 +952,12 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
     if (w >= n_work) break;
     const int gl = w / n_chunks, c = w % n_chunks;
     const int g = group_offset + gl;
+    // a duty-cycled range runs twice: items on the window's plateau (items[] = 1, k_duty_items)
+    // here in the plain instantiation, the others in the duty one; each writes only its own
+    if (items && items[(size_t)__builtin_amdgcn_readfirstlane(g * n_chunks + c)] != (DUTY ? 0 : 1)) {
+      __syncthreads();  // every wave has read next_work before it is rewritten
+      continue;
+    }
     const int p0 = c * rows, p1 = min(P, p0 + rows);
     for (int i = threadIdx.x; i < (TILE / 64) * CH * NOUT; i += TILE) (&acc[0][0][0])[i] = 0.0;
     double chunk_cut = 0.0;

@@ -12,7 +12,10 @@ constexpr int BLOCKS_PER_CU = 4;     // ... and persistent blocks launched per C
 constexpr int MAX_MODES = 6;         // modes per context (ModeList)
 constexpr int NJ_STAGE = 30;         // node rows per tile that the fused kernel stages in LDS
 constexpr int BIN_REC = 10;          // doubles per bin of the fused kernel's weight and data record
-constexpr int MAX_TILE_RANGES = 16;  // fused-kernel launches per batch (runs of detectors by duty flag)
+constexpr int MAX_TILE_RANGES = 16;  // runs of detectors by duty flag per context
+// fused-kernel launches per batch: one per range, and a second for a duty-cycled range (its work
+// items on the window's plateau run in the plain instantiation), each with its own work counter
+constexpr int N_WORK_COUNTERS = 2 * MAX_TILE_RANGES;
 
 // ---- per-row constant block ------------------------------------------------------------
 enum PConst {

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(64) void k_setup(const double* __restrict__ params,
                         double* __restrict__ setup, double* __restrict__ pconst, int* __restrict__ work_counter) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p == 0)  // work queues of the fused kernel launches that follow on this stream
-    for (int i = 0; i < MAX_TILE_RANGES; ++i) work_counter[i] = 0;
+    for (int i = 0; i < N_WORK_COUNTERS; ++i) work_counter[i] = 0;
   if (p >= P) return;
   const double* th = params + (size_t)p * PES_N_PARAMS;
   double Mc = th[0], q = th[1], c1 = th[2], c2 = th[3], dL = th[4], phi = th[5], iota = th[6];
@@ -377,6 +377,44 @@ __global__ __launch_bounds__(64) void k_duty_off(const DetectorDev* __restrict__
       if (m != 0ull) pconst[(size_t)p * PC_STRIDE + PC_OFFANY] = 1.0;  // every writer writes 1
     }
   }
+}
+
+// Per work item (tile group x row chunk) of a duty-cycled detectors' tile range: 1 when no
+// (tile, row) of it needs the window: over every mode's node intervals of the tile the window is
+// flat 1 (interval_on; modes whose cut lies below the tile's first frequency aside), or the row
+// evaluates nothing there (the tile above its cuts, an invalid row).  Such items run in the plain
+// instantiation of the fused kernel, whose response and gradient equal the duty one's there; the
+// others in the duty instantiation.  The item index is that of the fused kernel (global group
+// g = group_offset + gl, chunk c: g n_chunks + c).  One block of 64 per item; enable = 0 (no node
+// intervals: direct evaluation at every bin) sends every item to the duty instantiation.
+__global__ __launch_bounds__(64) void k_duty_items(const DetectorDev* __restrict__ dets, int n_det,
+                                                   const double* __restrict__ tile_f0, const int* __restrict__ tile_jr,
+                                                   const double* __restrict__ pconst, const double* __restrict__ nodes,
+                                                   int NNS, double u0, double du, ModeList modes, int P, int rows,
+                                                   int n_chunks, int tile_begin, int tile_end, int tiles_per_group,
+                                                   int group_offset, int enable, unsigned char* __restrict__ items) {
+  const int gl = blockIdx.x / n_chunks, c = blockIdx.x % n_chunks;
+  const int t_first = tile_begin + gl * tiles_per_group, t_end = min(tile_end, t_first + tiles_per_group);
+  const int p0 = c * rows, p1 = min(P, p0 + rows);
+  const int nt = t_end - t_first, np = p1 - p0;
+  int ok = enable;
+  for (int k = threadIdx.x; ok && k < nt * np; k += blockDim.x) {
+    const int tile = t_first + k % nt, p = p0 + k / nt;
+    const double* pc = pconst + (size_t)p * PC_STRIDE;
+    const double f0 = tile_f0[tile];
+    if (pc[PC_VALID] == 0.0 || f0 > pc[PC_FCUT]) continue;
+    const DetectorDev& det = dets[tile_detector(dets, n_det, tile)];
+    const int j0 = tile_jr[2 * tile], j1 = j0 + tile_jr[2 * tile + 1];
+    const double tc = pc[PC_TC];
+    for (int mi = 0; mi < modes.n && ok; ++mi) {
+      if (f0 > pc[PC_MODE + mi * PC_MODE_STRIDE + MF_CUT]) continue;
+      const double* nm = nodes + ((size_t)p * modes.n + mi) * NNS * 4;
+      for (int j = j0; j < j1 && ok; ++j)
+        ok = det.duty && interval_on(det, nm + (size_t)j * 4, exp(u0 + du * j), exp(u0 + du * (j + 1)), du, tc);
+    }
+  }
+  ok = __syncthreads_and(ok);
+  if (threadIdx.x == 0) items[(size_t)(group_offset + gl) * n_chunks + c] = ok ? 1 : 0;
 }
 
 // ---- response / waveform output kernels (parity checks) -------------------------------------
@@ -889,11 +927,11 @@ __global__ void k_whittle_final(const double* __restrict__ work, int nb, double 
       const double *tile_dd, int tiles_per_group, const double *pconst, const double *nodes,                 \
       const double *nodes_grad, int NNS, double du, ModeList modes, int P, int rows, const int *tile_jr,     \
       int tile_begin, int tile_end, int group_offset, int *work_counter, double *partial,                    \
-      const unsigned long long *off_bits
+      const unsigned long long *off_bits, const unsigned char *items
 #define PES_LNL_NAMES                                                                                     \
   dets, n_det, n_tiles, bin_f, bin_j, bin_s, bin_rec, tile_f0, tile_dd, tiles_per_group, pconst, \
       nodes, nodes_grad, NNS, du, modes, P, rows, tile_jr, tile_begin, tile_end, group_offset, work_counter, partial, \
-      off_bits
+      off_bits, items
 
 }  // namespace
 
@@ -1151,16 +1189,17 @@ static int noalloc_base(int device, char** out) {
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // byte offsets of the scratch buffers of a batch of P rows inside the array, and their total
-static size_t noalloc_layout(const pes_context* ctx, int P, bool grad, size_t off[7]) {
+static size_t noalloc_layout(const pes_context* ctx, int P, bool grad, size_t off[8]) {
   const size_t NNS = (size_t)ctx->n_nodes + 1, rows = (size_t)P, nm = (size_t)ctx->modes.n;
   size_t o = 0;
-  off[0] = o; o += align256(MAX_TILE_RANGES * sizeof(int));                // work counters
+  off[0] = o; o += align256(N_WORK_COUNTERS * sizeof(int));                // work counters
   off[1] = o; o += align256(rows * N_SETUP * 5 * sizeof(double));          // setup
   off[2] = o; o += align256(rows * PC_STRIDE * sizeof(double));            // per-row constants
   off[3] = o; o += align256(rows * nm * NNS * 4 * sizeof(double));         // node tables
   off[4] = o; if (grad) o += align256(rows * nm * NNS * 16 * sizeof(double));  // their gradients
   off[5] = o; o += align256((size_t)n_groups_of(ctx) * (grad ? 1 + PES_N_PARAMS : 1) * rows * sizeof(double));
   off[6] = o; if (ctx->any_duty) o += align256(rows * off_words_of(ctx) * sizeof(unsigned long long));  // off tiles
+  off[7] = o; if (ctx->any_duty) o += align256((size_t)n_groups_of(ctx) * rows);  // work items: plain or duty
   return o;
 }
 
@@ -1295,7 +1334,7 @@ extern "C" int pes_context_finalize(pes_context* ctx) {
     HIP_TRY(hipMemcpy(ctx->d_tile_jr, tjr.data(), 2 * (size_t)nt * sizeof(int), hipMemcpyHostToDevice));
   }
   if (ctx->direct) ctx->d_node_f = ctx->d_bin_f;
-  HIP_TRY(hipMalloc(&ctx->d_work_counter, MAX_TILE_RANGES * sizeof(int)));  // one per fused-kernel launch
+  HIP_TRY(hipMalloc(&ctx->d_work_counter, N_WORK_COUNTERS * sizeof(int)));  // one per fused-kernel launch
   {  // the device address of the no-allocation entry point's scratch, once per device
     char* base = nullptr;
     const int rc = noalloc_base(ctx->device, &base);
@@ -1337,9 +1376,21 @@ extern "C" int pes_detector_bins(const pes_context* ctx, int det) {
 
 // (re)allocate scratch for P rows; allocation happens only when P, the gradient flag or the
 // number of tile groups grows
+// The partial-sum buffer ([groups][12][cap] doubles) carries, for duty contexts, one byte per
+// work item (tile group x row chunk) after the sums: whether the item runs in the plain or the
+// duty instantiation (k_duty_items)
+static size_t items_offset_doubles(const pes_context* ctx, size_t cap) {
+  return (size_t)n_groups_of(ctx) * (1 + PES_N_PARAMS) * cap;
+}
+static size_t partial_doubles(const pes_context* ctx, size_t cap) {
+  size_t n = items_offset_doubles(ctx, cap);
+  if (ctx->any_duty) n += ((size_t)n_groups_of(ctx) * cap + 7) / 8;
+  return n;
+}
+
 static int ensure_scratch(pes_context* ctx, int P, bool grad) {
   {
-    const size_t need = (size_t)n_groups_of(ctx) * (1 + PES_N_PARAMS) * (size_t)(P > ctx->cap_P ? P : ctx->cap_P);
+    const size_t need = partial_doubles(ctx, (size_t)(P > ctx->cap_P ? P : ctx->cap_P));
     const bool grow = need > ctx->cap_partial || P > ctx->cap_P || (grad && !ctx->cap_grad);
     if (grow && ctx->pinned)
       return fail(PES_ERR_STATE,
@@ -1380,11 +1431,16 @@ struct BatchScratch {
   double *setup, *pconst, *nodes, *nodes_grad, *partial;
   int* work_counter;
   unsigned long long* off_bits;  // duty contexts only
+  unsigned char* items;          // duty contexts only: [groups][chunks] plain (1) or duty (0)
 };
 
 static BatchScratch context_scratch(const pes_context* ctx) {
-  return {ctx->d_setup, ctx->d_pconst,       ctx->d_nodes,     ctx->d_nodes_grad,
-          ctx->d_partial, ctx->d_work_counter, ctx->d_off_bits};
+  unsigned char* items =
+      ctx->any_duty && ctx->d_partial
+          ? reinterpret_cast<unsigned char*>(ctx->d_partial + items_offset_doubles(ctx, (size_t)ctx->cap_P))
+          : nullptr;
+  return {ctx->d_setup,   ctx->d_pconst,       ctx->d_nodes,    ctx->d_nodes_grad,
+          ctx->d_partial, ctx->d_work_counter, ctx->d_off_bits, items};
 }
 
 static int check_batch_args(const pes_context* ctx, const double* d_params, int P) {
@@ -1443,9 +1499,9 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
   const int rows = rows_per_item(ctx, P, grad, grid);
   const int n_chunks = (P + rows - 1) / rows;
   const int nout = grad ? 1 + PES_N_PARAMS : 1;
-  // one launch per run of detectors with the same duty flag: the duty-cycled detectors' tiles
-  // run the DUTY instantiation, the others the plain one (whose register allocation is that of
-  // a kernel without the window code)
+  // one launch per run of detectors with the same duty flag (two for a duty-cycled run, below):
+  // the duty-cycled detectors' tiles run the DUTY instantiation, the others the plain one (whose
+  // register allocation is that of a kernel without the window code)
   TileRange ranges[MAX_TILE_RANGES];
   const int n_ranges = tile_ranges(ctx, ranges);
   const int off_words = off_words_of(ctx);
@@ -1455,37 +1511,54 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
                        ctx->u0, ctx->du, ctx->modes, P, off_words, ctx->direct ? 0 : 1, sc.off_bits);
     HIP_TRY(hipGetLastError());
   }
+  const int n_det = (int)ctx->hdets.size();
+  // a duty-cycled range's work items on the window's plateau (for every row and mode of the item)
+  // run in the plain instantiation, the others in the duty one: two launches of the range
+  for (int r = 0; r < n_ranges; ++r) {
+    if (!ranges[r].duty) continue;
+    const int n_work =
+        (ranges[r].tile_end - ranges[r].tile_begin + ctx->tiles_per_group - 1) / ctx->tiles_per_group * n_chunks;
+    hipLaunchKernelGGL(k_duty_items, dim3(n_work), dim3(64), 0, st, ctx->d_dets, n_det, ctx->d_tile_f0,
+                       ctx->d_tile_jr, sc.pconst, sc.nodes, NNS, ctx->u0, ctx->du, ctx->modes, P, rows, n_chunks,
+                       ranges[r].tile_begin, ranges[r].tile_end, ctx->tiles_per_group, ranges[r].group_offset,
+                       ctx->direct ? 0 : 1, sc.items);
+    HIP_TRY(hipGetLastError());
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (profile) {
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, st));
   }
-  const int n_det = (int)ctx->hdets.size();
   for (int r = 0; r < n_ranges; ++r) {
     const int tile_begin = ranges[r].tile_begin, tile_end = ranges[r].tile_end;
     const int group_offset = ranges[r].group_offset;
-    const bool duty = ranges[r].duty != 0;
-    int* counter = sc.work_counter + r;
     const int n_work = (tile_end - tile_begin + ctx->tiles_per_group - 1) / ctx->tiles_per_group * n_chunks;
     const int grid_r = std::min(grid, n_work);
+    for (int pass = 0; pass < (ranges[r].duty ? 2 : 1); ++pass) {
+      // pass 0: the plain instantiation (all items of a plain range; the plateau items of a
+      // duty-cycled one), pass 1: the duty instantiation (the other items)
+      const bool duty = pass == 1;
+      const unsigned char* items = ranges[r].duty ? sc.items : nullptr;
+      int* counter = sc.work_counter + (pass == 0 ? r : MAX_TILE_RANGES + r);
 #define PES_LNL_ARGS                                                                                         \
   ctx->d_dets, n_det, ctx->n_tiles, ctx->d_bin_f, ctx->d_bin_j, ctx->d_bin_s, ctx->d_bin_rec,                 \
       ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, sc.pconst, sc.nodes, sc.nodes_grad, NNS, ctx->du, \
       ctx->modes, P, rows, ctx->d_tile_jr, tile_begin, tile_end, group_offset, counter, sc.partial,       \
-      sc.off_bits
-    if (!grad)
-      HIP_TRY(launch_lnl_fwd(precision, duty, grid_r, st, PES_LNL_ARGS));
-    else if (precision == 0 && !duty)
-      hipLaunchKernelGGL((k_lnl<0, true, false>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
-    else if (precision == 0)
-      hipLaunchKernelGGL((k_lnl<0, true, true>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
-    else if (!duty)
-      hipLaunchKernelGGL((k_lnl<1, true, false>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
-    else
-      hipLaunchKernelGGL((k_lnl<1, true, true>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
+      sc.off_bits, items
+      if (!grad)
+        HIP_TRY(launch_lnl_fwd(precision, duty, grid_r, st, PES_LNL_ARGS));
+      else if (precision == 0 && !duty)
+        hipLaunchKernelGGL((k_lnl<0, true, false>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
+      else if (precision == 0)
+        hipLaunchKernelGGL((k_lnl<0, true, true>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
+      else if (!duty)
+        hipLaunchKernelGGL((k_lnl<1, true, false>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
+      else
+        hipLaunchKernelGGL((k_lnl<1, true, true>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
 #undef PES_LNL_ARGS
-    HIP_TRY(hipGetLastError());
+      HIP_TRY(hipGetLastError());
+    }
   }
   if (profile) {
     HIP_TRY(hipEventRecord(e1, st));
@@ -1502,7 +1575,7 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
 
 extern "C" size_t pes_lnl_batch_noalloc_bytes(const pes_context* ctx, int P, int grad) {
   if (!ctx || !ctx->finalized || P < 1) return 0;
-  size_t off[7];
+  size_t off[8];
   return noalloc_layout(ctx, P, grad != 0, off);
 }
 
@@ -1514,7 +1587,7 @@ extern "C" int pes_lnl_batch_noalloc(pes_context* ctx, const double* d_params, i
   int rc = check_batch_args(ctx, d_params, P);
   if (rc != PES_OK || P == 0) return rc;
   const bool grad = d_grad != nullptr;
-  size_t off[7];
+  size_t off[8];
   const size_t need = noalloc_layout(ctx, P, grad, off);
   if (need > (size_t)PES_NOALLOC_SCRATCH_BYTES) {
     char msg[256];
@@ -1536,6 +1609,7 @@ extern "C" int pes_lnl_batch_noalloc(pes_context* ctx, const double* d_params, i
   sc.nodes_grad = grad ? reinterpret_cast<double*>(base + off[4]) : nullptr;
   sc.partial = reinterpret_cast<double*>(base + off[5]);
   sc.off_bits = ctx->any_duty ? reinterpret_cast<unsigned long long*>(base + off[6]) : nullptr;
+  sc.items = ctx->any_duty ? reinterpret_cast<unsigned char*>(base + off[7]) : nullptr;
   hipStream_t st = (hipStream_t)stream;
   rc = launch_setup_nodes(ctx, sc, d_params, P, grad, st);
   if (rc != PES_OK) return rc;

@@ -9,8 +9,9 @@ A/E/T to 1e-10 on every bin, including bins where all three link pairs take the 
 of the sinc (|pi f L (1 +- k.n_a)| >= 0.01 for every pair), the case a spilling build of the
 response kernel once returned wrong values for on the GPU.
 
-interval_off, the bound by which k_duty_off marks tiles that a duty cycle's off period covers, is
-checked against brute force the same way (tests/host_kernel/host_duty_off.cpp).
+interval_off and interval_on, the bounds by which the kernels find tiles that a duty cycle's off
+periods or plateaus cover, are checked against brute force the same way
+(tests/host_kernel/host_duty_off.cpp).
 """
 
 import os
@@ -165,11 +166,12 @@ def test_response_row_under_sanitizers_matches_oracle(binaries, case):
                 assert np.linalg.norm((g - r)[mask]) <= tol * np.linalg.norm(r[mask]), (name, p, c, "reciprocal bins")
 
 
-def test_duty_off_interval_bound_is_sound(tmp_path):
-    """interval_off (k_duty_off's test that a node interval's t_f + t_c lies in an off period of
-    the window) against brute force over 60,000 random intervals near the edges of three windows
-    (host build under ASan/UBSan): every marked interval sampled at 4001 points with the fused
-    kernel's own t and window has w = 0; a weakened bound (endpoints only, no margin) fails the
+def test_duty_interval_bounds_are_sound(tmp_path):
+    """interval_off and interval_on (whether a node interval's t_f + t_c lies in an off period, or
+    on a plateau clear of the ramps, of the window: k_duty_off, k_duty_items) against brute force
+    over 60,000 random intervals near the edges and ramp ends of three windows (host build under
+    ASan/UBSan): every marked interval sampled at 4001 points with the fused kernel's own t and
+    window has w = 0 (w = 1 and w' = 0); weakened bounds (endpoints only, no margin) fail the
     same sampling on some intervals, so the sampling can tell."""
     if CXX is None:
         pytest.skip("no clang++")
@@ -185,4 +187,5 @@ def test_duty_off_interval_bound_is_sound(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     counts = dict(zip(r.stdout.split()[0::2], map(int, r.stdout.split()[1::2])))
-    assert counts["marked"] > 500 and counts["weak_misses"] > 0, counts
+    assert counts["marked"] > 500 and counts["marked_on"] > 500, counts
+    assert counts["weak_misses"] > 0 and counts["weak_on_misses"] > 0, counts
