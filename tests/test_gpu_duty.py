@@ -214,3 +214,50 @@ def test_duty_off_period_rows_skip_exactly():
             core.profile_work()
     finally:
         core.set_profiling(False)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_duty_graph_capture_matches_eager(precision):
+    """A batch on a duty-cycled detector (off-period flags, plateau work items, and the range's two
+    launches) captured in a HIP graph: every replay with new rows written into the captured input
+    equals the eager result bitwise, for rows on the plateau, on the ramps and in the off period."""
+    import torch
+
+    from pespace_amd._lib import PespaceError
+    case = dict(CASES["tianqin"])
+    modes = case.pop("modes")
+    d, labels = make_detector(modes=modes, **case)
+    core = _core(modes, [(d, labels)])
+    P = 12
+    batches = []
+    for k in range(3):
+        rows = np.vstack([inj_row(), particle_cloud(P - 1, seed=40 + k)])
+        rows[1:4, 10] = 300000.0 + 1000.0 * np.arange(3) + k  # off period
+        rows[8:, 10] = 5.0e6 + 1000.0 * np.arange(4) + k      # on the plateau, far from the ramps
+        batches.append(rows)
+    # 12 rows give work items of 8 rows: rows 0-7 (ramps and off period) run in the duty
+    # instantiation, rows 8-11 (plateau) in the plain one
+    x = torch.as_tensor(batches[0], device="cuda").contiguous()
+    lnl = torch.empty(P, dtype=torch.float64, device="cuda")
+    g = torch.empty(P, 11, dtype=torch.float64, device="cuda")
+    core.reserve(P, grad=True)
+    core.pin_scratch(True)
+    try:
+        core.lnl(x, grad=True, precision=precision, out=lnl, grad_out=g)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            core.lnl(x, grad=True, precision=precision, out=lnl, grad_out=g)
+        for rows in batches[1:] + batches[:1]:
+            x.copy_(torch.as_tensor(rows, device="cuda"))
+            graph.replay()
+            torch.cuda.synchronize()
+            l_e, g_e = core.lnl(rows, grad=True, precision=precision)
+            torch.cuda.synchronize()
+            assert torch.equal(lnl, l_e) and torch.equal(g, g_e)
+            np.testing.assert_allclose(l_e.cpu().numpy(), olk.lnl_batch(rows, [d], modes, 0.0, INTERP), rtol=1e-10)
+        with pytest.raises(PespaceError, match="pinned"):
+            core.lnl(np.vstack(batches), grad=True, precision=precision)
+        del graph
+    finally:
+        core.pin_scratch(False)
