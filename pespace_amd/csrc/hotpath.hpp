@@ -906,11 +906,12 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
 // Launch bounds: the gradient kernel runs two waves per SIMD (256 registers of the unified
 // file per lane), the forward-only kernel three (168 registers, some spilled; measured 9%
 // faster than two for lnL-only batches when three blocks per CU are launched).
-// DUTY: the launch's tiles belong to duty-cycled detectors (the duty-window tile variants);
-// without, the kernel holds only the two orbit kinds' plain code.  items (duty-cycled ranges
-// only; null otherwise): the work items of the range that the plain instantiation takes (1) and
-// those the duty one takes (0).
-template <int PREC, bool GRAD, bool DUTY>
+// KD: the launch's tiles belong to detectors of one orbit kind (bit 0: heliocentric 0,
+// geocentric 1), duty-cycled (bit 1, the duty-window tile variants) or not; each instantiation
+// holds that one variant's code (a kernel holding both orbit kinds spilled 106 instead of 59-74
+// VGPRs).  items (duty-cycled ranges only; null otherwise): the work items of the range that the
+// plain instantiation takes (1) and those the duty one takes (0).
+template <int PREC, bool GRAD, int KD>
 __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* __restrict__ dets, int n_det, int n_tiles,
                                               const double* __restrict__ bin_f, const int* __restrict__ bin_j,
                                               const double* __restrict__ bin_s, const double* __restrict__ bin_rec,
@@ -937,11 +938,12 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
   const int wave = GRAD ? (int)(threadIdx.x >> 6) : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // rows (1 <= rows <= CH) per work item, chosen by the host.  This launch covers the tiles
   // [tile_begin, tile_end) in groups of tiles_per_group, whose partial sums go to groups
-  // group_offset + 0, 1, ...: a network with duty-cycled detectors runs their tiles in the DUTY
-  // instantiation and the others' in the plain one, one launch (and counter) per tile range
+  // group_offset + 0, 1, ...: one launch (and counter) per tile range of one orbit kind and duty
+  // flag, two for a duty-cycled range
   const int n_chunks = (P + rows - 1) / rows;
   const int n_groups_r = (tile_end - tile_begin + tiles_per_group - 1) / tiles_per_group;
   const int n_work = n_chunks * n_groups_r;
+  constexpr bool DUTY = kDuty<KD>;
   // work items are taken from a counter (zeroed by k_setup earlier on the same stream), so
   // blocks that draw cheap items (tiles above the mode cuts) take more of them; every block
   // leaves once the counter passes n_work
@@ -979,20 +981,12 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
 #define PES_TILE_ARGS                                                                                         \
   det, tile, f0, p0, p1, bin_f, bin_j, bin_s, bin_rec, tile_dd, pconst, nodes, nodes_grad, NNS, du, \
       modes, lane, wave, &acc[0][0][0], j0, nj, stage_n, &stage_g[0], buf, off_bits, n_tiles
-      // the host's tile ranges give this launch only duty-cycled detectors (DUTY) or only
-      // detectors without (kind bit 1 = DUTY)
-      constexpr int KD = DUTY ? 2 : 0;
-      if (nj <= NJ_STAGE) {
-        if (det.kind == 0)
-          tile_rows<PREC, GRAD, 0 | KD, true>(PES_TILE_ARGS);
-        else
-          tile_rows<PREC, GRAD, 1 | KD, true>(PES_TILE_ARGS);
-      } else {
-        if (det.kind == 0)
-          tile_rows<PREC, GRAD, 0 | KD, false>(PES_TILE_ARGS);
-        else
-          tile_rows<PREC, GRAD, 1 | KD, false>(PES_TILE_ARGS);
-      }
+      // the host's tile ranges give this launch detectors of one orbit kind (KD bit 0), all
+      // duty-cycled or none (bit 1)
+      if (nj <= NJ_STAGE)
+        tile_rows<PREC, GRAD, KD, true>(PES_TILE_ARGS);
+      else
+        tile_rows<PREC, GRAD, KD, false>(PES_TILE_ARGS);
 #undef PES_TILE_ARGS
     }
     __syncthreads();

@@ -940,17 +940,19 @@ __global__ void k_whittle_final(const double* __restrict__ work, int nb, double 
 // scheduled with their own compiler options (three waves per SIMD against two for the
 // gradient kernel).  This launcher is that unit's only export.
 namespace pes {
-hipError_t launch_lnl_fwd(int precision, bool duty, int grid, hipStream_t st, PES_LNL_PARAMS);
+hipError_t launch_lnl_fwd(int precision, int kd, int grid, hipStream_t st, PES_LNL_PARAMS);
 }
 
 #ifdef PES_LNL_FWD_UNIT
-hipError_t pes::launch_lnl_fwd(int precision, bool duty, int grid, hipStream_t st, PES_LNL_PARAMS) {
+hipError_t pes::launch_lnl_fwd(int precision, int kd, int grid, hipStream_t st, PES_LNL_PARAMS) {
   // lnL alone is the FP64 forward in every precision mode (the modes differ in the adjoint only)
   (void)precision;
-  if (duty)
-    hipLaunchKernelGGL((k_lnl<0, false, true>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES);
-  else
-    hipLaunchKernelGGL((k_lnl<0, false, false>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES);
+  switch (kd) {
+    case 0: hipLaunchKernelGGL((k_lnl<0, false, 0>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES); break;
+    case 1: hipLaunchKernelGGL((k_lnl<0, false, 1>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES); break;
+    case 2: hipLaunchKernelGGL((k_lnl<0, false, 2>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES); break;
+    default: hipLaunchKernelGGL((k_lnl<0, false, 3>), dim3(grid), dim3(TILE), 0, st, PES_LNL_NAMES); break;
+  }
   return hipGetLastError();
 }
 #else
@@ -1131,16 +1133,19 @@ extern "C" int pes_context_set_duty_cycle(pes_context* ctx, int det_index, doubl
 // each split into groups of tiles_per_group tiles (a group never straddles two runs), with the
 // partial sums of run r's groups at group_offset(r) + 0, 1, ...
 struct TileRange {
-  int tile_begin, tile_end, duty, group_offset;
+  int tile_begin, tile_end, duty, kind, group_offset;
 };
+// runs of consecutive detectors with the same orbit kind and duty flag: each is launched in the
+// fused kernel's instantiation for that kind (only its code, so that its register allocation is
+// not that of the other kind's code as well)
 static int tile_ranges(const pes_context* ctx, TileRange* out) {
   int n = 0, groups = 0;
   for (const auto& h : ctx->hdets) {
-    const int t0 = h.d.bin_offset / TILE, t1 = t0 + h.d.n_tiles, duty = h.d.duty ? 1 : 0;
-    if (n > 0 && out[n - 1].duty == duty && out[n - 1].tile_end == t0) {
+    const int t0 = h.d.bin_offset / TILE, t1 = t0 + h.d.n_tiles, duty = h.d.duty ? 1 : 0, kind = h.d.kind;
+    if (n > 0 && out[n - 1].duty == duty && out[n - 1].kind == kind && out[n - 1].tile_end == t0) {
       out[n - 1].tile_end = t1;
     } else {
-      out[n++] = {t0, t1, duty, 0};
+      out[n++] = {t0, t1, duty, kind, 0};
     }
   }
   for (int r = 0; r < n; ++r) {
@@ -1211,13 +1216,17 @@ extern "C" int pes_context_finalize(pes_context* ctx) {
   if (!ctx) return fail(PES_ERR_INVALID, "ctx is NULL");
   if (ctx->finalized) return fail(PES_ERR_STATE, "context already finalized");
   if (ctx->hdets.empty()) return fail(PES_ERR_STATE, "no detector added");
-  {  // runs of consecutive detectors with the same duty flag: one fused-kernel launch each
+  {  // runs of consecutive detectors with the same orbit kind and duty flag: a fused-kernel launch each
     int runs = 0;
-    for (size_t i = 0; i < ctx->hdets.size(); ++i)
-      runs += (i == 0 || (ctx->hdets[i].d.duty != 0) != (ctx->hdets[i - 1].d.duty != 0)) ? 1 : 0;
+    for (size_t i = 0; i < ctx->hdets.size(); ++i) {
+      const auto& a = ctx->hdets[i].d;
+      runs += (i == 0 || (a.duty != 0) != (ctx->hdets[i - 1].d.duty != 0) || a.kind != ctx->hdets[i - 1].d.kind)
+                  ? 1
+                  : 0;
+    }
     if (runs > MAX_TILE_RANGES)
-      return fail(PES_ERR_INVALID, "too many alternations of duty-cycled and continuous detectors (order the "
-                                   "detectors so that at most 16 runs of either remain)");
+      return fail(PES_ERR_INVALID, "too many alternations of orbit kinds or of duty-cycled and continuous "
+                                   "detectors (order the detectors so that at most 16 runs remain)");
   }
   HIP_TRY(hipSetDevice(ctx->device));
   long long total = 0;
@@ -1537,8 +1546,9 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
     const int grid_r = std::min(grid, n_work);
     for (int pass = 0; pass < (ranges[r].duty ? 2 : 1); ++pass) {
       // pass 0: the plain instantiation (all items of a plain range; the plateau items of a
-      // duty-cycled one), pass 1: the duty instantiation (the other items)
-      const bool duty = pass == 1;
+      // duty-cycled one), pass 1: the duty instantiation (the other items); both for the range's
+      // orbit kind only
+      const int kd = ranges[r].kind | (pass == 1 ? 2 : 0);
       const unsigned char* items = ranges[r].duty ? sc.items : nullptr;
       int* counter = sc.work_counter + (pass == 0 ? r : MAX_TILE_RANGES + r);
 #define PES_LNL_ARGS                                                                                         \
@@ -1546,16 +1556,25 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
       ctx->d_tile_f0, ctx->d_tile_dd, ctx->tiles_per_group, sc.pconst, sc.nodes, sc.nodes_grad, NNS, ctx->du, \
       ctx->modes, P, rows, ctx->d_tile_jr, tile_begin, tile_end, group_offset, counter, sc.partial,       \
       sc.off_bits, items
-      if (!grad)
-        HIP_TRY(launch_lnl_fwd(precision, duty, grid_r, st, PES_LNL_ARGS));
-      else if (precision == 0 && !duty)
-        hipLaunchKernelGGL((k_lnl<0, true, false>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
-      else if (precision == 0)
-        hipLaunchKernelGGL((k_lnl<0, true, true>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
-      else if (!duty)
-        hipLaunchKernelGGL((k_lnl<1, true, false>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
-      else
-        hipLaunchKernelGGL((k_lnl<1, true, true>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS);
+#define PES_LNL_GRAD(PR, KD) hipLaunchKernelGGL((k_lnl<PR, true, KD>), dim3(grid_r), dim3(TILE), 0, st, PES_LNL_ARGS)
+      if (!grad) {
+        HIP_TRY(launch_lnl_fwd(precision, kd, grid_r, st, PES_LNL_ARGS));
+      } else if (precision == 0) {
+        switch (kd) {
+          case 0: PES_LNL_GRAD(0, 0); break;
+          case 1: PES_LNL_GRAD(0, 1); break;
+          case 2: PES_LNL_GRAD(0, 2); break;
+          default: PES_LNL_GRAD(0, 3); break;
+        }
+      } else {
+        switch (kd) {
+          case 0: PES_LNL_GRAD(1, 0); break;
+          case 1: PES_LNL_GRAD(1, 1); break;
+          case 2: PES_LNL_GRAD(1, 2); break;
+          default: PES_LNL_GRAD(1, 3); break;
+        }
+      }
+#undef PES_LNL_GRAD
 #undef PES_LNL_ARGS
       HIP_TRY(hipGetLastError());
     }
