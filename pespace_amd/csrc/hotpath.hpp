@@ -944,6 +944,9 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
   const int n_groups_r = (tile_end - tile_begin + tiles_per_group - 1) / tiles_per_group;
   const int n_work = n_chunks * n_groups_r;
   constexpr bool DUTY = kDuty<KD>;
+  // the duty pass of a range whose items all run in the plain one: k_duty_items counted none at
+  // work_counter[MAX_TILE_RANGES] (every block leaves before its first barrier)
+  if (DUTY && items && work_counter[MAX_TILE_RANGES] == 0) return;
   // work items are taken from a counter (zeroed by k_setup earlier on the same stream), so
   // blocks that draw cheap items (tiles above the mode cuts) take more of them; every block
   // leaves once the counter passes n_work

@@ -14,8 +14,9 @@ constexpr int NJ_STAGE = 30;         // node rows per tile that the fused kernel
 constexpr int BIN_REC = 10;          // doubles per bin of the fused kernel's weight and data record
 constexpr int MAX_TILE_RANGES = 16;  // runs of detectors by duty flag per context
 // fused-kernel launches per batch: one per range, and a second for a duty-cycled range (its work
-// items on the window's plateau run in the plain instantiation), each with its own work counter
-constexpr int N_WORK_COUNTERS = 2 * MAX_TILE_RANGES;
+// items on the window's plateau run in the plain instantiation), each with its own work counter;
+// a third set counts a duty-cycled range's items that need the window (k_duty_items)
+constexpr int N_WORK_COUNTERS = 3 * MAX_TILE_RANGES;
 
 // ---- per-row constant block ------------------------------------------------------------
 enum PConst {

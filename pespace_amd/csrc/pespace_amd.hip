@@ -392,7 +392,8 @@ __global__ __launch_bounds__(64) void k_duty_items(const DetectorDev* __restrict
                                                    const double* __restrict__ pconst, const double* __restrict__ nodes,
                                                    int NNS, double u0, double du, ModeList modes, int P, int rows,
                                                    int n_chunks, int tile_begin, int tile_end, int tiles_per_group,
-                                                   int group_offset, int enable, unsigned char* __restrict__ items) {
+                                                   int group_offset, int enable, unsigned char* __restrict__ items,
+                                                   int* __restrict__ n_duty_items) {
   const int gl = blockIdx.x / n_chunks, c = blockIdx.x % n_chunks;
   const int t_first = tile_begin + gl * tiles_per_group, t_end = min(tile_end, t_first + tiles_per_group);
   const int p0 = c * rows, p1 = min(P, p0 + rows);
@@ -414,7 +415,10 @@ __global__ __launch_bounds__(64) void k_duty_items(const DetectorDev* __restrict
     }
   }
   ok = __syncthreads_and(ok);
-  if (threadIdx.x == 0) items[(size_t)(group_offset + gl) * n_chunks + c] = ok ? 1 : 0;
+  if (threadIdx.x == 0) {
+    items[(size_t)(group_offset + gl) * n_chunks + c] = ok ? 1 : 0;
+    if (!ok) atomicAdd(n_duty_items, 1);  // the range's duty pass ends at once when none need it
+  }
 }
 
 // ---- response / waveform output kernels (parity checks) -------------------------------------
@@ -1530,7 +1534,7 @@ static int launch_lnl_reduce(pes_context* ctx, const BatchScratch& sc, int P, do
     hipLaunchKernelGGL(k_duty_items, dim3(n_work), dim3(64), 0, st, ctx->d_dets, n_det, ctx->d_tile_f0,
                        ctx->d_tile_jr, sc.pconst, sc.nodes, NNS, ctx->u0, ctx->du, ctx->modes, P, rows, n_chunks,
                        ranges[r].tile_begin, ranges[r].tile_end, ctx->tiles_per_group, ranges[r].group_offset,
-                       ctx->direct ? 0 : 1, sc.items);
+                       ctx->direct ? 0 : 1, sc.items, sc.work_counter + 2 * MAX_TILE_RANGES + r);
     HIP_TRY(hipGetLastError());
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
