@@ -374,8 +374,8 @@ def main():
     elif not args.no_autotune:
         # schedule chosen by the autotuner on a 512-row sample of the batch (untimed)
         t0 = time.perf_counter()
-        # rows per work item: 0 = the kernel's chunk (40 with the gradient, 160 without), or fixed
-        rows_c = (0, 20, 10) if grad else (0, 80, 40)
+        # rows per work item: 0 = the kernel's chunk (64 with the gradient, 160 without), or fixed
+        rows_c = (0, 40, 20) if grad else (0, 80, 40)
         best, times = core.autotune(params[:min(512, args.particles)], grad=grad,
                                     precision="fp64" if args.precision == "all" else args.precision,
                                     candidates=[(t, b, r) for t in (8, 16, 32) for b in (2, 3, 4) for r in rows_c],

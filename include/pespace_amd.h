@@ -213,7 +213,7 @@ int pes_context_pin_scratch(pes_context* ctx, int pinned);
 int pes_whittle_lnl(const double* d_data, const double* d_resp, const double* d_psd, int n_chan, int n, double df,
                     double* d_work, double* d_out, void* stream);
 
-/* Schedule of the fused kernel: work items of tiles_per_group 256-bin tiles x 40 rows (160 for
+/* Schedule of the fused kernel: work items of tiles_per_group 256-bin tiles x 64 rows (160 for
  * lnL without gradient), taken from a queue by blocks_per_cu x (number of CUs) persistent
  * blocks of 256 threads.  Defaults
  * 32 and 4; pespace_amd.core.LikelihoodCore.autotune picks them by timing.  Changing the
@@ -224,7 +224,7 @@ int pes_context_set_schedule(pes_context* ctx, int tiles_per_group, int blocks_p
 int pes_context_get_schedule(const pes_context* ctx, int* tiles_per_group, int* blocks_per_cu);
 
 /* Parameter rows per work item of the fused kernel (the rows a block evaluates over one tile
- * group): 0 (the default) picks them per batch (the kernel's chunk, 40 with the gradient and 160
+ * group): 0 (the default) picks them per batch (the kernel's chunk, 64 with the gradient and 160
  * without, fewer when the grid has too few tile groups); 1..160 fixes them (capped at the
  * kernel's chunk).  A tunable of the project's autotuner, beside the tile-group size and the
  * blocks per CU; no counterpart in the reference. */
