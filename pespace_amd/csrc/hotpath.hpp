@@ -980,7 +980,8 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
     const int g = group_offset + gl;
     // a duty-cycled range runs twice: items on the window's plateau (items[] = 1, k_duty_items)
     // here in the plain instantiation, the others in the duty one; each writes only its own
-    if (items && items[(size_t)__builtin_amdgcn_readfirstlane(g * n_chunks + c)] != (DUTY ? 0 : 1)) {
+    if (items && items[(size_t)__builtin_amdgcn_readfirstlane(g) * n_chunks + __builtin_amdgcn_readfirstlane(c)] !=
+                     (DUTY ? 0 : 1)) {
       __syncthreads();  // every wave has read next_work before it is rewritten
       continue;
     }
