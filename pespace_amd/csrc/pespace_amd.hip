@@ -756,12 +756,12 @@ __global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Ext
     return f * fma(Pp[c], sp[s] - rp[r], Pc[c] * (sc[s] - rc[r]));
   };
   const size_t N = (size_t)n;
-  out[i] = link(0, 1, 2, up[2]);          // 12
-  out[N + i] = link(1, 0, 2, dn[2]);      // 21
-  out[2 * N + i] = link(1, 2, 0, up[0]);  // 23
-  out[3 * N + i] = link(2, 1, 0, dn[0]);  // 32
-  out[4 * N + i] = link(2, 0, 1, up[1]);  // 31
-  out[5 * N + i] = link(0, 2, 1, dn[1]);  // 13
+  __builtin_nontemporal_store(link(0, 1, 2, up[2]), out + i);
+  __builtin_nontemporal_store(link(1, 0, 2, dn[2]), out + N + i);
+  __builtin_nontemporal_store(link(1, 2, 0, up[0]), out + 2 * N + i);
+  __builtin_nontemporal_store(link(2, 1, 0, dn[0]), out + 3 * N + i);
+  __builtin_nontemporal_store(link(2, 0, 1, up[1]), out + 4 * N + i);
+  __builtin_nontemporal_store(link(0, 2, 1, dn[1]), out + 5 * N + i);
 }
 
 // Integer and fractional parts of the delays m L / dt on the extended link grid: D^m y at output
@@ -790,13 +790,13 @@ __host__ __device__ constexpr double td_coef(int c, int l, int m) {
 template <bool ORTH>
 __device__ inline void td_store(double* __restrict__ out, size_t N, int i, double X, double Y, double Z) {
   if (ORTH) {
-    out[i] = (Z - X) / sqrt(2.0);
-    out[N + i] = (X - 2.0 * Y + Z) / sqrt(6.0);
-    out[2 * N + i] = (X + Y + Z) / sqrt(3.0);
+    __builtin_nontemporal_store((Z - X) / sqrt(2.0), out + i);
+    __builtin_nontemporal_store((X - 2.0 * Y + Z) / sqrt(6.0), out + N + i);
+    __builtin_nontemporal_store((X + Y + Z) / sqrt(3.0), out + 2 * N + i);
   } else {
-    out[i] = X;
-    out[N + i] = Y;
-    out[2 * N + i] = Z;
+    __builtin_nontemporal_store(X, out + i);
+    __builtin_nontemporal_store(Y, out + N + i);
+    __builtin_nontemporal_store(Z, out + 2 * N + i);
   }
 }
 
