@@ -15,6 +15,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import h5file
 from .noise import FrequencyDomainNoiseModel, available_noise_models
 
 BACKENDS = ("hip",)
@@ -24,15 +25,6 @@ _DATA_INFO_KEYS = ("channels", "duration", "delta_time", "start_time", "minimum_
                    "sampling_frequency", "delta_frequency", "time_series_length", "time_samples_array",
                    "full_frequency_series_length", "full_frequency_samples_array", "frequency_mask_array",
                    "frequency_samples_array", "frequency_series_length")
-
-
-def _require_h5py():
-    try:
-        import h5py
-    except ImportError as error:
-        raise ImportError("TDIChannelData files are HDF5 (the reference's layout); h5py is required to read or "
-                          "write them and is not installed") from error
-    return h5py
 
 
 def _text(v):
@@ -307,7 +299,8 @@ class TDIChannelData:
     # The reference's HDF5 layout: class_attrs/{scaling, backend, _np_fp, _np_cp},
     # data_info/<every DataInformation field>, tdi_data/<domain>/<channel>,
     # noise_data/<kind>/<channel>.  The layout logic works on a flat {path: value} tree;
-    # save_to_file / recover_from_file move that tree through h5py.
+    # save_to_file / recover_from_file move that tree through h5file (h5py, or the HDF5 C library
+    # with h5py's datatypes).
 
     def _file_tree(self):
         """The datasets save_to_file writes, as {path: value} (tdi.py:1039-1088)."""
@@ -374,22 +367,12 @@ class TDIChannelData:
 
     def save_to_file(self, filename: str) -> None:
         """tdi.py:1022-1088: the instance as an HDF5 file (fails if the file exists)."""
-        h5py = _require_h5py()
-        with h5py.File(filename, "x") as fh:
-            for path, value in self._file_tree().items():
-                if isinstance(value, str) or (isinstance(value, np.ndarray) and value.dtype.kind == "U"):
-                    value = np.array(value, dtype=h5py.string_dtype(encoding="utf-8"))
-                fh.create_dataset(path, data=value)
+        h5file.write_tree(filename, self._file_tree())
 
     @staticmethod
     def recover_from_file(filename: str) -> "TDIChannelData":
         """tdi.py:1090-1292: an instance from a file written by save_to_file."""
-        h5py = _require_h5py()
-        tree = {}
-        with h5py.File(filename, "r") as fh:
-            fh.visititems(lambda name, obj: tree.__setitem__(name, obj[()]) if isinstance(obj, h5py.Dataset)
-                          else None)
-        return TDIChannelData._from_file_tree(tree, filename)
+        return TDIChannelData._from_file_tree(h5file.read_tree(filename), filename)
 
     @property
     def fd_data_numpy(self):

@@ -1,19 +1,70 @@
-"""TDIChannelData files (tdi.py:1022-1292): the reference's HDF5 dataset layout and its
-consistency checks on recovery.
+"""TDIChannelData files (tdi.py:1022-1292): the reference's HDF5 dataset layout, its
+consistency checks on recovery, and the files themselves.
 
-h5py is not installed in this image, so the layout is tested through the dataset tree that
-save_to_file writes and recover_from_file reads (paths -> arrays); the HDF5 round trip itself
-runs only where h5py exists, and otherwise the file methods must fail with a clear ImportError.
+The layout logic is tested on the dataset tree that save_to_file writes and recover_from_file
+reads (paths -> arrays).  The files go through pespace_amd.detector.h5file: h5py where it is
+importable, else the HDF5 C library through ctypes.  Where an interpreter with h5py is at hand
+(PESPACE_H5PY_PYTHON, else a conda installation's), h5py itself checks the files both ways
+(tests/h5py_crosscheck.py in that interpreter): it reads ours with the dtypes it writes itself
+(bool, <f8, <i8, <c16, UTF-8 variable-length strings), and ours reads a file it writes the way
+tdi.py:1049-1088 does; h5dump's datatype, dataspace and storage description of the two files is
+the same apart from the data offsets.
 """
 
-import importlib.util
+import functools
+import json
+import os
+import shutil
+import subprocess
+import sys
 
 import numpy as np
 import pytest
 
+from pespace_amd.detector import h5file
 from pespace_amd.detector.tdi import TDIChannelData
 
-HAS_H5PY = importlib.util.find_spec("h5py") is not None
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _has_hdf5():
+    try:
+        h5file.backend()
+    except ImportError:
+        return False
+    return True
+
+
+HAS_HDF5 = _has_hdf5()
+
+
+@functools.lru_cache(maxsize=None)
+def _h5py_python():
+    """An interpreter other than this one that imports h5py, or None."""
+    cands = [os.environ.get("PESPACE_H5PY_PYTHON"), "/opt/conda/bin/python3.9", "/opt/conda/bin/python3"]
+    for exe in cands:
+        if not exe or not os.path.exists(exe) or os.path.realpath(exe) == os.path.realpath(sys.executable):
+            continue
+        r = subprocess.run([exe, "-c", "import h5py, numpy"], env=_clean_env(), capture_output=True, timeout=120)
+        if r.returncode == 0:
+            return exe
+    return None
+
+
+def _clean_env():
+    return {"PATH": os.environ.get("PATH", "/usr/bin:/bin"), "HOME": os.environ.get("HOME", "/tmp"),
+            "PYTHONNOUSERSITE": "1"}
+
+
+def _h5py(*args):
+    exe = _h5py_python()
+    r = subprocess.run([exe, os.path.join(HERE, "h5py_crosscheck.py"), *map(str, args)], env=_clean_env(),
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+
+
+need_h5py = pytest.mark.skipif("not _h5py_python()", reason="no interpreter with h5py")
+need_hdf5 = pytest.mark.skipif(not HAS_HDF5, reason="neither h5py nor the HDF5 C library is available")
 
 
 def _data():
@@ -120,21 +171,99 @@ def test_explicit_grid_has_no_file_layout():
         td._file_tree()
 
 
-@pytest.mark.skipif(HAS_H5PY, reason="h5py is installed")
-def test_file_methods_need_h5py(tmp_path):
-    with pytest.raises(ImportError, match="h5py"):
+def test_file_methods_need_hdf5(tmp_path, monkeypatch):
+    """Without h5py and without the C library the file methods fail with a clear ImportError."""
+    monkeypatch.setenv("PESPACE_HDF5_LIBRARY", str(tmp_path / "missing" / "libhdf5.so"))
+    monkeypatch.setattr(h5file, "_CLIB", None)
+    monkeypatch.setattr(h5file, "_h5py", lambda: None)
+    with pytest.raises(ImportError, match="PESPACE_HDF5_LIBRARY"):
         _data().save_to_file(str(tmp_path / "d.h5"))
-    with pytest.raises(ImportError, match="h5py"):
+    with pytest.raises(ImportError, match="PESPACE_HDF5_LIBRARY"):
         TDIChannelData.recover_from_file(str(tmp_path / "d.h5"))
 
 
-@pytest.mark.skipif(not HAS_H5PY, reason="h5py is not installed")
+@need_hdf5
 def test_hdf5_round_trip(tmp_path):
     td = _data()
+    n_t = td.data_info.time_series_length
+    td._store_td([np.sin(np.arange(n_t) * (0.01 + 0.001 * i)) for i in range(3)])
     path = str(tmp_path / "d.h5")
     td.save_to_file(path)
     back = TDIChannelData.recover_from_file(path)
     for c in "AET":
         np.testing.assert_array_equal(getattr(back.fd_data, c), getattr(td.fd_data, c))
+        np.testing.assert_array_equal(getattr(back.td_data, c), getattr(td.td_data, c))
+        np.testing.assert_array_equal(getattr(back.fd_noise_power_density, c), getattr(td.fd_noise_power_density, c))
+    assert back.data_info.channels == ("A", "E", "T") and back.scaling == td.scaling
     with pytest.raises(OSError):  # mode "x": an existing file is not overwritten
         td.save_to_file(path)
+    with pytest.raises(FileNotFoundError):
+        TDIChannelData.recover_from_file(str(tmp_path / "none.h5"))
+    # what h5py's dataset[()] gives: strings as bytes, numeric scalars as numpy scalars
+    tree = h5file.read_tree(path)
+    assert tree["class_attrs/backend"] == b"hip" and tree["class_attrs/_np_cp"] == b"complex128"
+    assert list(tree["data_info/channels"]) == [b"A", b"E", b"T"]
+    assert isinstance(tree["class_attrs/scaling"], np.bool_)
+    assert tree["data_info/time_series_length"].dtype == np.int64
+    assert tree["tdi_data/fd_data/A"].dtype == np.complex128
+
+
+def _expected_dtype(value):
+    a = np.asarray(value)
+    return {"b": "|b1", "f": "<f8", "i": "<i8", "c": "<c16", "U": "|O"}[a.dtype.kind]
+
+
+@need_hdf5
+@need_h5py
+def test_h5py_reads_our_file(tmp_path):
+    td = _data()
+    path = tmp_path / "ours.h5"
+    td.save_to_file(str(path))
+    _h5py("dump", path, tmp_path / "d.npz", tmp_path / "d.json")
+    meta = json.loads((tmp_path / "d.json").read_text())
+    vals = np.load(tmp_path / "d.npz", allow_pickle=False)
+    tree = td._file_tree()
+    assert set(meta) == set(tree)
+    for key, value in tree.items():
+        m = meta[key]
+        assert m["dtype"] == _expected_dtype(value), (key, m)
+        assert m["shape"] == list(np.shape(value)), key
+        if m["dtype"] == "|O":
+            assert m["string"] == "utf-8" and m["class"] == 3, key  # variable-length UTF-8 (H5T_STRING)
+            assert m["pytype"] == ("bytes" if np.ndim(value) == 0 else "ndarray"), key
+        got = vals[key.replace("/", "|")]
+        np.testing.assert_array_equal(got, np.asarray(value), err_msg=key)
+    assert meta["class_attrs/scaling"]["class"] == 8  # h5py's bool: an enum (FALSE, TRUE)
+    assert meta["tdi_data/fd_data/A"]["class"] == 6  # complex: a compound (r, i)
+
+
+@need_hdf5
+@need_h5py
+def test_our_reader_reads_h5py_file(tmp_path):
+    td = _data()
+    tree = td._file_tree()
+    np.savez(tmp_path / "t.npz", **{k.replace("/", "|"): np.asarray(v) for k, v in tree.items()})
+    (tmp_path / "t.json").write_text(json.dumps(list(tree)))
+    _h5py("write", tmp_path / "t.npz", tmp_path / "t.json", tmp_path / "h5py.h5")
+    back = TDIChannelData.recover_from_file(str(tmp_path / "h5py.h5"))
+    for c in "AET":
+        np.testing.assert_array_equal(getattr(back.fd_data, c), getattr(td.fd_data, c))
+        np.testing.assert_array_equal(getattr(back.fd_noise_power_density, c), getattr(td.fd_noise_power_density, c))
+    np.testing.assert_array_equal(back.frequency_samples, td.frequency_samples)
+    # the same values and Python types as from our own file
+    td.save_to_file(str(tmp_path / "ours.h5"))
+    a, b = h5file.read_tree(tmp_path / "h5py.h5"), h5file.read_tree(tmp_path / "ours.h5")
+    assert list(a) == list(b)
+    for k in a:
+        assert type(a[k]) is type(b[k]), k
+        np.testing.assert_array_equal(np.asarray(a[k]), np.asarray(b[k]), err_msg=k)
+    # h5dump's description of both files: datatypes, dataspaces, layout, filters and fill values
+    h5dump = os.path.join(os.path.dirname(_h5py_python()), "h5dump")
+    if not os.path.exists(h5dump):
+        h5dump = shutil.which("h5dump")
+    if h5dump:
+        def header(p):
+            out = subprocess.run([h5dump, "-p", "-H", str(p)], capture_output=True, text=True, timeout=120,
+                                 env=_clean_env()).stdout
+            return [ln for ln in out.splitlines()[1:] if "OFFSET" not in ln]
+        assert header(tmp_path / "h5py.h5") == header(tmp_path / "ours.h5")
