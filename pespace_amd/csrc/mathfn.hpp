@@ -14,7 +14,8 @@
 // from a 256-entry table of correctly rounded values held in LDS (4 KiB): about 16 VALU
 // instructions and one LDS read instead of ~37, error within ~2 ulp.  sincos_tab_s (arguments
 // of a few radians) takes one reduction step; sincos_tab_sum takes a + x with a already reduced
-// (tab_reduce, both steps) and reduces x with both steps.
+// (tab_reduce, both steps) and reduces x with both steps; sincos_tab_gmem reads the table entry
+// from constant memory instead of LDS.
 // rcp_d: hardware reciprocal estimate refined by two Newton steps (correctly rounded to
 // about one ulp).
 #pragma once
@@ -129,6 +130,21 @@ __device__ inline void sincos_tab_sum(double ra, int ka_lo, double x, double* s,
   *c = fma(t.y, cr, -t.x * sr);
 }
 
+// sin, cos of x (|x| < 5e13) with the table entry read from global (constant) memory instead of
+// LDS, for kernels that keep no LDS copy of the table: both reduction steps, the polynomials of
+// sincos_tab_kr (|r| <= pi / kTrigN)
+__device__ inline void sincos_tab_gmem(double x, double* s, double* c) {
+  double r;
+  int k_lo;
+  tab_reduce(x, r, k_lo);
+  const int idx = k_lo & (kTrigN - 1);
+  const double ts = kTrigTab[idx][0], tc = kTrigTab[idx][1];
+  const double r2 = r * r;
+  const double sr = fma(r * r2, fma(r2, 1.0 / 120.0, -1.0 / 6.0), r);
+  const double cr = fma(r2, fma(r2, fma(r2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+  *s = fma(ts, cr, tc * sr);
+  *c = fma(tc, cr, -ts * sr);
+}
 
 __device__ inline double rcp_d(double x) {
   double r = __builtin_amdgcn_rcp(x);

@@ -583,14 +583,26 @@ __device__ inline TdPos td_pos(const TdWave& w, double x) {
 __device__ inline double td_lerp(const TdPos& q, double v0, double v1) {
   return q.inside ? v0 + (v1 - v0) * q.fr : 0.0;
 }
+// A position x in a block's LDS window of the waveform that starts at sample xlo (k_links_td,
+// STAGED): the window index of the sample below x and the fraction (both exact: x - xlo is); a
+// position outside the waveform's samples [0, xmax] is given the index zero_slot of two zero
+// entries past the window, so that its interpolation reads zero without a select per value
+// (its fraction is finite: zero times it is zero)
+__device__ inline TdPos td_win_pos(double x, double xlo, double xmax, int wlen, int zero_slot) {
+  const double xw = x - xlo;
+  const int j = min(max((int)xw, 0), wlen - 2);
+  const bool inside = x >= 0.0 && x <= xmax;
+  return {inside ? j : zero_slot, xw - (double)j, true};
+}
 
 // y_rs(t) = n.[H(t - L - k.x_s) - H(t - k.x_r)].n / (2 (1 - k.n_sr)) at t_i = t0 + i dt, with the
 // constellation at t_i; out [6][n] in the order 12, 21, 23, 32, 31, 13 (link_rs: sent from s).
 // One thread per sample.  The geometry takes the closed forms of the fused kernel's mode_geom
 // (node offsets xd_i = al_i U + be_i V; for link vector n_a = (xd_a+2 - xd_a+1) / L every
 // projection is n_a.v = (sqrt3 / L)(al_a v.V - be_a v.U)): one sincos of the revolution angle
-// (and, KIND 1, of node 1's rotation angle), six scalar products, no library sincos and no
-// division (the three denominators by rcp_d).  The waveform positions are formed in sample units
+// (and, KIND 1, of node 1's rotation angle; the table sincos, its entry read from constant
+// memory), six scalar products, no library sincos and no division (the three denominators by
+// rcp_d).  The waveform positions are formed in sample units
 // (x = (t - t_w0) / dt - (L + k.x_a) / dt).  Neighbouring threads read neighbouring samples.
 //
 // STAGED: the block first copies the stretch of h+, hx its samples can reach into LDS (all loads
@@ -625,7 +637,8 @@ __global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Ext
   const int i = lb * blockDim.x + threadIdx.x;
   constexpr bool STAGED = KW > 0;
   constexpr int WMAX = 256 * KW;
-  __shared__ double win_p[STAGED ? WMAX : 1], win_c[STAGED ? WMAX : 1];
+  // the window, then two zero entries that positions outside the waveform's samples read
+  __shared__ double win_p[STAGED ? WMAX + 2 : 1], win_c[STAGED ? WMAX + 2 : 1];
   // STAGED: the window loads are issued first and land while the geometry is computed
   double vp[STAGED ? KW : 1], vc[STAGED ? KW : 1];
   int wlo = 0, wlen = 0;
@@ -646,7 +659,7 @@ __global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Ext
   // block's barrier)
   const double t = t0 + dt * (double)i;
   double sa, ca;
-  sincos_d(fma(kOmegaYear, t, det.rev0), &sa, &ca);
+  sincos_tab_gmem(fma(kOmegaYear, t, det.rev0), &sa, &ca);
   const double kx0 = AU_SEC * fma(ex.k.x, ca, ex.k.y * sa);  // k.x0, x0 = AU (cos, sin, 0)
   double np[3], nq[3], kn[3], kx[3];                         // n_a.p, n_a.q, n_a.k; k.x_i
   if constexpr (KIND == 0) {
@@ -665,7 +678,7 @@ __global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Ext
     }
   } else {
     double s1, c1;
-    sincos_d(fma(det.omega_rot, t, det.rot0), &s1, &c1);
+    sincos_tab_gmem(fma(det.omega_rot, t, det.rot0), &s1, &c1);
     const double cr = -0.5, sr = 0.5 * SQRT3;  // rotation by 2 pi / 3
     double al[3], be[3];
     al[0] = s1;
@@ -693,8 +706,15 @@ __global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Ext
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const double xr = fma(-kx[a], w.inv_dt, base);
-    qs[a] = td_pos(w, xr - Ls);
-    qr[a] = td_pos(w, xr);
+    if constexpr (STAGED) {
+      // every position of the block lies in its window (the host checked the window's length)
+      const double xlo = (double)wlo, xmax = (double)(w.n - 1);
+      qs[a] = td_win_pos(xr - Ls, xlo, xmax, wlen, WMAX);
+      qr[a] = td_win_pos(xr, xlo, xmax, wlen, WMAX);
+    } else {
+      qs[a] = td_pos(w, xr - Ls);
+      qr[a] = td_pos(w, xr);
+    }
   }
   if constexpr (STAGED) {
 #pragma unroll
@@ -705,6 +725,10 @@ __global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Ext
         win_c[q] = vc[k];
       }
     }
+    if (threadIdx.x < 2) {
+      win_p[WMAX + threadIdx.x] = 0.0;
+      win_c[WMAX + threadIdx.x] = 0.0;
+    }
     __syncthreads();
     if (i >= n) return;
   }
@@ -712,8 +736,7 @@ __global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Ext
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     if constexpr (STAGED) {
-      // inside positions lie in the window; outside ones (result zero) are kept in the buffer
-      const int js = min(max(qs[a].j - wlo, 0), wlen - 2), jr = min(max(qr[a].j - wlo, 0), wlen - 2);
+      const int js = qs[a].j, jr = qr[a].j;
       hs[a][0] = win_p[js];
       hs[a][1] = win_p[js + 1];
       hs[a][2] = win_c[js];
@@ -738,10 +761,17 @@ __global__ __launch_bounds__(256) void k_links_td(DetectorDev det, TdWave w, Ext
   for (int a = 0; a < 3; ++a) {
     Pp[a] = fma(np[a], np[a], -nq[a] * nq[a]);  // n.e+.n
     Pc[a] = 2.0 * np[a] * nq[a];                // n.ex.n
-    sp[a] = td_lerp(qs[a], hs[a][0], hs[a][1]);
-    sc[a] = td_lerp(qs[a], hs[a][2], hs[a][3]);
-    rp[a] = td_lerp(qr[a], hr[a][0], hr[a][1]);
-    rc[a] = td_lerp(qr[a], hr[a][2], hr[a][3]);
+    if constexpr (STAGED) {  // outside positions read the zero entries
+      sp[a] = fma(hs[a][1] - hs[a][0], qs[a].fr, hs[a][0]);
+      sc[a] = fma(hs[a][3] - hs[a][2], qs[a].fr, hs[a][2]);
+      rp[a] = fma(hr[a][1] - hr[a][0], qr[a].fr, hr[a][0]);
+      rc[a] = fma(hr[a][3] - hr[a][2], qr[a].fr, hr[a][2]);
+    } else {
+      sp[a] = td_lerp(qs[a], hs[a][0], hs[a][1]);
+      sc[a] = td_lerp(qs[a], hs[a][2], hs[a][3]);
+      rp[a] = td_lerp(qr[a], hr[a][0], hr[a][1]);
+      rc[a] = td_lerp(qr[a], hr[a][2], hr[a][3]);
+    }
   }
   // link (r, s) lies along n_c with c the third node; 1 - k.n_sr = 1 + k.n_c when s -> r runs
   // against n_c (links 12, 23, 31) and 1 - k.n_c along it (21, 32, 13)
