@@ -819,10 +819,11 @@ __host__ __device__ constexpr double td_coef(int c, int l, int m) {
 
 template <bool ORTH>
 __device__ inline void td_store(double* __restrict__ out, size_t N, int i, double X, double Y, double Z) {
-  if (ORTH) {
-    __builtin_nontemporal_store((Z - X) / sqrt(2.0), out + i);
-    __builtin_nontemporal_store((X - 2.0 * Y + Z) / sqrt(6.0), out + N + i);
-    __builtin_nontemporal_store((X + Y + Z) / sqrt(3.0), out + 2 * N + i);
+  if (ORTH) {  // products with 1/sqrt2, 1/sqrt6, 1/sqrt3 rounded (no IEEE division)
+    constexpr double r2 = 0.70710678118654752440, r6 = 0.40824829046386301637, r3 = 0.57735026918962576451;
+    __builtin_nontemporal_store((Z - X) * r2, out + i);
+    __builtin_nontemporal_store((X - 2.0 * Y + Z) * r6, out + N + i);
+    __builtin_nontemporal_store((X + Y + Z) * r3, out + 2 * N + i);
   } else {
     __builtin_nontemporal_store(X, out + i);
     __builtin_nontemporal_store(Y, out + N + i);
