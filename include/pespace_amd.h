@@ -213,7 +213,7 @@ int pes_context_pin_scratch(pes_context* ctx, int pinned);
 int pes_whittle_lnl(const double* d_data, const double* d_resp, const double* d_psd, int n_chan, int n, double df,
                     double* d_work, double* d_out, void* stream);
 
-/* Schedule of the fused kernel: work items of tiles_per_group 256-bin tiles x 64 rows (160 for
+/* Schedule of the fused kernel: work items of tiles_per_group 256-bin tiles x 48 rows (160 for
  * lnL without gradient), taken from a queue by blocks_per_cu x (number of CUs) persistent
  * blocks of 256 threads.  Defaults
  * 32 and 4; pespace_amd.core.LikelihoodCore.autotune picks them by timing.  Changing the

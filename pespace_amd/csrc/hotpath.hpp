@@ -9,11 +9,14 @@
 //     factors) from one table sincos of 2 pi f t_c (reduced once per bin and row) plus
 //     Psi + 2 pi f k.x0 (reduced per mode), the remainders added: the total is never rounded and
 //     each term keeps the rounding it has as a separate factor.
-//   1 MIXED: in the adjoint's recompute, the pair phasors and the total-phase phasor (its phase
-//     summed and reduced in FP64) come from the FP32 hardware sine/cosine and the sinc
-//     denominators from the FP32 reciprocal; the orbit angle keeps the FP64 table sincos (its
-//     error is multiplied by 2 pi f AU/c ~ 150 rad in the delay phase) and all algebra stays
-//     FP64.  The gradient then carries FP32-level relative errors (~1e-7).
+//   1 MIXED: in the adjoint's recompute, the pair phasors come from the FP32 hardware
+//     sine/cosine and the sinc denominators from the FP32 reciprocal; the orbit angle keeps the
+//     FP64 table sincos (its error is multiplied by 2 pi f AU/c ~ 150 rad in the delay phase)
+//     and all algebra stays FP64.  The gradient then carries FP32-level relative errors (~1e-7).
+// In both modes the gradient kernel's adjoint takes each mode's total-phase phasor from the
+// forward pass (kept in LDS, mode_geom UPH), so that phase is formed once per (bin, mode, row).
+// (The response-only kernels in the mixed mode form it from the FP32 hardware sine/cosine of the
+// phase summed and reduced in FP64.)
 // Why FP32 goes no further: without packed FP32 (off by build rule) an FP32 FMA issues at the
 // FP64 rate on gfx950, and the mode loops are ~90% FMA/add/mul, so FP32 pays only in the
 // transcendental units; an FP32 forward would bias lnL coherently (per-row constants and the
@@ -227,9 +230,13 @@ __device__ inline bool interval_on(const DetectorDev& det, const double* __restr
   return tl - start >= det.duty_taper && th - start <= det.duty_on - det.duty_taper;
 }
 
-template <int PREC, int KIND>
+// UPH: 0 forms the unit phasor u; 1 forms it and stores it at uslot (this lane's LDS slot for
+// the mode: the gradient kernel's forward pass); 2 takes it from uslot (the adjoint's
+// recompute, which then needs neither the phase nor k.x0)
+template <int PREC, int KIND, int UPH = 0>
 __device__ inline void mode_geom(const DetectorDev& det, const double* __restrict__ mf,
-                                 const double* __restrict__ n0, const BinPre& b, const RowC& rc, ModeG& g) {
+                                 const double* __restrict__ n0, const BinPre& b, const RowC& rc, ModeG& g,
+                                 double* __restrict__ uslot = nullptr) {
   const double A0 = n0[0], dA0 = n0[1], P0 = n0[2], dP0 = n0[3];
   const double A1 = n0[4], dA1 = n0[5], P1 = n0[6], dP1 = n0[7];
   g.A = fma(b.hb1, A1 - A0, fma(b.hb2, dA0, fma(b.hb3, dA1, A0)));
@@ -299,18 +306,24 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
     g.z1 = {c1, s1};
     g.z01 = mul(g.z0, g.z1);
   }
-  double sp, cp;
-  if constexpr (Trig<PREC>::kFp32) {
-    // the total phase summed and reduced to revolutions in FP64, its sine and cosine in FP32
-    sincos_rev_fc(fma(2.0 * b.pif, rc.tc + g.kx0, Psi), &sp, &cp);
+  if constexpr (UPH == 2) {
+    const double2 uu = *reinterpret_cast<const double2*>(uslot);
+    g.u = {uu.x, uu.y};
   } else {
-    // 2 pi f t_c (rounded once, as the reference's own factor, and reduced once per bin and
-    // row) plus Psi + 2 pi f k.x0, each reduced on its own and the remainders added: the total
-    // is never rounded, so the phase's rounding error does not move with Psi or k.x0, and lnL
-    // stays as smooth in the parameters as with three separate factors
-    sincos_tab_sum(rc.tc_r, rc.tc_k, fma(2.0 * b.pif, g.kx0, Psi), &sp, &cp);
+    double sp, cp;
+    if constexpr (Trig<PREC>::kFp32) {
+      // the total phase summed and reduced to revolutions in FP64, its sine and cosine in FP32
+      sincos_rev_fc(fma(2.0 * b.pif, rc.tc + g.kx0, Psi), &sp, &cp);
+    } else {
+      // 2 pi f t_c (rounded once, as the reference's own factor, and reduced once per bin and
+      // row) plus Psi + 2 pi f k.x0, each reduced on its own and the remainders added: the total
+      // is never rounded, so the phase's rounding error does not move with Psi or k.x0, and lnL
+      // stays as smooth in the parameters as with three separate factors
+      sincos_tab_sum(rc.tc_r, rc.tc_k, fma(2.0 * b.pif, g.kx0, Psi), &sp, &cp);
+    }
+    g.u = {cp, -sp};
+    if constexpr (UPH == 1) *reinterpret_cast<double2*>(uslot) = make_double2(cp, -sp);
   }
-  g.u = {cp, -sp};
   g.hm = scale(g.u, kDuty<KIND> ? g.A * w : g.A);
   g.hp = mul(Cx{mf[MF_FPR], mf[MF_FPI]}, g.hm);
   g.hc = mul(Cx{mf[MF_FCR], mf[MF_FCI]}, g.hm);
@@ -425,17 +438,19 @@ __device__ inline void tdi_out(const DetectorDev& det, const TdiC& t, const Link
 // The six links of one (bin, row) summed over the modes, without the common factor
 // (-i xL) exp(-i xL) of every link term.
 // nb: node entries (j, j+1) of this bin for mode 0; mode mi's are at nb + mi * mstride
-template <int PREC, int KIND>
+// UPH 1: each mode's unit phasor is stored in u_lds ([mode][lane] double pairs) for the adjoint
+template <int PREC, int KIND, int UPH = 0>
 __device__ inline void links_row(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
                                  const double* __restrict__ nb, size_t mstride, const BinPre& b, const RowC& rc,
-                                 LinksT& L) {
+                                 LinksT& L, double* __restrict__ u_lds = nullptr) {
 #pragma unroll
   for (int i = 0; i < 6; ++i) L.l[i] = {0.0, 0.0};
   for (int mi = 0; mi < modes.n; ++mi) {
     const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
     if (b.f > mf[MF_CUT]) continue;
     ModeG g;
-    mode_geom<PREC, KIND>(det, mf, nb + mi * mstride, b, rc, g);
+    mode_geom<PREC, KIND, UPH>(det, mf, nb + mi * mstride, b, rc, g,
+                               UPH ? u_lds + 2 * (mi * TILE + threadIdx.x) : nullptr);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       PairV v;
@@ -447,12 +462,12 @@ __device__ inline void links_row(const DetectorDev& det, const double* __restric
   }
 }
 
-template <int PREC, int KIND>
+template <int PREC, int KIND, int UPH = 0>
 __device__ inline void response_row(const DetectorDev& det, const double* __restrict__ pc, const ModeList& modes,
                                     const double* __restrict__ nb, size_t mstride, const BinPre& b,
-                                    const RowC& rc, const TdiC& tdc, Cx out[3]) {
+                                    const RowC& rc, const TdiC& tdc, Cx out[3], double* __restrict__ u_lds = nullptr) {
   LinksT L;
-  links_row<PREC, KIND>(det, pc, modes, nb, mstride, b, rc, L);
+  links_row<PREC, KIND, UPH>(det, pc, modes, nb, mstride, b, rc, L, u_lds);
   tdi_out(det, tdc, L, out);
 }
 
@@ -469,13 +484,14 @@ struct AdjAcc {
 };
 
 // wl: adjoints of the six links (the T channel's folded in); mid: mode id
+// uslot: this lane's LDS slot of the mode's unit phasor, stored by the forward pass
 template <int PREC, int KIND>
 __device__ inline void mode_adjoint(const DetectorDev& det, const double* __restrict__ mf, int mid,
                                     const double* __restrict__ n0, const double* __restrict__ g0, const BinPre& b,
                                     const RowC& rc, const Cx wl[6], const double* __restrict__ pc,
-                                    AdjAcc& acc) {
+                                    AdjAcc& acc, double* __restrict__ uslot) {
   ModeG g;
-  mode_geom<PREC, KIND>(det, mf, n0, b, rc, g);
+  mode_geom<PREC, KIND, 2>(det, mf, n0, b, rc, g, uslot);
   // adjoints of n_a.p, n_a.q, n_a.k and of k.xd_a
   double npb[3], nqb[3], knb[3], kxb[3];
   Cx Hp = {0.0, 0.0}, Hc = {0.0, 0.0};
@@ -726,7 +742,7 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
                                  const double* __restrict__ nodes, const double* __restrict__ nodes_grad, int NNS,
                                  double du, const ModeList& modes, int lane, int wave, double* acc, int j0,
                                  int nj, double* stage_n, double* stage_g, int& buf,
-                                 const unsigned long long* __restrict__ off_bits, int n_tiles, double* rec_lds) {
+                                 const unsigned long long* __restrict__ off_bits, int n_tiles, double* lane_lds) {
   constexpr int NOUT = GRAD ? 1 + PES_N_PARAMS : 1;
   const int bi = tile * TILE + threadIdx.x;
   // a duty-cycled detector's tile whose window is zero for every mode of row p throughout
@@ -747,16 +763,17 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
   // the bin's weight and data record (w, Re d, Im d per channel).  The forward-only kernel reads
   // it once per tile into the lane's own LDS column and takes it from there after each row's
   // mode loop (1.7% faster than re-reading it through the caches); the gradient kernel, whose
-  // LDS holds its 64-row accumulators, re-reads it per row from the caches, requested before
-  // the mode loops so that their latency is hidden behind them (0.6% faster than the LDS column)
+  // LDS holds its row accumulators and the modes' phasors, re-reads it per row from the caches,
+  // requested before the mode loops so that their latency is hidden behind them (0.6% faster
+  // than an LDS column)
   constexpr bool kRecLds = !GRAD, kRecEarly = GRAD;
   if constexpr (kRecLds) {
     const double2* r2 = reinterpret_cast<const double2*>(bin_rec + (size_t)bi * BIN_REC);
 #pragma unroll
     for (int i = 0; i < BIN_REC / 2; ++i) {
       const double2 t = r2[i];
-      rec_lds[(2 * i) * TILE + threadIdx.x] = t.x;
-      rec_lds[(2 * i + 1) * TILE + threadIdx.x] = t.y;
+      lane_lds[(2 * i) * TILE + threadIdx.x] = t.x;
+      lane_lds[(2 * i + 1) * TILE + threadIdx.x] = t.y;
     }
   }
   constexpr int SN = MAX_MODES * NJ_STAGE * 4, SG = MAX_MODES * NJ_STAGE * 16;  // doubles per buffer
@@ -786,7 +803,7 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
       auto load_rec = [&]() {
         if constexpr (kRecLds) {
 #pragma unroll
-          for (int i = 0; i < BIN_REC; ++i) rec[i] = rec_lds[i * TILE + threadIdx.x];
+          for (int i = 0; i < BIN_REC; ++i) rec[i] = lane_lds[i * TILE + threadIdx.x];
         } else {
           int off = bi * BIN_REC;
           asm volatile("" : "+v"(off));  // a per-row offset: the loads stay in the row loop
@@ -827,7 +844,8 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
       TdiC tdc;
       tdi_consts(det, b, tdc);
       Cx out[3];
-      response_row<0, KIND>(det, pc, modes, nb, nstride, b, rc, tdc, out);  // FP64 in every mode
+      // FP64 in every mode; the gradient kernel keeps each mode's unit phasor for the adjoint
+      response_row<0, KIND, GRAD ? 1 : 0>(det, pc, modes, nb, nstride, b, rc, tdc, out, lane_lds);
       // the forward-only kernel (168 registers) takes the record here instead: held across its
       // mode loop, the record's registers were spilled (measured 0.9% slower)
       if constexpr (!kRecEarly) load_rec();
@@ -877,7 +895,7 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
           const double* mf = pc + PC_MODE + mi * PC_MODE_STRIDE;
           if (b.f > mf[MF_CUT]) continue;
           mode_adjoint<PREC, KIND>(det, mf, modes.id[mi], nb + mi * nstride, gb + mi * gstride, b, rc, wl, pc,
-                                   aa);
+                                   aa, lane_lds + 2 * (mi * TILE + threadIdx.x));
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) G[i] = aa.gint[i];
@@ -950,8 +968,10 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
   __shared__ typename std::conditional<GRAD, double[2 * MAX_MODES * NJ_STAGE * 16], double[1]>::type stage_g;
   int buf = 0;
   __shared__ int next_work;
-  // forward-only: the tile's bin records, [record entry][lane] (each lane its own column)
-  __shared__ typename std::conditional<GRAD, double[1], double[BIN_REC * TILE]>::type rec_lds;
+  // each lane's own LDS column: forward-only, the tile's bin records [record entry][lane];
+  // gradient, each mode's unit phasor from the forward pass [mode][lane] (pairs), read back by
+  // the adjoint instead of its phase and sincos
+  __shared__ double lane_lds[GRAD ? 2 * MAX_MODES * TILE : BIN_REC * TILE];
   trig_table_load();
   // the forward-only kernel holds the (uniform) wave index in a scalar register: 1.3% faster
   // there, 0.2% slower with the gradient
@@ -1005,7 +1025,7 @@ __global__ __launch_bounds__(TILE, GRAD ? 2 : 3) void k_lnl(const DetectorDev* _
       const int j0 = tile_jr[2 * tile], nj = tile_jr[2 * tile + 1];
 #define PES_TILE_ARGS                                                                                         \
   det, tile, f0, p0, p1, bin_f, bin_j, bin_s, bin_rec, tile_dd, pconst, nodes, nodes_grad, NNS, du, \
-      modes, lane, wave, &acc[0][0][0], j0, nj, stage_n, &stage_g[0], buf, off_bits, n_tiles, &rec_lds[0]
+      modes, lane, wave, &acc[0][0][0], j0, nj, stage_n, &stage_g[0], buf, off_bits, n_tiles, &lane_lds[0]
       // the host's tile ranges give this launch detectors of one orbit kind (KD bit 0), all
       // duty-cycled or none (bit 1)
       if (nj <= NJ_STAGE)

@@ -4,10 +4,11 @@
 #pragma once
 
 constexpr int TILE = 256;      // bins per tile = threads per block
-// parameter rows per work item of the gradient kernel (64: 0.4% faster than 40 on the one-year
-// network) and of the lnL-only one (160: 1.1% faster than 256)
+// parameter rows per work item of the gradient kernel (48: the most that leaves two blocks per
+// CU the LDS for the staged node rows, the accumulators and the forward pass's unit phasors)
+// and of the lnL-only one (160: 1.1% faster than 256)
 template <bool GRAD>
-constexpr int chunk_rows() { return GRAD ? 64 : 160; }
+constexpr int chunk_rows() { return GRAD ? 48 : 160; }
 constexpr int TILES_PER_GROUP = 32;  // default schedule: tiles per work item ...
 constexpr int BLOCKS_PER_CU = 4;     // ... and persistent blocks launched per CU
 constexpr int MAX_MODES = 6;         // modes per context (ModeList)
