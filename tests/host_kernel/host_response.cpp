@@ -49,6 +49,7 @@ int run(int argc, char** argv) {
   threadIdx.x = 0;
   trig_table_load();
   std::vector<double> out((size_t)P * 3 * n * 2);
+  std::vector<double> u_col((size_t)2 * MAX_MODES * TILE);  // the gradient kernel's per-lane phasor column
   double adj_check = 0.0;
   for (int p = 0; p < P; ++p) {
     const double* pcp = pc.data() + (size_t)p * PC_STRIDE;
@@ -60,10 +61,12 @@ int run(int argc, char** argv) {
       tdi_consts(det, b, tdc);
       Cx o[3];
       const double* nb = nodes.data() + ((size_t)p * nm * NNS + b.j) * 4;
+      // the forward pass stores each mode's unit phasor in the lane's column, as in the gradient
+      // kernel, and the adjoint below reads it back
       if (kind == 0)
-        response_row<0, 0>(det, pcp, modes, nb, (size_t)NNS * 4, b, rc, tdc, o);
+        response_row<0, 0, 1>(det, pcp, modes, nb, (size_t)NNS * 4, b, rc, tdc, o, u_col.data());
       else
-        response_row<0, 1>(det, pcp, modes, nb, (size_t)NNS * 4, b, rc, tdc, o);
+        response_row<0, 1, 1>(det, pcp, modes, nb, (size_t)NNS * 4, b, rc, tdc, o, u_col.data());
       for (int c = 0; c < 3; ++c) {
         out[(((size_t)p * 3 + c) * n + k) * 2] = o[c].re;
         out[(((size_t)p * 3 + c) * n + k) * 2 + 1] = o[c].im;
@@ -78,12 +81,13 @@ int run(int argc, char** argv) {
         AdjAcc a0, a1;
         std::memset(&a0, 0, sizeof a0);
         std::memset(&a1, 0, sizeof a1);
+        double* us = u_col.data() + 2 * ((size_t)mi * TILE + threadIdx.x);
         if (kind == 0) {
-          mode_adjoint<0, 0>(det, mf, modes.id[mi], nb + (size_t)mi * NNS * 4, g0, b, rc, wl, pcp, a0);
-          mode_adjoint<1, 0>(det, mf, modes.id[mi], nb + (size_t)mi * NNS * 4, g0, b, rc, wl, pcp, a1);
+          mode_adjoint<0, 0>(det, mf, modes.id[mi], nb + (size_t)mi * NNS * 4, g0, b, rc, wl, pcp, a0, us);
+          mode_adjoint<1, 0>(det, mf, modes.id[mi], nb + (size_t)mi * NNS * 4, g0, b, rc, wl, pcp, a1, us);
         } else {
-          mode_adjoint<0, 1>(det, mf, modes.id[mi], nb + (size_t)mi * NNS * 4, g0, b, rc, wl, pcp, a0);
-          mode_adjoint<1, 1>(det, mf, modes.id[mi], nb + (size_t)mi * NNS * 4, g0, b, rc, wl, pcp, a1);
+          mode_adjoint<0, 1>(det, mf, modes.id[mi], nb + (size_t)mi * NNS * 4, g0, b, rc, wl, pcp, a0, us);
+          mode_adjoint<1, 1>(det, mf, modes.id[mi], nb + (size_t)mi * NNS * 4, g0, b, rc, wl, pcp, a1, us);
         }
         adj_check += a0.tc + a1.tc + a0.iota + a0.mphi;
       }
