@@ -191,14 +191,18 @@ def prior_sweep(core, sizes=(256, 3001, 8192), seed=2027):
 
 
 def traffic_of(workload, precision, grad):
-    """HBM bytes per launch of the fused kernel from the committed rocprofv3 counter summary."""
+    """HBM bytes per step of the fused kernel and the fraction of cycles its SIMDs issue VALU
+    work, from the committed rocprofv3 counter summaries (FETCH_SIZE / WRITE_SIZE scaled by a
+    plain copy; SQ_ACTIVE_INST_VALU over GRBM_GUI_ACTIVE); (None, None) for a workload they do
+    not cover."""
     try:
         with open(TRAFFIC_FILE) as fh:
             doc = json.load(fh)
         e = doc["runs"][f"{workload}/{precision}/{'grad' if grad else 'lnl'}"]
-        return e["fetch_kib"] * 1024 * doc["fetch_scale"] + e["write_kib"] * 1024 * doc["write_scale"]
+        return (e["fetch_kib"] * 1024 * doc["fetch_scale"] + e["write_kib"] * 1024 * doc["write_scale"],
+                e.get("valu_busy"))
     except (OSError, KeyError, ValueError):
-        return None
+        return None, None
 
 
 def cpu_baseline(dets, wf, rows, n_rows=2):
@@ -409,7 +413,7 @@ def main():
     if grad:
         flops += h["bm"] * OPS_BM_ADJ + h["br"] * OPS_BR_ADJ
     achieved = flops / (h["kernel_ms"] * 1e-3)
-    traffic = traffic_of(args.workload, head, grad)
+    traffic, valu_busy = traffic_of(args.workload, head, grad)
     line = {
         "metric": "lnL+grad evals/s, 4096-particle batch, mixed vs FP64 on 1 GPU, |dlnL|<1e-3" if grad
         else "lnL evals/s (no gradient)",
@@ -430,10 +434,14 @@ def main():
                    "schedule": tune if tune else dict(zip(("tiles_per_group", "blocks_per_cu"), core.schedule()))},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": traffic,
+                     "hbm_GB_per_s": traffic / (h["kernel_ms"] * 1e-3) / 1e9 if traffic else None,
+                     "valu_busy": valu_busy,
                      "work": {"bin_mode_rows": h["bm"], "bin_rows": h["br"], "flops": flops},
                      "note": "algorithmic FP64 flops of the fused lnL(+grad) kernel over its HIP-event time; "
                              "sines/cosines/reciprocals not counted as flops; all algebra is FP64 in every "
-                             "precision mode (mixed takes FP32 sin/cos/rcp in the adjoint only)"},
+                             "precision mode (mixed takes FP32 sin/cos/rcp in the adjoint only); traffic and "
+                             "valu_busy from the committed rocprofv3 counter summaries (profiles/), "
+                             "hbm_GB_per_s = traffic over this run's kernel time"},
     }
     for prec in precs:
         line[f"{prec}_evals_per_s"] = K * P / res[prec]["dt"]
