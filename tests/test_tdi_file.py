@@ -267,3 +267,42 @@ def test_our_reader_reads_h5py_file(tmp_path):
                                  env=_clean_env()).stdout
             return [ln for ln in out.splitlines()[1:] if "OFFSET" not in ln]
         assert header(tmp_path / "h5py.h5") == header(tmp_path / "ours.h5")
+
+
+_H5PY_BRANCH = r'''
+import importlib.util, sys
+import numpy as np
+spec = importlib.util.spec_from_file_location("h5file", sys.argv[1])
+m = importlib.util.module_from_spec(spec)
+spec.loader.exec_module(m)
+assert m.backend() == "h5py"
+tree = {"a/flag": np.array(True), "a/name": "hip", "a/names": np.array(["A", "E", "T"]),
+        "b/n": np.array(131073), "b/x": np.linspace(0.0, 1.0, 7), "c/z": np.arange(5) * (1.0 + 2.0j),
+        "c/mask": np.array([True, False, True])}
+m.write_tree(sys.argv[2], tree)
+back = m.read_tree(sys.argv[2])
+assert back["a/name"] == b"hip" and list(back["a/names"]) == [b"A", b"E", b"T"]
+assert back["a/flag"] == True and back["b/n"] == 131073
+np.testing.assert_array_equal(back["c/z"], tree["c/z"])
+print("ok")
+'''
+
+
+@need_hdf5
+@need_h5py
+def test_h5py_branch_of_h5file(tmp_path):
+    """h5file's h5py route (run by the interpreter that has h5py) writes what the C-library route
+    reads back with the same values and types."""
+    path = tmp_path / "h5py_branch.h5"
+    exe = _h5py_python()
+    r = subprocess.run([exe, "-c", _H5PY_BRANCH, os.path.join(os.path.dirname(HERE), "pespace_amd", "detector",
+                                                                "h5file.py"), str(path)],
+                       env=_clean_env(), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr[-2000:]
+    back = h5file.read_tree(path)
+    assert back["a/name"] == b"hip" and list(back["a/names"]) == [b"A", b"E", b"T"]
+    assert isinstance(back["a/flag"], np.bool_) and back["a/flag"]
+    assert back["b/n"] == 131073 and back["b/n"].dtype == np.int64
+    np.testing.assert_array_equal(back["b/x"], np.linspace(0.0, 1.0, 7))
+    np.testing.assert_array_equal(back["c/z"], np.arange(5) * (1.0 + 2.0j))
+    np.testing.assert_array_equal(back["c/mask"], [True, False, True])
