@@ -68,14 +68,16 @@ WORKLOADS = {
 #     transcendentals: 4 sincos, 3 reciprocals
 #   forward, per (bin, row) evaluated: TDI constants 29, X, Y, Z -> A, E, T (T from the
 #     pair differences of the link sums) 82, residual and accumulation 24
-#   adjoint (lnL + gradient adds): per (bin, mode, row) the recomputed forward 267 and the
-#     adjoint 416 (link pairs 141, constellation and time 90, waveform side and node-table
-#     contraction 185); per (bin, row) 199 (link adjoints with the T channel's folded in, the
-#     p, q, k adjoints contracted with their lambda, beta, psi derivatives)
+#   adjoint (lnL + gradient adds): per (bin, mode, row) the recomputed forward 253 (the forward
+#     less the total phase Psi + 2 pi f (t_c + k.x0), 14, whose phasor the adjoint reads back
+#     from the forward pass) and the adjoint 416 (link pairs 141, constellation and time 90,
+#     waveform side and node-table contraction 185); per (bin, row) 199 (link adjoints with the
+#     T channel's folded in, the p, q, k adjoints contracted with their lambda, beta, psi
+#     derivatives)
 # "evaluated" = the bin lies at or below the mode's cut (the (bin, row) pairs: below the row's
 # largest cut); the kernel skips the rest, whose response is exactly zero.
 OPS_BM_FWD, OPS_BR_FWD = 279, 135
-OPS_BM_ADJ, OPS_BR_ADJ = 683, 199
+OPS_BM_ADJ, OPS_BR_ADJ = 669, 199
 PEAK_FP64 = 78.6e12  # MI355X FP64 vector, FLOP/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lnl_traffic.json")
 
