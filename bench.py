@@ -271,6 +271,38 @@ def cpu_baseline(dets, wf, rows, n_rows=2):
                        f"CPU backend: 144 ns per bin per lnL (XAS, 65,471 bins)")
 
 
+def fisher_measure(core, reps=3):
+    """The Fisher matrix path (fisher.py:27-162) on the benchmark network at the injection: a
+    noiseless copy of the context (data = the response at the injection, its detectors and PSDs)
+    and the matrix itself, minus the Hessian of lnL from differences of the exact adjoint
+    gradient: two batched lnL + gradient launches of 45 rows each, the fiducial and 44 shifts (the
+    pilot steps, then steps from the pilot's diagonal; steps h and h/2 Richardson-combined).  first_call_s: the
+    copy's construction, its first launch (context build) and the first matrix; matrix_ms: the
+    matrix on the built copy, mean of `reps` after that."""
+    import torch
+    from pespace_amd.core import PARAM_NAMES
+    from pespace_amd.inference.fisher import _fisher_core, _noiseless_core
+    row0 = np.array([INJ[k] for k in PARAM_NAMES])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nc = _noiseless_core(core, row0)
+    try:
+        F = _fisher_core(nc, row0)
+        torch.cuda.synchronize()
+        first = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            F = _fisher_core(nc, row0)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+    finally:
+        nc.close()
+    d = np.diag(F)
+    return {"first_call_s": first, "matrix_ms": dt * 1e3, "n_params": int(F.shape[0]),
+            "gradient_rows_per_matrix": 2 * (1 + 4 * int(F.shape[0])),
+            "finite_positive_diagonal": bool(np.all(np.isfinite(F)) and np.all(d > 0))}
+
+
 def td_measure(reps=20, dt=5.0):
     """Time-domain links (k_links_td) and TDI 1.5 A/E/T (k_tdi_td) over one year of LISA samples
     at 5 s (6,311,521 outputs plus the delays' lead-in) for a chirping waveform sampled at the
@@ -337,6 +369,7 @@ def main():
     ap.add_argument("--schedule", default=None,
                     help="fixed schedule 'tiles_per_group,blocks_per_cu[,rows_per_item]' (no autotune)")
     ap.add_argument("--no-td", action="store_true", help="skip the time-domain links/TDI timing")
+    ap.add_argument("--no-fisher", action="store_true", help="skip the Fisher matrix timing")
     ap.add_argument("--no-duty", action="store_true", help="network workloads without TianQin's duty cycle")
     ap.add_argument("--td-only", action="store_true", help="only time the time-domain links and TDI")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -465,6 +498,8 @@ def main():
     if grad and not args.no_prior_sweep and WORKLOADS[args.workload]["detectors"] == ("LISA", "Taiji", "TianQin") \
             and WORKLOADS[args.workload]["grid"] == "year":
         line["mixed_prior_batches"] = prior_sweep(core)
+    if grad and not args.no_fisher and WORKLOADS[args.workload]["detectors"] == ("LISA", "Taiji", "TianQin"):
+        line["fisher"] = fisher_measure(core)
     if not args.no_td:
         line["td_response"] = td_measure()
     if not args.no_cpu_baseline:
