@@ -224,7 +224,7 @@ int pes_context_set_schedule(pes_context* ctx, int tiles_per_group, int blocks_p
 int pes_context_get_schedule(const pes_context* ctx, int* tiles_per_group, int* blocks_per_cu);
 
 /* Parameter rows per work item of the fused kernel (the rows a block evaluates over one tile
- * group): 0 (the default) picks them per batch (the kernel's chunk, 64 with the gradient and 160
+ * group): 0 (the default) picks them per batch (the kernel's chunk, 48 with the gradient and 160
  * without, fewer when the grid has too few tile groups); 1..160 fixes them (capped at the
  * kernel's chunk).  A tunable of the project's autotuner, beside the tile-group size and the
  * blocks per CU; no counterpart in the reference. */

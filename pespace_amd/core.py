@@ -185,7 +185,7 @@ class LikelihoodCore:
     # -- schedule of the fused kernel ------------------------------------------------------
     def set_schedule(self, tiles_per_group, blocks_per_cu, rows_per_item=None):
         """Work items of ``tiles_per_group`` 256-bin tiles x ``rows_per_item`` parameter rows
-        (0: chosen per batch, 40 with the gradient and 160 without unless the grid is small;
+        (0: chosen per batch, 48 with the gradient and 160 without unless the grid is small;
         None: unchanged), ``blocks_per_cu`` persistent blocks per CU."""
         self._build()
         check(lib().pes_context_set_schedule(self._ctx, int(tiles_per_group), int(blocks_per_cu)))

@@ -47,6 +47,9 @@ def _noiseless_core(core, row):
         nc.add_detector(d["kind"], d["arm"], d["rot0"], d["rev0"], d["gen"], d["orth"], d["f"], data, d["psd"],
                         d["mask"], df=d["df"], weight=d["weight"], omega_rotation=d["omega"],
                         lambda_ref=d["lam"], beta_ref=d["bet"], duty_cycle=d["duty"])
+    # the source context's schedule (autotuned or set by the caller) instead of the default; the
+    # rows per work item stay chosen per batch (the Fisher batches hold 45 rows)
+    nc.set_schedule(*core.schedule())
     return nc
 
 
