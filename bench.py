@@ -413,7 +413,7 @@ def main():
     elif not args.no_autotune:
         # schedule chosen by the autotuner on a 512-row sample of the batch (untimed)
         t0 = time.perf_counter()
-        # rows per work item: 0 = the kernel's chunk (64 with the gradient, 160 without), or fixed
+        # rows per work item: 0 = the kernel's chunk (48 with the gradient, 160 without), or fixed
         rows_c = (0, 40, 20) if grad else (0, 80, 40)
         best, times = core.autotune(params[:min(512, args.particles)], grad=grad,
                                     precision="fp64" if args.precision == "all" else args.precision,
