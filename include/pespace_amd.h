@@ -216,7 +216,7 @@ int pes_whittle_lnl(const double* d_data, const double* d_resp, const double* d_
 /* Schedule of the fused kernel: work items of tiles_per_group 256-bin tiles x 48 rows (160 for
  * lnL without gradient), taken from a queue by blocks_per_cu x (number of CUs) persistent
  * blocks of 256 threads.  Defaults
- * 32 and 4; pespace_amd.core.LikelihoodCore.autotune picks them by timing.  Changing the
+ * 8 and 4; pespace_amd.core.LikelihoodCore.autotune picks them by timing.  Changing the
  * schedule, a larger batch or a first gradient call may reallocate scratch at the next call:
  * reserve and pin the scratch (below) before capturing a HIP graph.
  * No counterpart in the reference. */
