@@ -136,6 +136,9 @@ struct RowC {
   V3 p, q, k;
   double tc_r;  // 2 pi f t_c - tc_k 2 pi / kTrigN, |tc_r| <= pi / kTrigN
   int tc_k;     // low bits of the integer tc_k
+  // geocentric orbits: (sqrt3/L) p.U, (sqrt3/L) p.V, the same for q, and k.U, k.V with the
+  // detector's constant U, V (row_geo; unused for heliocentric orbits)
+  double pU, pV, qU, qV, kU, kV;
 };
 __device__ inline RowC load_row(const double* __restrict__ pc, const BinPre& b) {
   RowC r;
@@ -146,6 +149,16 @@ __device__ inline RowC load_row(const double* __restrict__ pc, const BinPre& b) 
   // 2 pi f t_c rounded once, as the reference's own factor
   tab_reduce(2.0 * b.pif * r.tc, r.tc_r, r.tc_k);
   return r;
+}
+// the row's projections on a geocentric detector's constant U, V, once per row instead of per
+// mode and pass
+__device__ inline void row_geo(const DetectorDev& det, RowC& r) {
+  r.pU = fma(r.p.x, det.sgxs, fma(r.p.y, det.sgys, r.p.z * det.sgzs));
+  r.pV = fma(r.p.x, det.sgxc, r.p.y * det.sgyc);
+  r.qU = fma(r.q.x, det.sgxs, fma(r.q.y, det.sgys, r.q.z * det.sgzs));
+  r.qV = fma(r.q.x, det.sgxc, r.q.y * det.sgyc);
+  r.kU = fma(r.k.x, det.gxs, fma(r.k.y, det.gys, r.k.z * det.gzs));
+  r.kV = fma(r.k.x, det.gxc, r.k.y * det.gyc);
 }
 
 // One mode at one bin: waveform, orbit, and the projections of the three link pairs.
@@ -277,12 +290,12 @@ __device__ inline void mode_geom(const DetectorDev& det, const double* __restric
     g.be[1] = fma(c1, cr, -s1 * sr);
     g.al[2] = fma(g.al[1], cr, g.be[1] * sr);
     g.be[2] = fma(g.be[1], cr, -g.al[1] * sr);
-    g.pU = fma(rc.p.x, det.sgxs, fma(rc.p.y, det.sgys, rc.p.z * det.sgzs));
-    g.pV = fma(rc.p.x, det.sgxc, rc.p.y * det.sgyc);
-    g.qU = fma(rc.q.x, det.sgxs, fma(rc.q.y, det.sgys, rc.q.z * det.sgzs));
-    g.qV = fma(rc.q.x, det.sgxc, rc.q.y * det.sgyc);
-    g.kU = fma(rc.k.x, det.gxs, fma(rc.k.y, det.gys, rc.k.z * det.gzs));
-    g.kV = fma(rc.k.x, det.gxc, rc.k.y * det.gyc);
+    g.pU = rc.pU;
+    g.pV = rc.pV;
+    g.qU = rc.qU;
+    g.qV = rc.qV;
+    g.kU = rc.kU;
+    g.kV = rc.kV;
     const double kUs = det.c3L * g.kU, kVs = det.c3L * g.kV;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -840,7 +853,8 @@ __device__ inline void tile_rows(const DetectorDev& det, int tile, double f0, in
         nstride = (size_t)NNS * 4;
         gstride = (size_t)NNS * 16;
       }
-      const RowC rc = load_row(pc, b);
+      RowC rc = load_row(pc, b);
+      if constexpr (kGeo<KIND>) row_geo(det, rc);
       TdiC tdc;
       tdi_consts(det, b, tdc);
       Cx out[3];
@@ -1059,7 +1073,8 @@ __global__ __launch_bounds__(128) void k_response_t(const DetectorDev* __restric
   BinPre b;
   bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
   const double* pc = pconst + (size_t)p * PC_STRIDE;
-  const RowC rc = load_row(pc, b);
+  RowC rc = load_row(pc, b);
+  row_geo(det, rc);  // used by the geocentric variants only
   TdiC tdc;
   tdi_consts(det, b, tdc);
   Cx o[3];
@@ -1095,7 +1110,8 @@ __global__ __launch_bounds__(128) void k_links(const DetectorDev* __restrict__ d
   BinPre b;
   bin_prepare(det, bin_f[bi], bin_s[bi], bin_j[bi], du, b);
   const double* pc = pconst + (size_t)p * PC_STRIDE;
-  const RowC rc = load_row(pc, b);
+  RowC rc = load_row(pc, b);
+  row_geo(det, rc);  // used by the geocentric variants only
   LinksT L;
   const double* nb = nodes + ((size_t)p * modes.n * NNS + b.j) * 4;
   switch (det.kind | (det.duty ? 2 : 0)) {

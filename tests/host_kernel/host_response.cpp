@@ -56,7 +56,8 @@ int run(int argc, char** argv) {
     for (int k = 0; k < n; ++k) {
       BinPre b;
       bin_prepare(det, f[k], s[k], j[k], du, b);
-      const RowC rc = load_row(pcp, b);
+      RowC rc = load_row(pcp, b);
+      row_geo(det, rc);
       TdiC tdc;
       tdi_consts(det, b, tdc);
       Cx o[3];
